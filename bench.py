@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the wPFAW3D per-time-step hot path on one MI355X.
+
+Workload: the 256^3 wormholing configuration (16,777,216 cells, seeded
+Gaussian-random-field rock, inlet plane at 1e5 Pa, outlet plane at "p = 0",
+500-step run in the full configuration).  A step is one pass of the hot
+path: dt rule, permeability, pressure solve, velocity + phase-field +
+concentration + Euler update, with every input already resident in HBM.
+
+Two paths run from the same initial state:
+  plain   FP64 MG-preconditioned CG, rtol 1e-10       (the FP64 reference path)
+  fast    FP32 MG-PCG inside FP64 iterative refinement, rtol 1e-10 on the
+          FP64 true residual (the headline path)
+`value` is the fast path's time-steps/s; the plain path's rate, the ratio
+and the fast path's relative pressure error against the plain path after
+the timed steps are reported beside it.
+
+roofline: the dominant kernel of the fast path, timed with HIP events on the
+library's stream over the timed region; achieved = algorithmic bytes per
+launch / average launch time.
+cpu_baseline: the FP64 oracle (C restatement, OpenMP) timed on this host for
+one step of the same workload from the state the GPU timed region starts
+from, CG + Jacobi with rtol 1e-10 and maxit 1000 (the reference's cap).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-steps/s at 256³: learned+FP32-refined vs plain FP64 GPU, ≤1e-6 p err"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per cell (each array element the math must read or
+# write once per launch; mask = 1 byte)
+def bytes_per_cell(kernel: str, r: int) -> float:
+    return {
+        "smooth": 7 * r + 1,        # x, b, tx, ty, tz, invD, mask -> y
+        "matvec": 5 * r + 1,        # p, tx, ty, tz, mask -> q
+        "cg_update": 7 * r,         # x, p, r, q, invD -> x, r
+        "restrict": 5 * r + 1 + r / 8,
+        "prolong": 2 * r + 1 + r / 8,
+        "true_resid": 5 * 8 + 1 + r,  # x, t x3, invD (FP64), mask -> r (precision r)
+        "step_fields": 6 * 8 + 1 + 2 * 8,  # phi, c, p, t x3, mask -> phi', c'
+        "perm": 2 * 8 + 1 + 4 * 8,  # phi, krock, mask -> tx, ty, tz, invD
+    }[kernel]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def run_path(case, opts, warmup, steps, profile_kernel=None):
+    from mizuho_edit_amd.wpfaw3d import WPFAW3D
+    g = WPFAW3D(case, device=0, opts=opts)
+    for s in range(warmup):
+        rec = g.step()
+        log(f"  warmup {s + 1}/{warmup}: its={rec['iterations']} outer={rec['outer']} "
+            f"dt={rec['dtime']:.3e}")
+    state = {n: g.download(n) for n in ("pressure", "phase", "concent")}
+    state.update(itime=g.itime, time=g.time, dtime=g.dtime, taumin=g.taumin)
+    sync()
+    if profile_kernel:
+        g.profile_start(profile_kernel)
+    t0 = time.perf_counter()
+    its = 0
+    for s in range(steps):
+        rec = g.step()
+        its += rec["iterations"]
+        if (s + 1) % max(1, steps // 5) == 0:
+            log(f"  step {s + 1}/{steps}: its={rec['iterations']} outer={rec['outer']}")
+    sync()
+    wall = time.perf_counter() - t0
+    prof = g.profile_stop() if profile_kernel else None
+    return g, wall, its, state, prof
+
+
+def cpu_baseline(case, state, nthreads):
+    from oracle.oracle import OracleState
+    st = OracleState(case)
+    st.pressure[...] = state["pressure"]
+    st.phase[...] = state["phase"]
+    st.concent[...] = state["concent"]
+    st.itime, st.time, st.dtime, st.taumin = (state["itime"], state["time"], state["dtime"],
+                                             state["taumin"])
+    t0 = time.perf_counter()
+    st.run(1, rtol=1e-10, maxit=1000, nthreads=nthreads)
+    wall = time.perf_counter() - t0
+    return wall, int(st.hist[-1][2])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--kernel", default="smooth", help="kernel reported in roofline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs the hot path on one GPU only; multi-GPU runs are "
+              "not part of it (see DESIGN.md for how the grid would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from mizuho_edit_amd import io
+    from mizuho_edit_amd.wpfaw3d import solver_opts
+
+    n = a.grid
+    log(f"building the {n}^3 workload (seed {a.seed})")
+    case = io.workload("plain-256", seed=a.seed, grid=(n, n, n))
+    ncell = n ** 3
+
+    log("plain FP64 path")
+    g64, wall64, its64, _, _ = run_path(case, solver_opts("mgcg", rtol=1e-10), a.warmup, a.steps)
+    p64 = g64.download("pressure")
+    g64.close()
+
+    log("headline path (FP32 MG-PCG + FP64 refinement)")
+    fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4)
+    gf, wallf, itsf, state0, prof = run_path(case, fast_opts, a.warmup, a.steps, a.kernel)
+    pf = gf.download("pressure")
+    gf.close()
+    perr = float(np.abs(pf - p64).max() / np.abs(p64).max())
+
+    sps = a.steps / wallf
+    sps64 = a.steps / wall64
+    total_ms, count = prof
+    avg_s = total_ms / 1e3 / max(count, 1)
+    bpc = bytes_per_cell(a.kernel, 4 if a.kernel in ("smooth", "matvec", "cg_update", "restrict",
+                                                      "prolong", "true_resid") else 8)
+    achieved = bpc * ncell / avg_s / 1e9 if count else 0.0
+
+    cpu = None
+    if not a.no_cpu:
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        log(f"CPU oracle: one step at {n}^3 on {nthreads} threads")
+        cwall, cits = cpu_baseline(case, state0, nthreads)
+        cpu = {"value": 1.0 / cwall, "unit": "time-steps/s", "cores": nthreads, "kind": "port",
+               "sample": f"one time step of the {n}^3 workload from the GPU timed region's start "
+                         f"state (FP64 C restatement, CG+Jacobi rtol 1e-10 maxit 1000, "
+                         f"{cits} iterations, {cwall:.1f} s)"}
+
+    out = {
+        "metric": METRIC,
+        "value": sps,
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wallf / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 stencils + f32 solve in f64 refinement",
+        "data": "synthetic: seeded Gaussian random field rock (corr 4 cells), Kozeny-Carman k",
+        "config": {"workload": f"wormholing {n}^3, steps {a.warmup + 1}..{a.warmup + a.steps} "
+                               f"of the 500-step run", "grid": [n, n, n],
+                   "solver": "FP32 MG-PCG (5 levels, 2 Jacobi sweeps) in FP64 refinement, rtol 1e-10",
+                   "parallelism": "1 GPU"},
+        "plain_fp64_steps_per_s": sps64,
+        "speedup_vs_plain_fp64": sps / sps64,
+        "p_rel_err_vs_plain_fp64": perr,
+        "cell_updates_per_s": sps * ncell,
+        "iterations_per_step": {"plain_fp64": its64 / a.steps, "fast_fp32_inner": itsf / a.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": a.kernel, "launches": count, "avg_us": avg_s * 1e6,
+                     "bytes_per_cell": bpc},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,217 @@
+/*
+ * wpf.h -- C ABI of the MI355X (gfx950) wPFAW3D per-time-step hot path.
+ *
+ * One context owns one grid resident in HBM.  The host keeps the reference
+ * module's view of the state: haloed Fortran-order arrays (index 0..n+1 on
+ * every axis, x fastest), exactly as allocated in wPFAW3D.f90:440-449.  The
+ * library stores interior cells only, structure-of-arrays, and applies the
+ * single-rank ghost rule ghost(i,j,k) = field(clamp(i),clamp(j),clamp(k))
+ * (wPFAW3D.f90:1645-1657 after parallel.f90:278-293) inside its kernels.
+ *
+ * Every entry point returns a status code (WPF_OK == 0).  The reference has
+ * no status codes: it stops on a KSPSolve error (wPFAW3D.f90:1005) and never
+ * checks convergence; here non-convergence and NaN are reported.
+ *
+ * Threading: one host thread per context; calls on one context are ordered
+ * on that context's single HIP stream.  The caller owns host memory, the
+ * library owns device memory.
+ */
+#ifndef WPF_H
+#define WPF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WPF_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+#define WPF_OK          0
+#define WPF_ERR_ARG     1   /* bad argument or size mismatch               */
+#define WPF_ERR_HIP     2   /* HIP runtime error                            */
+#define WPF_ERR_NOCONV  3   /* pressure solve hit maxit before tolerance    */
+#define WPF_ERR_NAN     4   /* NaN or Inf in a solver residual              */
+#define WPF_ERR_NODEV   5   /* no GPU visible                               */
+#define WPF_ERR_STATE   6   /* call order (e.g. solve before permeability)  */
+#define WPF_ERR_NOMEM   7   /* device allocation failed                     */
+#define WPF_ERR_IO      8   /* file missing or malformed                    */
+
+/* ---- field identifiers (haloed, Fortran order) ------------------------- */
+#define WPF_FIELD_PERMEAB_ROCK 0  /* permeab_rock(0:nx+1,0:ny+1,0:nz+1)      */
+#define WPF_FIELD_PRESSURE     1  /* pressure                                 */
+#define WPF_FIELD_PHASE        2  /* phase                                    */
+#define WPF_FIELD_CONCENT      3  /* concent                                  */
+#define WPF_FIELD_DPHASEDT     4  /* dphasedt                                 */
+#define WPF_FIELD_DCONCENTDT   5  /* dconcentdt                               */
+#define WPF_FIELD_PERMEAB      6  /* permeab(0:3,...), download only          */
+#define WPF_FIELD_VELOCITY     7  /* velocity(1:3,...), download only         */
+
+/* The subset of Param_type (wPFAW3D.f90:27-76) that the hot path reads.
+ * Global and local sizes coincide: one GPU holds the whole grid. */
+typedef struct wpf_params {
+    int32_t nx, ny, nz;             /* nx_g, ny_g, nz_g                      */
+    int32_t ix_ref, iy_ref, iz_ref; /* reference cell (1-based)              */
+    double dtime;                   /* dt of steps 1-2 (cfg "dt")            */
+    double tfac;                    /* dt factor (cfg "tfac")                */
+    double epsilon;                 /* interface width [m] (cfg "w")         */
+    double dx;                      /* grid spacing [m] = dxw*w              */
+    double lambda;                  /* phi-c coupling                        */
+    double porosity;
+    double rhos, rhoa;
+    double viscosity;               /* cfg "visco"                           */
+    double cwtp;
+    double diffusion;               /* cfg "D"                               */
+    double b100;
+    double ral, rat;                /* dispersivities                        */
+    double permeab_hole;            /* cfg "kwh"                             */
+    double alpha;                   /* 0.5/b/kc (wPFAW3D.f90:397-399)        */
+    double pini;
+} wpf_params;
+
+/* Well_type (wPFAW3D.f90:82-90); indices are 1-based global cell indices. */
+typedef struct wpf_well {
+    int32_t ix_g, iy_g, iz_g, pad;
+    double pressure;   /* Dirichlet value; a well is Dirichlet iff != 0      */
+    double injection;  /* injection rate; an inlet iff != 0                  */
+    double phase;
+    double concent;
+} wpf_well;
+
+/* ---- pressure solver -------------------------------------------------- */
+#define WPF_SOLVER_CG_JACOBI 0  /* FP64 matrix-free CG + Jacobi              */
+#define WPF_SOLVER_MGCG      1  /* FP64 CG + geometric-multigrid V-cycle     */
+#define WPF_SOLVER_MIXED     2  /* FP32 MG-PCG inside FP64 refinement        */
+
+#define WPF_FLAG_LEARNED_WARM  1  /* learned warm start (needs a net)        */
+#define WPF_FLAG_LEARNED_FCG   2  /* learned stage inside flexible CG        */
+#define WPF_FLAG_EXTRAPOLATE   4  /* linear-in-time initial guess            */
+
+typedef struct wpf_solver_opts {
+    int32_t solver;       /* WPF_SOLVER_*                                    */
+    int32_t maxit;        /* Krylov iteration cap (reference: 1000)          */
+    double  rtol;         /* reference: 1e-10                                */
+    double  atol;         /* PETSc default 1e-50                             */
+    int32_t mg_levels;    /* multigrid levels including the fine one (5)     */
+    int32_t mg_sweeps;    /* damped-Jacobi sweeps before and after (2)       */
+    double  mg_omega;     /* Jacobi damping                                  */
+    double  mg_coarse_scale; /* weight of the coarse-grid correction         */
+    double  inner_rtol;   /* mixed: FP32 inner reduction per refinement step */
+    int32_t inner_maxit;  /* mixed: FP32 iterations cap per refinement step  */
+    int32_t flags;        /* WPF_FLAG_*                                      */
+} wpf_solver_opts;
+
+/* Residual norms are Jacobi-preconditioned 2-norms of the reference's own
+ * system (Dirichlet rows included), the norm PETSc's default test uses for
+ * a left-preconditioned Krylov method: converged when
+ * rnorm <= max(rtol*bnorm, atol). */
+typedef struct wpf_solve_stats {
+    int32_t iterations;   /* Krylov iterations (FP32 inner total if mixed)   */
+    int32_t outer;        /* refinement steps (mixed), 0 otherwise           */
+    int32_t converged;    /* 1 when the test above holds on the true residual*/
+    int32_t learned;      /* 1 when a learned stage was accepted             */
+    double  rnorm0;       /* initial true residual                           */
+    double  rnorm;        /* final true residual                             */
+    double  bnorm;        /* right-hand side                                  */
+    double  seconds;      /* wall time of the solve (host clock)             */
+} wpf_solve_stats;
+
+/* showProgress reductions (wPFAW3D.f90:1695-1810) */
+typedef struct wpf_diag {
+    double pv;            /* dx^3*poro*0.5*sum(1-phase)   (1:1695)           */
+    double vcal;          /* dx^3*(1-poro)*0.5*sum(1+phase)  (1:1776)        */
+    double rwhmax;        /* max distance of phase>0 cells from ref cell     */
+    double phase_max, phase_min, concent_max, concent_min; /* haloed arrays  */
+    double pinj;          /* pressure at the reference cell                  */
+} wpf_diag;
+
+typedef struct wpf_ctx wpf_ctx;
+
+/* ---- lifecycle --------------------------------------------------------- */
+/* Build a context: allocate the device fields and derive the per-cell type
+ * mask from lw(0:nx+1,0:ny+1,0:nz+1) and the well table, the state that
+ * loadConfig builds (wPFAW3D.f90:451-461).  Fields start as the reference
+ * initialises them (1:463-471) with pressure = pini.  dev < 0 keeps the
+ * current device. */
+int wpf_create(const wpf_params* p, const int32_t* lw_haloed,
+               const wpf_well* wells, int32_t nwell, int32_t dev,
+               wpf_ctx** out);
+int wpf_destroy(wpf_ctx* ctx);
+int wpf_device_count(int32_t* n);
+const char* wpf_status_string(int32_t status);
+/* Text of the last error on this thread (empty if none). */
+const char* wpf_last_error(void);
+
+/* ---- host <-> device ----------------------------------------------------- */
+/* Copy a haloed host field in.  Only interior cells are read. */
+int wpf_upload(wpf_ctx* ctx, int32_t field, const double* host_haloed);
+/* Copy a field out, halos filled as updateHalo leaves them. */
+int wpf_download(wpf_ctx* ctx, int32_t field, double* host_haloed);
+
+/* ---- the reference's module procedures ---------------------------------- */
+/* calcPermeability, wPFAW3D.f90:561-599: node permeability and harmonic
+ * face means; also prepares the matrix-free pressure operator. */
+int wpf_calc_permeability(wpf_ctx* ctx);
+/* solvePressure, wPFAW3D.f90:603-1160 (assembly 630-936, KSPSolve
+ * 989-1005, scatter 1016-1050): matrix-free, same linear system. */
+int wpf_solve_pressure(wpf_ctx* ctx, const wpf_solver_opts* opts,
+                       wpf_solve_stats* stats);
+/* calcVelocity, wPFAW3D.f90:1164-1193. */
+int wpf_calc_velocity(wpf_ctx* ctx);
+/* calcPhaseDeriv(taumin), wPFAW3D.f90:1197-1475 including the MPI_MIN
+ * reduction of taumin at 1:1439-1440. */
+int wpf_calc_phase_deriv(wpf_ctx* ctx, double* taumin);
+/* calcConcentDeriv, wPFAW3D.f90:1554-1624. */
+int wpf_calc_concent_deriv(wpf_ctx* ctx);
+/* evolvePhase / evolveConcent, wPFAW3D.f90:1628-1640 (forward Euler). */
+int wpf_evolve_phase(wpf_ctx* ctx, double dtime);
+int wpf_evolve_concent(wpf_ctx* ctx, double dtime);
+/* updateHalo, wPFAW3D.f90:1645-1657.  Ghosts are clamped inside every
+ * kernel, so this only validates the field id. */
+int wpf_update_halo(wpf_ctx* ctx, int32_t field);
+/* getPressure, wPFAW3D.f90:2151-2166 (1-based global indices). */
+int wpf_get_pressure(wpf_ctx* ctx, int32_t ix_g, int32_t iy_g, int32_t iz_g,
+                     double* p);
+/* showProgress reductions, wPFAW3D.f90:1695-1810. */
+int wpf_diagnostics(wpf_ctx* ctx, wpf_diag* d);
+
+/* ---- fused time step ----------------------------------------------------- */
+/* One pass of the loop body of wPFAW3D.f90:2225-2294: the dt rule
+ * (2227-2230), calcPermeability, solvePressure, then velocity, phase and
+ * concentration derivatives and both Euler updates fused in one kernel.
+ * itime is the 1-based step number; *time and *dtime are updated as the
+ * reference updates them; *taumin carries the previous step's value in and
+ * this step's value out. */
+int wpf_step(wpf_ctx* ctx, const wpf_solver_opts* opts, int32_t itime,
+             double* time, double* dtime, double* taumin,
+             wpf_solve_stats* stats);
+
+/* Breakthrough: first step at which a non-well interior cell in the plane
+ * ix_g = nx-1 (next to the outlet plane) holds phase > 0.  Returns 0 while
+ * it has not happened.  Checked by wpf_step after every Euler update. */
+int wpf_breakthrough_step(wpf_ctx* ctx, int32_t* step);
+/* Move the breakthrough plane to ix_g (1-based) and clear the recorded step.
+ * The reference has no breakthrough event; this is the workload's own
+ * definition, applied identically to the oracle. */
+int wpf_set_breakthrough_plane(wpf_ctx* ctx, int32_t ix_g);
+
+/* ---- measurement ---------------------------------------------------------
+ * Time every launch of one kernel with HIP events on the context's stream
+ * between wpf_profile_start and wpf_profile_stop; *total_ms is the summed
+ * kernel time and *count the number of launches timed. */
+#define WPF_KERNEL_PERM        0  /* k_perm: permeability + operator        */
+#define WPF_KERNEL_STEP_FIELDS 1  /* k_step_fields: fused stencil update    */
+#define WPF_KERNEL_SMOOTH      2  /* fine-level damped-Jacobi sweep         */
+#define WPF_KERNEL_MATVEC      3  /* fine-level matvec + dot                */
+#define WPF_KERNEL_TRUE_RESID  4  /* FP64 true residual                     */
+#define WPF_KERNEL_RESTRICT    5  /* fine-level residual + restriction      */
+#define WPF_KERNEL_PROLONG     6  /* fine-level prolongation                */
+#define WPF_KERNEL_CG_UPDATE   7  /* fine-level x/r update                  */
+int wpf_profile_start(wpf_ctx* ctx, int32_t kernel);
+int wpf_profile_stop(wpf_ctx* ctx, double* total_ms, int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WPF_H */

@@ -1,0 +1,579 @@
+// k_fields.hip -- stencil kernels of the wPFAW3D time step on gfx950.
+//
+//   k_perm         calcPermeability + face coefficients t = k_face/mu and the
+//                  Jacobi diagonal of the pressure operator
+//                  (wPFAW3D.f90:561-599, 828-849)
+//   k_velocity     calcVelocity (wPFAW3D.f90:1164-1193)
+//   k_phase_deriv  calcPhaseDeriv + taumin MIN reduction (1197-1440)
+//   k_concent_deriv calcConcentDeriv (1554-1624)
+//   k_evolve       evolvePhase / evolveConcent (1628-1640)
+//   k_step_fields  velocity + phase + concentration + both Euler updates +
+//                  taumin + breakthrough test, fused: one read of phi, c, p,
+//                  t and the mask, one write of phi' and c'
+//   k_diag         showProgress reductions (1695-1810)
+//
+// Ghost cells are never stored: an index that leaves the grid is clamped,
+// which is exactly what updateHalo leaves in the halo on one rank.
+#include "wpf_internal.h"
+#include "phase_cell.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace wpf {
+
+static PhaseConsts make_consts(const wpf_params& p)
+{
+    PhaseConsts K;
+    K.dx = p.dx;
+    K.two_dx = 2.0 * p.dx;
+    K.dx2 = p.dx * p.dx;
+    K.four_dx2 = 4.0 * (p.dx * p.dx);
+    K.w0 = 2.0 * p.dx;
+    K.maxdphi = 2.0 / (2.0 * p.dx);
+    K.eps = p.epsilon;
+    K.lambda = p.lambda;
+    K.diffusion = p.diffusion;
+    K.ral = p.ral;
+    K.rat = p.rat;
+    K.tau_num = 5.0 / 3.0 * p.alpha * p.lambda * (p.epsilon * p.epsilon);
+    K.alpha = p.alpha;
+    K.kwh = p.permeab_hole;
+    K.inv_mu_unused = 0.0;
+    return K;
+}
+
+struct Dims {
+    int nx, ny, nz;
+    int64_t plane;
+};
+
+static inline Dims dims_of(const wpf_ctx* c) { return Dims{c->g.nx, c->g.ny, c->g.nz, c->g.plane}; }
+
+static inline dim3 grid3(const Dims& d, dim3 blk)
+{
+    return dim3((d.nx + blk.x - 1) / blk.x, (d.ny + blk.y - 1) / blk.y, (d.nz + blk.z - 1) / blk.z);
+}
+
+__device__ __forceinline__ double kcell(double kwh, double phi, double kr)
+{
+    // permeab(0) = kwh*f + k_rock*(1-f), f = (1+phase)/2   (1:567-574)
+    const double f = 0.5 * (1.0 + phi);
+    return kwh * f + kr * (1.0 - f);
+}
+__device__ __forceinline__ double hmean(double a, double b) { return 2.0 / (1.0 / a + 1.0 / b); }
+
+// ---------------------------------------------------------------------------
+// calcPermeability + operator set-up.  Each thread rebuilds the six faces of
+// its cell (the -faces exactly as the neighbour computes them, same operand
+// order) so the diagonal comes out of the same pass.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_perm(Dims d, double kwh, double inv_mu_dummy, double mu,
+                                              const double* __restrict__ phi,
+                                              const double* __restrict__ kr,
+                                              const uint8_t* __restrict__ mask,
+                                              double* __restrict__ tx, double* __restrict__ ty,
+                                              double* __restrict__ tz, double* __restrict__ invD)
+{
+    (void)inv_mu_dummy;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    if (i >= d.nx || j >= d.ny || k >= d.nz) return;
+    const int64_t c = i + (int64_t)d.nx * (j + (int64_t)d.ny * k);
+    const int ip = min(i + 1, d.nx - 1), im = max(i - 1, 0);
+    const int jp = min(j + 1, d.ny - 1), jm = max(j - 1, 0);
+    const int kp = min(k + 1, d.nz - 1), km = max(k - 1, 0);
+    const int64_t row = (int64_t)d.nx * (j + (int64_t)d.ny * k);
+    auto at = [&](int ii, int jj, int kk) { return ii + (int64_t)d.nx * (jj + (int64_t)d.ny * kk); };
+    const double k0 = kcell(kwh, phi[c], kr[c]);
+    const int64_t cxp = row + ip, cxm = row + im;
+    const int64_t cyp = at(i, jp, k), cym = at(i, jm, k);
+    const int64_t czp = at(i, j, kp), czm = at(i, j, km);
+    const double kxp = kcell(kwh, phi[cxp], kr[cxp]);
+    const double kxm = kcell(kwh, phi[cxm], kr[cxm]);
+    const double kyp = kcell(kwh, phi[cyp], kr[cyp]);
+    const double kym = kcell(kwh, phi[cym], kr[cym]);
+    const double kzp = kcell(kwh, phi[czp], kr[czp]);
+    const double kzm = kcell(kwh, phi[czm], kr[czm]);
+    // t = permeab(face)/viscosity, 0 on the global edge   (1:843-848)
+    const double txp = (i < d.nx - 1) ? hmean(k0, kxp) / mu : 0.0;
+    const double txm = (i > 0) ? hmean(kxm, k0) / mu : 0.0;
+    const double typ = (j < d.ny - 1) ? hmean(k0, kyp) / mu : 0.0;
+    const double tym = (j > 0) ? hmean(kym, k0) / mu : 0.0;
+    const double tzp = (k < d.nz - 1) ? hmean(k0, kzp) / mu : 0.0;
+    const double tzm = (k > 0) ? hmean(kzm, k0) / mu : 0.0;
+    tx[c] = txp;
+    ty[c] = typ;
+    tz[c] = tzp;
+    const double dsum = txp + txm + typ + tym + tzp + tzm;  // -tc   (1:849)
+    const bool fixed = (mask[c] & WPF_MASK_FIXED) != 0;
+    invD[c] = (fixed || dsum == 0.0) ? 0.0 : 1.0 / dsum;
+}
+
+int launch_perm(wpf_ctx* c)
+{
+    const Dims d = dims_of(c);
+    dim3 blk(64, 4, 1);
+    prof_mark(c, WPF_KERNEL_PERM, true);
+    hipLaunchKernelGGL(k_perm, grid3(d, blk), blk, 0, c->stream, d, c->prm.permeab_hole, 0.0,
+                       c->prm.viscosity, c->phi[c->cur], c->krock, c->mask, c->tx.p, c->ty.p,
+                       c->tz.p, c->invD.p);
+    prof_mark(c, WPF_KERNEL_PERM, false);
+    WPF_HIP(hipGetLastError());
+    c->have_perm = true;
+    return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// calcVelocity: v = -(t * ((p(+) - p)/dx)) on the face (i+1/2).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double face_v(double t, double pp, double p0, double dx)
+{
+    return -(t * ((pp - p0) / dx));
+}
+
+__global__ void __launch_bounds__(256) k_velocity(Dims d, double dx, const double* __restrict__ p,
+                                                  const double* __restrict__ tx,
+                                                  const double* __restrict__ ty,
+                                                  const double* __restrict__ tz,
+                                                  double* __restrict__ vx, double* __restrict__ vy,
+                                                  double* __restrict__ vz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    if (i >= d.nx || j >= d.ny || k >= d.nz) return;
+    const int64_t c = i + (int64_t)d.nx * (j + (int64_t)d.ny * k);
+    const int64_t cxp = (i < d.nx - 1) ? c + 1 : c;
+    const int64_t cyp = (j < d.ny - 1) ? c + d.nx : c;
+    const int64_t czp = (k < d.nz - 1) ? c + d.plane : c;
+    const double p0 = p[c];
+    vx[c] = face_v(tx[c], p[cxp], p0, dx);
+    vy[c] = face_v(ty[c], p[cyp], p0, dx);
+    vz[c] = face_v(tz[c], p[czp], p0, dx);
+}
+
+static int ensure_velocity(wpf_ctx* c)
+{
+    if (c->vx) return WPF_OK;
+    size_t b = sizeof(double) * (size_t)c->g.n;
+    WPF_HIP(hipMalloc((void**)&c->vx, b));
+    WPF_HIP(hipMalloc((void**)&c->vy, b));
+    WPF_HIP(hipMalloc((void**)&c->vz, b));
+    WPF_HIP(hipMemsetAsync(c->vx, 0, b, c->stream));
+    WPF_HIP(hipMemsetAsync(c->vy, 0, b, c->stream));
+    WPF_HIP(hipMemsetAsync(c->vz, 0, b, c->stream));
+    return WPF_OK;
+}
+
+int launch_velocity(wpf_ctx* c)
+{
+    WPF_CHECK(c->have_perm, WPF_ERR_STATE, "calcVelocity before calcPermeability");
+    int rc = ensure_velocity(c);
+    if (rc) return rc;
+    const Dims d = dims_of(c);
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_velocity, grid3(d, blk), blk, 0, c->stream, d, c->prm.dx, c->p.p, c->tx.p,
+                       c->ty.p, c->tz.p, c->vx, c->vy, c->vz);
+    WPF_HIP(hipGetLastError());
+    c->have_velocity = true;
+    return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// shared stencil gathers
+// ---------------------------------------------------------------------------
+struct Nb {
+    int64_t c, xp, xm, yp, ym, zp, zm;
+    int ip, im, jp, jm, kp, km;
+};
+
+__device__ __forceinline__ Nb neighbours(const Dims& d, int i, int j, int k)
+{
+    Nb n;
+    n.ip = min(i + 1, d.nx - 1); n.im = max(i - 1, 0);
+    n.jp = min(j + 1, d.ny - 1); n.jm = max(j - 1, 0);
+    n.kp = min(k + 1, d.nz - 1); n.km = max(k - 1, 0);
+    const int64_t nx = d.nx, ny = d.ny;
+    n.c = i + nx * (j + ny * k);
+    n.xp = n.ip + nx * (j + ny * k);
+    n.xm = n.im + nx * (j + ny * k);
+    n.yp = i + nx * (n.jp + ny * k);
+    n.ym = i + nx * (n.jm + ny * k);
+    n.zp = i + nx * (j + ny * n.kp);
+    n.zm = i + nx * (j + ny * n.km);
+    return n;
+}
+
+__device__ __forceinline__ void gather19(const Dims& d, const double* __restrict__ f, int i, int j,
+                                         int k, const Nb& n, double* s)
+{
+    const int64_t nx = d.nx, ny = d.ny;
+    auto at = [&](int ii, int jj, int kk) { return f[ii + nx * (jj + ny * kk)]; };
+    s[0] = f[n.c];
+    s[1] = f[n.xp]; s[2] = f[n.xm];
+    s[3] = f[n.yp]; s[4] = f[n.ym];
+    s[5] = f[n.zp]; s[6] = f[n.zm];
+    s[7] = at(n.ip, n.jp, k); s[8] = at(n.ip, n.jm, k);
+    s[9] = at(n.im, n.jp, k); s[10] = at(n.im, n.jm, k);
+    s[11] = at(i, n.jp, n.kp); s[12] = at(i, n.jm, n.kp);
+    s[13] = at(i, n.jp, n.km); s[14] = at(i, n.jm, n.km);
+    s[15] = at(n.ip, j, n.kp); s[16] = at(n.im, j, n.kp);
+    s[17] = at(n.ip, j, n.km); s[18] = at(n.im, j, n.km);
+}
+
+// block-level min of tau, one atomicMin on the ordered key per block
+__device__ __forceinline__ void block_min_key(double v, unsigned long long* key)
+{
+    __shared__ double sh[16];
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthr = blockDim.x * blockDim.y * blockDim.z;
+    v = wave_min(v);
+    if ((tid & 63) == 0) sh[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double m = sh[0];
+        for (int w = 1; w < (nthr >> 6); w++) m = fmin(m, sh[w]);
+        atomicMin(key, dkey(m));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// calcPhaseDeriv (per-procedure form, velocity read from vx/vy/vz)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_phase_deriv(Dims d, PhaseConsts K,
+                                                     const double* __restrict__ phi,
+                                                     const double* __restrict__ conc,
+                                                     const double* __restrict__ vx,
+                                                     const double* __restrict__ vy,
+                                                     const double* __restrict__ vz,
+                                                     const uint8_t* __restrict__ mask,
+                                                     double* __restrict__ dphidt,
+                                                     unsigned long long* tau_key)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    double tau = 1.0e6;
+    if (i < d.nx && j < d.ny && k < d.nz) {
+        const Nb n = neighbours(d, i, j, k);
+        double s[19];
+        gather19(d, phi, i, j, k, n, s);
+        // the face below the first cell of an axis carries -0 (zero gradient)
+        const double vxm = (i > 0) ? vx[n.c - 1] : -0.0;
+        const double vym = (j > 0) ? vy[n.c - d.nx] : -0.0;
+        const double vzm = (k > 0) ? vz[n.c - d.plane] : -0.0;
+        const bool well = (mask[n.c] & WPF_MASK_WELL) != 0;
+        dphidt[n.c] = phase_cell(K, s, conc[n.c], vx[n.c], vxm, vy[n.c], vym, vz[n.c], vzm, well, &tau);
+    }
+    block_min_key(tau, tau_key);
+}
+
+int launch_phase_deriv(wpf_ctx* c, double* taumin)
+{
+    WPF_CHECK(c->have_velocity, WPF_ERR_STATE, "calcPhaseDeriv before calcVelocity");
+    const Dims d = dims_of(c);
+    const PhaseConsts K = make_consts(c->prm);
+    // taumin starts at 1e6 (1:1223)
+    double init = 1.0e6;
+    unsigned long long key;
+    {
+        unsigned long long u;
+        memcpy(&u, &init, 8);
+        key = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+    }
+    WPF_HIP(hipMemcpyAsync(c->tau_key + 3, &key, 8, hipMemcpyHostToDevice, c->stream));
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_phase_deriv, grid3(d, blk), blk, 0, c->stream, d, K, c->phi[c->cur],
+                       c->conc[c->cur], c->vx, c->vy, c->vz, c->mask, c->dphidt, c->tau_key + 3);
+    WPF_HIP(hipGetLastError());
+    unsigned long long out;
+    WPF_HIP(hipMemcpyAsync(&out, c->tau_key + 3, 8, hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipStreamSynchronize(c->stream));
+    unsigned long long u = (out & 0x8000000000000000ull) ? (out & 0x7fffffffffffffffull) : ~out;
+    memcpy(taumin, &u, 8);
+    return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// calcConcentDeriv (per-procedure form)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_concent_deriv(Dims d, PhaseConsts K,
+                                                       const double* __restrict__ conc,
+                                                       const double* __restrict__ vx,
+                                                       const double* __restrict__ vy,
+                                                       const double* __restrict__ vz,
+                                                       const double* __restrict__ dphidt,
+                                                       const uint8_t* __restrict__ mask,
+                                                       double* __restrict__ dcdt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    if (i >= d.nx || j >= d.ny || k >= d.nz) return;
+    const Nb n = neighbours(d, i, j, k);
+    if (mask[n.c] & WPF_MASK_WELL) {
+        dcdt[n.c] = 0.0;
+        return;
+    }
+    double cc[7] = {conc[n.c], conc[n.xp], conc[n.xm], conc[n.yp], conc[n.ym], conc[n.zp], conc[n.zm]};
+    const double vxm = (i > 0) ? vx[n.c - 1] : -0.0;
+    const double vym = (j > 0) ? vy[n.c - d.nx] : -0.0;
+    const double vzm = (k > 0) ? vz[n.c - d.plane] : -0.0;
+    dcdt[n.c] = concent_cell(K, cc, dphidt[n.c], vx[n.c], vxm, vy[n.c], vym, vz[n.c], vzm);
+}
+
+int launch_concent_deriv(wpf_ctx* c)
+{
+    WPF_CHECK(c->have_velocity, WPF_ERR_STATE, "calcConcentDeriv before calcVelocity");
+    const Dims d = dims_of(c);
+    const PhaseConsts K = make_consts(c->prm);
+    dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(k_concent_deriv, grid3(d, blk), blk, 0, c->stream, d, K, c->conc[c->cur],
+                       c->vx, c->vy, c->vz, c->dphidt, c->mask, c->dcdt);
+    WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Euler update (interior; halo derivatives are 0 so halos only follow the
+// clamp, which the kernels apply on read)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_evolve(int64_t n, double dt, double* __restrict__ f,
+                                                const double* __restrict__ df)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) f[c] = f[c] + df[c] * dt;
+}
+
+int launch_evolve(wpf_ctx* c, int which, double dtime)
+{
+    const int64_t n = c->g.n;
+    double* f = which == 0 ? c->phi[c->cur] : c->conc[c->cur];
+    const double* df = which == 0 ? c->dphidt : c->dcdt;
+    hipLaunchKernelGGL(k_evolve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       dtime, f, df);
+    WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused: velocity (from p and t) + calcPhaseDeriv + calcConcentDeriv + both
+// Euler updates + taumin + breakthrough test.  dt follows the rule of
+// wPFAW3D.f90:2227-2230 from the previous step's taumin, decoded on device,
+// so a sequence of steps needs no host round trip.
+// ---------------------------------------------------------------------------
+struct StepScalars {
+    double dt_pref;    // tfac*0.5*dx^2/eps^2
+    double eps2_D;     // eps^2/D
+    double dt_cfg;     // dt of steps 1-2 (or the host's dt when device_dt = 0)
+    int itime;
+    int device_dt;
+    int bt_plane;
+    int key_prev, key_cur, key_next;
+};
+
+__global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConsts K, StepScalars S,
+                                                     const double* __restrict__ phi,
+                                                     const double* __restrict__ conc,
+                                                     const double* __restrict__ p,
+                                                     const double* __restrict__ tx,
+                                                     const double* __restrict__ ty,
+                                                     const double* __restrict__ tz,
+                                                     const uint8_t* __restrict__ mask,
+                                                     double* __restrict__ phi_out,
+                                                     double* __restrict__ conc_out,
+                                                     unsigned long long* __restrict__ keys,
+                                                     double* __restrict__ scal,
+                                                     int32_t* __restrict__ bt_step)
+{
+    double dt = S.dt_cfg;
+    if (S.device_dt && S.itime >= 3) {
+        const double taumin_prev = dkey_inv(keys[S.key_prev]);
+        dt = S.dt_pref * fmin(S.eps2_D, taumin_prev);
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    double tau = 1.0e6;
+    if (i < d.nx && j < d.ny && k < d.nz) {
+        const Nb n = neighbours(d, i, j, k);
+        const double dx = K.dx;
+        const double p0 = p[n.c];
+        // face velocities of the six faces (the -face of the first cell of an
+        // axis is the zero-gradient boundary face: -(t*0) = -0)
+        const double vxp = face_v(tx[n.c], p[n.xp], p0, dx);
+        const double vyp = face_v(ty[n.c], p[n.yp], p0, dx);
+        const double vzp = face_v(tz[n.c], p[n.zp], p0, dx);
+        const double vxm = (i > 0) ? face_v(tx[n.xm], p0, p[n.xm], dx) : -0.0;
+        const double vym = (j > 0) ? face_v(ty[n.ym], p0, p[n.ym], dx) : -0.0;
+        const double vzm = (k > 0) ? face_v(tz[n.zm], p0, p[n.zm], dx) : -0.0;
+        double s[19];
+        gather19(d, phi, i, j, k, n, s);
+        const bool well = (mask[n.c] & WPF_MASK_WELL) != 0;
+        const double c0 = conc[n.c];
+        const double dph = phase_cell(K, s, c0, vxp, vxm, vyp, vym, vzp, vzm, well, &tau);
+        double dcd = 0.0;
+        if (!well) {
+            double cc[7] = {c0, conc[n.xp], conc[n.xm], conc[n.yp], conc[n.ym], conc[n.zp], conc[n.zm]};
+            dcd = concent_cell(K, cc, dph, vxp, vxm, vyp, vym, vzp, vzm);
+        }
+        const double phn = s[0] + dph * dt;
+        phi_out[n.c] = phn;
+        conc_out[n.c] = c0 + dcd * dt;
+        if (i == S.bt_plane && !well && phn > 0.0) atomicMin(bt_step, S.itime);
+    }
+    block_min_key(tau, &keys[S.key_cur]);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
+        threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
+        // key_next was last read two steps ago: reset it for the next step
+        keys[S.key_next] = dkey(1.0e6);
+        scal[0] = scal[0] + dt;  // time
+        scal[1] = dt;            // dtime
+    }
+}
+
+int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt)
+{
+    const Dims d = dims_of(c);
+    const PhaseConsts K = make_consts(c->prm);
+    StepScalars S;
+    const wpf_params& p = c->prm;
+    S.dt_pref = p.tfac * 0.5 * (p.dx * p.dx) / (p.epsilon * p.epsilon);
+    S.eps2_D = (p.epsilon * p.epsilon) / p.diffusion;
+    S.dt_cfg = dtime_host;
+    S.itime = itime;
+    S.device_dt = device_dt ? 1 : 0;
+    S.bt_plane = c->bt_plane;
+    S.key_prev = (itime + 2) % 3;
+    S.key_cur = itime % 3;
+    S.key_next = (itime + 1) % 3;
+    dim3 blk(64, 4, 1);
+    prof_mark(c, WPF_KERNEL_STEP_FIELDS, true);
+    hipLaunchKernelGGL(k_step_fields, grid3(d, blk), blk, 0, c->stream, d, K, S, c->phi[c->cur],
+                       c->conc[c->cur], c->p.p, c->tx.p, c->ty.p, c->tz.p, c->mask,
+                       c->phi[1 - c->cur], c->conc[1 - c->cur], c->tau_key, c->scal, c->bt_step);
+    prof_mark(c, WPF_KERNEL_STEP_FIELDS, false);
+    WPF_HIP(hipGetLastError());
+    c->cur = 1 - c->cur;
+    return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// showProgress reductions: sums of (1-phase) and (1+phase), max r^2 of
+// phase > 0 cells, min/max of phase and concent.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_diag(Dims d, double dx, double xref, double yref,
+                                              double zref, const double* __restrict__ phi,
+                                              const double* __restrict__ conc,
+                                              double* __restrict__ partials,
+                                              unsigned* __restrict__ counter,
+                                              double* __restrict__ result,
+                                              unsigned long long* __restrict__ keys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    double v[2] = {0.0, 0.0};
+    double r2 = 0.0, pmx = -1.0e300, pmn = 1.0e300, cmx = -1.0e300, cmn = 1.0e300;
+    if (i < d.nx && j < d.ny && k < d.nz) {
+        const int64_t c = i + (int64_t)d.nx * (j + (int64_t)d.ny * k);
+        const double ph = phi[c], cc = conc[c];
+        v[0] = 1.0 - ph;
+        v[1] = 1.0 + ph;
+        if (ph > 0.0) {
+            const double x = dx * i + 0.5 * dx, y = dx * j + 0.5 * dx, z = dx * k + 0.5 * dx;
+            r2 = (x - xref) * (x - xref) + (y - yref) * (y - yref) + (z - zref) * (z - zref);
+        }
+        pmx = pmn = ph;
+        cmx = cmn = cc;
+    }
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    // max via min of the negated value
+    __shared__ double sh[5][16];
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    double m[5] = {-r2, -pmx, pmn, -cmx, cmn};
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        double t = wave_min(m[q]);
+        if ((tid & 63) == 0) sh[q][tid >> 6] = t;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 0; q < 5; q++) {
+            double t = sh[q][0];
+            for (int w = 1; w < (int)(blockDim.x * blockDim.y) / 64; w++) t = fmin(t, sh[q][w]);
+            atomicMin(&keys[q], dkey(t));
+        }
+    }
+    block_reduce_store<2>(v, partials, counter, result, nb, bid);
+}
+
+}  // namespace wpf
+
+namespace wpf {
+
+int launch_diag(wpf_ctx* c, wpf_diag* out)
+{
+    const Dims d = dims_of(c);
+    const wpf_params& p = c->prm;
+    const double dx = p.dx;
+    const double xref = dx * (p.ix_ref - 1) + 0.5 * dx;
+    const double yref = dx * (p.iy_ref - 1) + 0.5 * dx;
+    const double zref = dx * (p.iz_ref - 1) + 0.5 * dx;
+    dim3 blk(64, 4, 1);
+    dim3 grd = grid3(d, blk);
+    const int nb = (int)(grd.x * grd.y * grd.z);
+    WPF_CHECK(nb <= c->red.max_blocks, WPF_ERR_ARG, "diagnostics grid too large");
+    unsigned long long init[5];
+    const double big = 1.0e308;
+    for (int q = 0; q < 5; q++) {
+        unsigned long long u;
+        memcpy(&u, &big, 8);
+        init[q] = u | 0x8000000000000000ull;
+    }
+    unsigned long long* keys = c->tau_key + 4;
+    WPF_HIP(hipMemcpyAsync(keys, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_diag, grd, blk, 0, c->stream, d, dx, xref, yref, zref, c->phi[c->cur],
+                       c->conc[c->cur], c->red.partials, c->red.counter, c->red.result, keys);
+    WPF_HIP(hipGetLastError());
+    double sums[2];
+    unsigned long long kout[5];
+    WPF_HIP(hipMemcpyAsync(sums, c->red.result, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipMemcpyAsync(kout, keys, sizeof(kout), hipMemcpyDeviceToHost, c->stream));
+    double pin = 0.0;
+    const int64_t cref = (int64_t)(p.ix_ref - 1) + (int64_t)d.nx * ((p.iy_ref - 1) + (int64_t)d.ny * (p.iz_ref - 1));
+    if (cref >= 0 && cref < c->g.n)
+        WPF_HIP(hipMemcpyAsync(&pin, c->p.p + cref, 8, hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipStreamSynchronize(c->stream));
+    double m[5];
+    for (int q = 0; q < 5; q++) {
+        unsigned long long k = kout[q];
+        unsigned long long u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+        memcpy(&m[q], &u, 8);
+    }
+    out->pv = dx * dx * dx * p.porosity * 0.5 * sums[0];
+    out->vcal = dx * dx * dx * (1.0 - p.porosity) * 0.5 * sums[1];
+    out->rwhmax = sqrt(-m[0]);
+    out->phase_max = -m[1];
+    out->phase_min = m[2];
+    out->concent_max = -m[3];
+    out->concent_min = m[4];
+    out->pinj = pin;
+    return WPF_OK;
+}
+
+int check_breakthrough(wpf_ctx* c, int itime)
+{
+    (void)itime;
+    int32_t v = 0;
+    WPF_HIP(hipMemcpyAsync(&v, c->bt_step, 4, hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipStreamSynchronize(c->stream));
+    c->bt_host = (v == 0x7fffffff) ? 0 : v;
+    return WPF_OK;
+}
+
+}  // namespace wpf

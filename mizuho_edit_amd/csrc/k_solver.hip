@@ -1,0 +1,910 @@
+// k_solver.hip -- matrix-free pressure solve for gfx950 (wPFAW3D.f90:603-1050).
+//
+// The reference assembles a COO matrix every step and hands it to PETSc.
+// Here the same linear system is applied matrix-free from the face
+// coefficients written by k_perm:
+//   free cell c:  (A x)_c = D_c x_c - sum_f t_f x_nb(f),   D_c = sum_f t_f
+//   Dirichlet cell: held at its value (identity row, columns eliminated)
+// i.e. the reference's rows negated (the reference's interior rows are
+// negative definite, tc = -sum t at 1:849), which makes A symmetric positive
+// definite on the free cells, so CG applies.
+//
+// Solvers
+//   CG + Jacobi, FP64                      (WPF_SOLVER_CG_JACOBI)
+//   CG + multigrid V-cycle, FP64           (WPF_SOLVER_MGCG)
+//   FP32 MG-PCG inside FP64 refinement     (WPF_SOLVER_MIXED)
+// Multigrid: 2x2x2 aggregation with the Galerkin coarse operator
+// A_c = P^T A P (still 7-point), damped-Jacobi smoothing, coarse correction
+// weighted by mg_coarse_scale, coarsest level by many Jacobi sweeps in one
+// workgroup.  The V-cycle is symmetric, so plain CG stays valid.
+//
+// Stopping test: the Jacobi-preconditioned 2-norm of the true residual of
+// the reference system (Dirichlet rows included) against the same norm of
+// its right-hand side, rnorm <= max(rtol*bnorm, atol), PETSc's default test.
+//
+// Launch shape: 1-D over cells, 256 threads, a fixed grid of at most
+// kMaxBlocks blocks, each block owning a contiguous chunk of cells; block b
+// is mapped to chunk (b % 8) * (nb / 8) + b / 8 so the chunks of one XCD
+// (blocks are dispatched round-robin over the 8 XCDs) are contiguous and
+// share that XCD's L2.
+#include "wpf_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace wpf {
+
+static constexpr int kThreads = 256;
+static constexpr int kMaxBlocks = 2048;
+
+// result slots: S_RN and S_T0 hold two values each
+enum Slot { S_PQ = 0, S_RN = 1, S_RZ0 = 4, S_RZ1 = 5, S_BN = 6, S_T0 = 8 };
+
+template <typename R>
+struct LV {
+    const R* tx;
+    const R* ty;
+    const R* tz;
+    const R* D;     // nullptr on the fine level (summed from t)
+    const R* invD;
+    const uint8_t* mask;  // fine level only
+    int nx, ny, nz;
+    int64_t plane, n;
+};
+
+template <typename R>
+static LV<R> view(const MGLevel<R>& L, const uint8_t* mask, bool fine)
+{
+    LV<R> v;
+    v.tx = L.tx.p; v.ty = L.ty.p; v.tz = L.tz.p;
+    v.D = fine ? nullptr : L.D.p;
+    v.invD = L.invD.p;
+    v.mask = fine ? mask : nullptr;
+    v.nx = L.g.nx; v.ny = L.g.ny; v.nz = L.g.nz;
+    v.plane = L.g.plane; v.n = L.g.n;
+    return v;
+}
+
+struct Chunk {
+    int64_t begin, end;
+};
+
+__device__ __forceinline__ Chunk my_chunk(int64_t n)
+{
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int64_t per = ((n + nb - 1) / nb + kThreads - 1) / kThreads * kThreads;
+    Chunk ch;
+    ch.begin = (int64_t)lb * per;
+    ch.end = min(n, ch.begin + per);
+    return ch;
+}
+
+static int blocks_for(int64_t n)
+{
+    int64_t b = (n + kThreads - 1) / kThreads;
+    if (b > kMaxBlocks) b = kMaxBlocks;
+    if (b >= 8) b = (b / 8) * 8;
+    return (int)std::max<int64_t>(1, b);
+}
+
+// off-diagonal sum and diagonal of row c
+template <typename R, bool FINE>
+__device__ __forceinline__ void row_parts(const LV<R>& L, const R* __restrict__ x, int64_t c, R& d,
+                                          R& s)
+{
+    const R txp = L.tx[c], txm = L.tx[c - 1];
+    const R typ = L.ty[c], tym = L.ty[c - L.nx];
+    const R tzp = L.tz[c], tzm = L.tz[c - L.plane];
+    if (FINE)
+        d = txp + txm + typ + tym + tzp + tzm;
+    else
+        d = L.D[c];
+    s = txp * x[c + 1] + txm * x[c - 1] + typ * x[c + L.nx] + tym * x[c - L.nx] +
+        tzp * x[c + L.plane] + tzm * x[c - L.plane];
+}
+
+template <typename R>
+__device__ __forceinline__ bool is_fixed(const LV<R>& L, int64_t c, bool fine)
+{
+    return fine && (L.mask[c] & WPF_MASK_FIXED);
+}
+
+// ---------------------------------------------------------------------------
+// true residual r = f - A x in FP64 (fine level, x holds Dirichlet values),
+// reduces ||invD r||^2 into result[0]; optionally also writes z = invD r and
+// reduces (r, z) into result[1]; optionally writes a scaled FP32 copy.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_true_resid(LV<double> L, const double* __restrict__ x,
+                                                         const double* __restrict__ f,
+                                                         double* __restrict__ r,
+                                                         double* __restrict__ z, float* __restrict__ r32,
+                                                         double s32, double* __restrict__ partials,
+                                                         unsigned* __restrict__ counter,
+                                                         double* __restrict__ result)
+{
+    const Chunk ch = my_chunk(L.n);
+    double acc[2] = {0.0, 0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        double d, s;
+        row_parts<double, true>(L, x, c, d, s);
+        double rv = (f ? f[c] : 0.0) - (d * x[c] - s);
+        if (L.mask[c] & WPF_MASK_FIXED) rv = 0.0;
+        if (r) r[c] = rv;
+        const double zv = rv * L.invD[c];
+        acc[0] += zv * zv;
+        if (z) {
+            z[c] = zv;
+            acc[1] += rv * zv;
+        }
+        if (r32) r32[c] = (float)(rv * s32);
+    }
+    block_reduce_store<2>(acc, partials, counter, result, gridDim.x, blockIdx.x);
+}
+
+// ||D^-1 b|| of the reference system: Dirichlet rows carry p_D; free rows
+// carry (f + sum over Dirichlet neighbours t p_D) / D.
+__global__ void __launch_bounds__(kThreads) k_bnorm(LV<double> L, const double* __restrict__ x,
+                                                    const double* __restrict__ f,
+                                                    double* __restrict__ partials,
+                                                    unsigned* __restrict__ counter,
+                                                    double* __restrict__ result)
+{
+    const Chunk ch = my_chunk(L.n);
+    double acc[1] = {0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        double v;
+        if (L.mask[c] & WPF_MASK_FIXED) {
+            v = x[c];
+        } else {
+            auto fx = [&](int64_t m) { return (L.mask[m] & WPF_MASK_FIXED) ? x[m] : 0.0; };
+            // masks outside the grid are never read: their coefficient is 0
+            double s = 0.0;
+            if (L.tx[c] != 0.0) s += L.tx[c] * fx(c + 1);
+            if (L.tx[c - 1] != 0.0) s += L.tx[c - 1] * fx(c - 1);
+            if (L.ty[c] != 0.0) s += L.ty[c] * fx(c + L.nx);
+            if (L.ty[c - L.nx] != 0.0) s += L.ty[c - L.nx] * fx(c - L.nx);
+            if (L.tz[c] != 0.0) s += L.tz[c] * fx(c + L.plane);
+            if (L.tz[c - L.plane] != 0.0) s += L.tz[c - L.plane] * fx(c - L.plane);
+            v = ((f ? f[c] : 0.0) + s) * L.invD[c];
+        }
+        acc[0] += v * v;
+    }
+    block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
+}
+
+// q = A p (masked), reduces (p, q).
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_matvec_dot(LV<R> L, const R* __restrict__ p,
+                                                         R* __restrict__ q,
+                                                         double* __restrict__ partials,
+                                                         unsigned* __restrict__ counter,
+                                                         double* __restrict__ result)
+{
+    const Chunk ch = my_chunk(L.n);
+    double acc[1] = {0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        R d, s;
+        row_parts<R, FINE>(L, p, c, d, s);
+        R qv = d * p[c] - s;
+        if (is_fixed(L, c, FINE)) qv = (R)0;
+        q[c] = qv;
+        acc[0] += (double)p[c] * (double)qv;
+    }
+    block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
+}
+
+// x += alpha p; r -= alpha q; alpha = res[num]/res[den].  Reduces
+// ||invD r||^2 (slot 0) and, when z != nullptr, writes z = invD r and
+// reduces (r, z) (slot 1).
+template <typename R, typename X>
+__global__ void __launch_bounds__(kThreads) k_cg_update(int64_t n, const double* __restrict__ res,
+                                                        int num, int den, X* __restrict__ x,
+                                                        const R* __restrict__ p, R* __restrict__ r,
+                                                        const R* __restrict__ q,
+                                                        const R* __restrict__ invD, R* __restrict__ z,
+                                                        double* __restrict__ partials,
+                                                        unsigned* __restrict__ counter,
+                                                        double* __restrict__ result)
+{
+    const double alpha = res[num] / res[den];
+    const R a = (R)alpha;
+    const Chunk ch = my_chunk(n);
+    double acc[2] = {0.0, 0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        x[c] = x[c] + (X)(a * p[c]);
+        const R rv = r[c] - a * q[c];
+        r[c] = rv;
+        const R zv = rv * invD[c];
+        acc[0] += (double)zv * (double)zv;
+        if (z) {
+            z[c] = zv;
+            acc[1] += (double)rv * (double)zv;
+        }
+    }
+    block_reduce_store<2>(acc, partials, counter, result, gridDim.x, blockIdx.x);
+}
+
+// p = z + beta p, beta = res[num]/res[den] (first iteration: p = z).
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_xpby(int64_t n, const double* __restrict__ res, int num,
+                                                   int den, int first, const R* __restrict__ z,
+                                                   R* __restrict__ p)
+{
+    const R beta = first ? (R)0 : (R)(res[num] / res[den]);
+    const Chunk ch = my_chunk(n);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads)
+        p[c] = first ? z[c] : z[c] + beta * p[c];
+}
+
+// (a, b) dot product
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_dot(int64_t n, const R* __restrict__ a,
+                                                  const R* __restrict__ b,
+                                                  double* __restrict__ partials,
+                                                  unsigned* __restrict__ counter,
+                                                  double* __restrict__ result)
+{
+    const Chunk ch = my_chunk(n);
+    double acc[1] = {0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads)
+        acc[0] += (double)a[c] * (double)b[c];
+    block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// multigrid
+// ---------------------------------------------------------------------------
+// first sweep from x = 0: x = w invD b
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_smooth0(int64_t n, R w, const R* __restrict__ b,
+                                                      const R* __restrict__ invD, R* __restrict__ x)
+{
+    const Chunk ch = my_chunk(n);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = w * invD[c] * b[c];
+}
+
+// damped Jacobi: y = (1-w) x + w invD (b + N x)   (fixed cells: invD = 0,
+// x = 0, so y stays 0)
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_smooth(LV<R> L, R w, const R* __restrict__ b,
+                                                     const R* __restrict__ x, R* __restrict__ y)
+{
+    const Chunk ch = my_chunk(L.n);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        R d, s;
+        row_parts<R, FINE>(L, x, c, d, s);
+        (void)d;
+        y[c] = ((R)1 - w) * x[c] + w * L.invD[c] * (b[c] + s);
+    }
+}
+
+// coarse right-hand side: bc[I] = sum over children of (b - A x)
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_resid_restrict(LV<R> L, int cnx, int cny, int cnz,
+                                                             const R* __restrict__ b,
+                                                             const R* __restrict__ x,
+                                                             R* __restrict__ bc)
+{
+    const int64_t nc = (int64_t)cnx * cny * cnz;
+    const Chunk ch = my_chunk(nc);
+    for (int64_t C = ch.begin + threadIdx.x; C < ch.end; C += kThreads) {
+        const int I = (int)(C % cnx);
+        const int J = (int)((C / cnx) % cny);
+        const int K = (int)(C / ((int64_t)cnx * cny));
+        R acc = (R)0;
+        for (int dk = 0; dk < 2; dk++) {
+            const int k = 2 * K + dk;
+            if (k >= L.nz) break;
+            for (int dj = 0; dj < 2; dj++) {
+                const int j = 2 * J + dj;
+                if (j >= L.ny) break;
+                for (int di = 0; di < 2; di++) {
+                    const int i = 2 * I + di;
+                    if (i >= L.nx) break;
+                    const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
+                    if (is_fixed(L, c, FINE)) continue;
+                    R d, s;
+                    row_parts<R, FINE>(L, x, c, d, s);
+                    acc += b[c] - (d * x[c] - s);
+                }
+            }
+        }
+        bc[C] = acc;
+    }
+}
+
+// x += scale * P e  (piecewise constant; fixed fine cells untouched)
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_prolong(LV<R> L, int cnx, int cny, R scale,
+                                                      const R* __restrict__ e, R* __restrict__ x)
+{
+    const Chunk ch = my_chunk(L.n);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        const int i = (int)(c % L.nx);
+        const int j = (int)((c / L.nx) % L.ny);
+        const int k = (int)(c / L.plane);
+        const int64_t C = (i >> 1) + (int64_t)cnx * ((j >> 1) + (int64_t)cny * (k >> 1));
+        if (is_fixed(L, c, FINE)) continue;
+        x[c] = x[c] + scale * e[C];
+    }
+}
+
+// Galerkin aggregation A_c = P^T A P on 2x2x2 aggregates.
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_coarsen(LV<R> L, int cnx, int cny, int cnz,
+                                                      int64_t cplane, R* __restrict__ ctx,
+                                                      R* __restrict__ cty, R* __restrict__ ctz,
+                                                      R* __restrict__ cD, R* __restrict__ cinvD)
+{
+    const int64_t nc = (int64_t)cnx * cny * cnz;
+    const Chunk ch = my_chunk(nc);
+    for (int64_t C = ch.begin + threadIdx.x; C < ch.end; C += kThreads) {
+        const int I = (int)(C % cnx);
+        const int J = (int)((C / cnx) % cny);
+        const int K = (int)(C / cplane);
+        R D = 0, tx = 0, ty = 0, tz = 0;
+        for (int dk = 0; dk < 2; dk++) {
+            const int k = 2 * K + dk;
+            if (k >= L.nz) break;
+            for (int dj = 0; dj < 2; dj++) {
+                const int j = 2 * J + dj;
+                if (j >= L.ny) break;
+                for (int di = 0; di < 2; di++) {
+                    const int i = 2 * I + di;
+                    if (i >= L.nx) break;
+                    const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
+                    if (is_fixed(L, c, FINE)) continue;
+                    if (FINE)
+                        D += L.tx[c] + L.tx[c - 1] + L.ty[c] + L.ty[c - L.nx] + L.tz[c] +
+                             L.tz[c - L.plane];
+                    else
+                        D += L.D[c];
+                    // +x face
+                    if (i + 1 < L.nx && !is_fixed(L, c + 1, FINE)) {
+                        if (di == 0) D -= (R)2 * L.tx[c];
+                        else tx += L.tx[c];
+                    }
+                    if (j + 1 < L.ny && !is_fixed(L, c + L.nx, FINE)) {
+                        if (dj == 0) D -= (R)2 * L.ty[c];
+                        else ty += L.ty[c];
+                    }
+                    if (k + 1 < L.nz && !is_fixed(L, c + L.plane, FINE)) {
+                        if (dk == 0) D -= (R)2 * L.tz[c];
+                        else tz += L.tz[c];
+                    }
+                }
+            }
+        }
+        // faces on the global edge of the coarse grid stay 0
+        if (I == cnx - 1) tx = 0;
+        if (J == cny - 1) ty = 0;
+        if (K == cnz - 1) tz = 0;
+        ctx[C] = tx;
+        cty[C] = ty;
+        ctz[C] = tz;
+        cD[C] = D;
+        cinvD[C] = D > (R)0 ? (R)1 / D : (R)0;
+    }
+}
+
+// coarsest level: nsweep damped-Jacobi sweeps from 0, one workgroup,
+// ping-pong between x and y (both global, L2-resident)
+template <typename R>
+__global__ void __launch_bounds__(1024) k_coarsest(LV<R> L, R w, int nsweep, const R* __restrict__ b,
+                                                   R* __restrict__ x, R* __restrict__ y)
+{
+    for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) x[c] = w * L.invD[c] * b[c];
+    __syncthreads();
+    for (int it = 1; it < nsweep; it++) {
+        for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) {
+            R d, s;
+            row_parts<R, false>(L, x, c, d, s);
+            (void)d;
+            y[c] = ((R)1 - w) * x[c] + w * L.invD[c] * (b[c] + s);
+        }
+        __syncthreads();
+        for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) x[c] = y[c];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// precision conversion of the fine operator: t32 = t64 * s, invD32 = 1/D32
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_to_f32(LV<double> L, double s, float* __restrict__ tx,
+                                                     float* __restrict__ ty, float* __restrict__ tz,
+                                                     float* __restrict__ invD)
+{
+    const Chunk ch = my_chunk(L.n);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        const float a = (float)(L.tx[c] * s), b = (float)(L.ty[c] * s), d = (float)(L.tz[c] * s);
+        tx[c] = a;
+        ty[c] = b;
+        tz[c] = d;
+        const float D = a + (float)(L.tx[c - 1] * s) + b + (float)(L.ty[c - L.nx] * s) + d +
+                        (float)(L.tz[c - L.plane] * s);
+        invD[c] = (L.invD[c] == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
+    }
+}
+
+// x64 += e32 (scaled back)
+__global__ void __launch_bounds__(kThreads) k_add_f32(int64_t n, const float* __restrict__ e,
+                                                      double* __restrict__ x)
+{
+    const Chunk ch = my_chunk(n);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = x[c] + (double)e[c];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_zero(int64_t n, R* __restrict__ x)
+{
+    const Chunk ch = my_chunk(n);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = (R)0;
+}
+
+// enforce Dirichlet values (sparse list)
+__global__ void k_set_dirichlet(int n, const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                double* __restrict__ x)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[idx[t]] = val[t];
+}
+
+// dense injection right-hand side f = q dx^2/dx^3 (1:817, sign of the SPD form)
+__global__ void k_set_inj(int n, const int32_t* __restrict__ idx, const double* __restrict__ val,
+                          double* __restrict__ f)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) f[idx[t]] = val[t];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct Launch {
+    hipStream_t s;
+    Reducer* red;
+};
+
+// Launch sites that should be timed by wpf_profile_start declare a local
+// `kid_` holding their WPF_KERNEL_* id, hiding this default.
+static const int kid_ = -1;
+
+#define LAUNCH1D(kern, n, ...)                                                          \
+    do {                                                                               \
+        prof_mark(c, kid_, true);                                                      \
+        hipLaunchKernelGGL(kern, dim3(blocks_for(n)), dim3(kThreads), 0, c->stream,    \
+                           __VA_ARGS__);                                               \
+        prof_mark(c, kid_, false);                                                     \
+        WPF_HIP(hipGetLastError());                                                    \
+    } while (0)
+
+static int read_results(wpf_ctx* c, int first, int count)
+{
+    WPF_HIP(hipMemcpyAsync(c->red.host + first, c->red.result + first, sizeof(double) * count,
+                           hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipStreamSynchronize(c->stream));
+    return WPF_OK;
+}
+
+template <typename R>
+static int alloc_level(MGLevel<R>& L, const Grid& g)
+{
+    L.g = g;
+    int rc;
+    if ((rc = dev_alloc(L.tx, g)) || (rc = dev_alloc(L.ty, g)) || (rc = dev_alloc(L.tz, g)) ||
+        (rc = dev_alloc(L.D, g)) || (rc = dev_alloc(L.invD, g)) || (rc = dev_alloc(L.x, g)) ||
+        (rc = dev_alloc(L.b, g)) || (rc = dev_alloc(L.r, g)))
+        return rc;
+    return WPF_OK;
+}
+template <typename R>
+static void free_level(MGLevel<R>& L)
+{
+    dev_free(L.tx); dev_free(L.ty); dev_free(L.tz); dev_free(L.D); dev_free(L.invD);
+    dev_free(L.x); dev_free(L.b); dev_free(L.r);
+}
+
+// Build the hierarchy below a fine level: levels[0] is the fine level
+// (its tx/ty/tz/invD point at the fine operator; its D is unused).
+static int target_levels(const Grid& g, int nlevels)
+{
+    int nl = 1, nx = g.nx, ny = g.ny, nz = g.nz;
+    while (nl < nlevels && (nx > 1 || ny > 1 || nz > 1)) {
+        nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+        nl++;
+    }
+    return nl;
+}
+
+template <typename R>
+static int ensure_hierarchy(wpf_ctx* c, std::vector<MGLevel<R>>& lv, int nlevels)
+{
+    (void)c;
+    const int want = target_levels(lv[0].g, nlevels);
+    if ((int)lv.size() == want) return WPF_OK;
+    for (size_t l = 1; l < lv.size(); l++) free_level(lv[l]);
+    lv.resize(1);
+    for (int l = 1; l < want; l++) {
+        const Grid& f = lv[l - 1].g;
+        MGLevel<R> L;
+        int rc = alloc_level(L, make_grid((f.nx + 1) / 2, (f.ny + 1) / 2, (f.nz + 1) / 2));
+        if (rc) return rc;
+        lv.push_back(L);
+    }
+    return WPF_OK;
+}
+
+template <typename R>
+static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv)
+{
+    for (size_t l = 1; l < lv.size(); l++) {
+        MGLevel<R>& C = lv[l];
+        const bool fine = (l == 1);
+        LV<R> F = view(lv[l - 1], c->mask, fine);
+        if (fine)
+            LAUNCH1D((k_coarsen<R, true>), C.g.n, F, C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p,
+                     C.ty.p, C.tz.p, C.D.p, C.invD.p);
+        else
+            LAUNCH1D((k_coarsen<R, false>), C.g.n, F, C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p,
+                     C.ty.p, C.tz.p, C.D.p, C.invD.p);
+    }
+    return WPF_OK;
+}
+
+struct MGOpts {
+    int sweeps;
+    double omega;
+    double cscale;
+    int coarse_sweeps;
+};
+
+// z = M b on level l (V-cycle).  Level 0 vectors are passed explicitly; on
+// coarse levels the level's own b/x/r are used.
+template <typename R>
+static int vcycle(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, const R* b, R* x, R* tmp,
+                  const MGOpts& o)
+{
+    MGLevel<R>& L = lv[l];
+    const bool fine = (l == 0);
+    LV<R> V = view(L, c->mask, fine);
+    const R w = (R)o.omega;
+    if (l + 1 == lv.size()) {
+        hipLaunchKernelGGL((k_coarsest<R>), dim3(1), dim3(1024), 0, c->stream, V, w, o.coarse_sweeps,
+                           b, x, tmp);
+        WPF_HIP(hipGetLastError());
+        return WPF_OK;
+    }
+    // Damped-Jacobi sweeps ping-pong between two buffers.  The first sweep
+    // (from 0) writes tmp and 2*sweeps-1 more sweeps follow, an odd number
+    // of flips, so the result lands in x.
+    R* cur = tmp;
+    R* oth = x;
+    const int kid_smooth = fine ? WPF_KERNEL_SMOOTH : -1;
+    LAUNCH1D(k_smooth0<R>, L.g.n, L.g.n, w, b, L.invD.p, cur);
+    for (int s = 1; s < o.sweeps; s++) {
+        const int kid_ = kid_smooth;
+        if (fine) LAUNCH1D((k_smooth<R, true>), L.g.n, V, w, b, (const R*)cur, oth);
+        else LAUNCH1D((k_smooth<R, false>), L.g.n, V, w, b, (const R*)cur, oth);
+        std::swap(cur, oth);
+    }
+    MGLevel<R>& C = lv[l + 1];
+    {
+        const int kid_ = fine ? WPF_KERNEL_RESTRICT : -1;
+        if (fine) LAUNCH1D((k_resid_restrict<R, true>), C.g.n, V, C.g.nx, C.g.ny, C.g.nz, b, (const R*)cur, C.b.p);
+        else LAUNCH1D((k_resid_restrict<R, false>), C.g.n, V, C.g.nx, C.g.ny, C.g.nz, b, (const R*)cur, C.b.p);
+    }
+    int rc = vcycle(c, lv, l + 1, C.b.p, C.x.p, C.r.p, o);
+    if (rc) return rc;
+    {
+        const int kid_ = fine ? WPF_KERNEL_PROLONG : -1;
+        if (fine) LAUNCH1D((k_prolong<R, true>), L.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
+        else LAUNCH1D((k_prolong<R, false>), L.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
+    }
+    for (int s = 0; s < o.sweeps; s++) {
+        const int kid_ = kid_smooth;
+        if (fine) LAUNCH1D((k_smooth<R, true>), L.g.n, V, w, b, (const R*)cur, oth);
+        else LAUNCH1D((k_smooth<R, false>), L.g.n, V, w, b, (const R*)cur, oth);
+        std::swap(cur, oth);
+    }
+    return WPF_OK;
+}
+
+static void fill_defaults(const wpf_solver_opts* in, wpf_solver_opts& o)
+{
+    o = *in;
+    if (o.maxit <= 0) o.maxit = 1000;
+    if (o.rtol <= 0) o.rtol = 1e-10;
+    if (o.atol <= 0) o.atol = 1e-50;
+    if (o.mg_levels <= 0) o.mg_levels = 5;
+    if (o.mg_sweeps <= 0) o.mg_sweeps = 2;
+    if (o.mg_omega <= 0) o.mg_omega = 0.8;
+    if (o.mg_coarse_scale <= 0) o.mg_coarse_scale = 2.0;
+    if (o.inner_rtol <= 0) o.inner_rtol = 1e-4;
+    if (o.inner_maxit <= 0) o.inner_maxit = 100;
+}
+
+// Prepare x (Dirichlet values) and the right-hand side; returns bnorm.
+static int prepare(wpf_ctx* c, double* bnorm, double** f_out)
+{
+    if (c->ndir > 0) {
+        hipLaunchKernelGGL(k_set_dirichlet, dim3((c->ndir + 255) / 256), dim3(256), 0, c->stream,
+                           c->ndir, c->dir_idx, c->dir_val, c->p.p);
+        WPF_HIP(hipGetLastError());
+    }
+    double* f = nullptr;
+    if (c->ninj > 0) {
+        // the dense source lives in the fine level's scratch q
+        f = c->q.p;
+        WPF_HIP(hipMemsetAsync(f, 0, sizeof(double) * (size_t)c->g.n, c->stream));
+        hipLaunchKernelGGL(k_set_inj, dim3((c->ninj + 255) / 256), dim3(256), 0, c->stream, c->ninj,
+                           c->inj_idx, c->inj_val, f);
+        WPF_HIP(hipGetLastError());
+    }
+    MGLevel<double> F;
+    F.g = c->g; F.tx = c->tx; F.ty = c->ty; F.tz = c->tz; F.invD = c->invD;
+    LV<double> L = view(F, c->mask, true);
+    LAUNCH1D(k_bnorm, c->g.n, L, c->p.p, f, c->red.partials, c->red.counter, c->red.result + S_BN);
+    int rc = read_results(c, S_BN, 1);
+    if (rc) return rc;
+    *bnorm = std::sqrt(c->red.host[S_BN]);
+    *f_out = f;
+    return WPF_OK;
+}
+
+// FP64 PCG with Jacobi (mg == false) or the FP64 V-cycle.
+static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats* st)
+{
+    const int64_t n = c->g.n;
+    double bnorm;
+    double* f;
+    int rc = prepare(c, &bnorm, &f);
+    if (rc) return rc;
+    if (f) {
+        // q holds f; PCG needs q: move f into xprev (free otherwise)
+        WPF_HIP(hipMemcpyAsync(c->xprev.p, f, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
+                               c->stream));
+        f = c->xprev.p;
+    }
+    const double tol = std::max(o.rtol * bnorm, o.atol);
+    MGLevel<double> F;
+    F.g = c->g; F.tx = c->tx; F.ty = c->ty; F.tz = c->tz; F.invD = c->invD;
+    LV<double> L = view(F, c->mask, true);
+    MGOpts mo{o.mg_sweeps, o.mg_omega, o.mg_coarse_scale, 32};
+    if (mg) {
+        if (c->mg64.empty()) c->mg64.resize(1);
+        c->mg64[0].g = c->g;
+        c->mg64[0].tx = c->tx; c->mg64[0].ty = c->ty; c->mg64[0].tz = c->tz;
+        c->mg64[0].invD = c->invD;
+        if ((rc = ensure_hierarchy(c, c->mg64, o.mg_levels))) return rc;
+        if ((rc = coarsen_all(c, c->mg64))) return rc;
+    }
+    double* x = c->p.p;
+    double* r = c->r.p;
+    double* z = c->z.p;
+    double* p = c->pp.p;
+    double* q = c->q.p;
+    int its = 0;
+    double rn = 0.0, rn0 = -1.0;
+    int restarts = 0;
+    for (;;) {
+        // true residual (and Jacobi z)
+        {
+            const int kid_ = WPF_KERNEL_TRUE_RESID;
+            LAUNCH1D(k_true_resid, n, L, x, f, r, mg ? nullptr : z, nullptr, 0.0, c->red.partials,
+                     c->red.counter, c->red.result + S_T0);
+        }
+        if ((rc = read_results(c, S_T0, 2))) return rc;
+        rn = std::sqrt(c->red.host[S_T0]);
+        if (!std::isfinite(rn)) {
+            set_error("pressure residual is not finite");
+            return WPF_ERR_NAN;
+        }
+        if (rn0 < 0) rn0 = rn;
+        if (rn <= tol || its >= o.maxit || restarts > 3) break;
+        int rzs = S_RZ0;
+        if (mg) {
+            // z = M r, rz = (r, z)
+            if ((rc = vcycle(c, c->mg64, 0, (const double*)r, z, q, mo))) return rc;
+            LAUNCH1D(k_dot<double>, n, n, (const double*)r, (const double*)z, c->red.partials,
+                     c->red.counter, c->red.result + rzs);
+        } else {
+            WPF_HIP(hipMemcpyAsync(c->red.result + rzs, c->red.result + S_T0 + 1, sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        }
+        LAUNCH1D(k_xpby<double>, n, n, c->red.result, 0, 0, 1, (const double*)z, p);
+        bool conv = false;
+        while (its < o.maxit) {
+            {
+                const int kid_ = WPF_KERNEL_MATVEC;
+                LAUNCH1D((k_matvec_dot<double, true>), n, L, (const double*)p, q, c->red.partials,
+                         c->red.counter, c->red.result + S_PQ);
+            }
+            const int rzn = (rzs == S_RZ0) ? S_RZ1 : S_RZ0;
+            const int kid_ = WPF_KERNEL_CG_UPDATE;
+            if (mg) {
+                LAUNCH1D((k_cg_update<double, double>), n, n, (const double*)c->red.result, rzs, S_PQ,
+                         x, (const double*)p, r, (const double*)q, (const double*)c->invD.p,
+                         (double*)nullptr, c->red.partials, c->red.counter, c->red.result + S_RN);
+            } else {
+                LAUNCH1D((k_cg_update<double, double>), n, n, (const double*)c->red.result, rzs, S_PQ,
+                         x, (const double*)p, r, (const double*)q, (const double*)c->invD.p, z,
+                         c->red.partials, c->red.counter, c->red.result + S_RN);
+            }
+            its++;
+            if ((rc = read_results(c, S_RN, 2))) return rc;
+            rn = std::sqrt(c->red.host[S_RN]);
+            if (!std::isfinite(rn)) {
+                set_error("pressure residual is not finite");
+                return WPF_ERR_NAN;
+            }
+            if (rn <= tol) {
+                conv = true;
+                break;
+            }
+            if (mg) {
+                if ((rc = vcycle(c, c->mg64, 0, (const double*)r, z, q, mo))) return rc;
+                LAUNCH1D(k_dot<double>, n, n, (const double*)r, (const double*)z, c->red.partials,
+                         c->red.counter, c->red.result + rzn);
+            } else {
+                WPF_HIP(hipMemcpyAsync(c->red.result + rzn, c->red.result + S_RN + 1, sizeof(double),
+                                       hipMemcpyDeviceToDevice, c->stream));
+            }
+            LAUNCH1D(k_xpby<double>, n, n, c->red.result, rzn, rzs, 0, (const double*)z, p);
+            rzs = rzn;
+        }
+        (void)conv;
+        restarts++;
+    }
+    st->iterations = its;
+    st->outer = 0;
+    st->rnorm0 = rn0;
+    st->rnorm = rn;
+    st->bnorm = bnorm;
+    st->converged = rn <= tol;
+    return st->converged ? WPF_OK : WPF_ERR_NOCONV;
+}
+
+// FP32 MG-PCG inside FP64 iterative refinement.
+static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
+{
+    const int64_t n = c->g.n;
+    double bnorm;
+    double* f;
+    int rc = prepare(c, &bnorm, &f);
+    if (rc) return rc;
+    if (f) {
+        WPF_HIP(hipMemcpyAsync(c->xprev.p, f, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
+                               c->stream));
+        f = c->xprev.p;
+    }
+    const double tol = std::max(o.rtol * bnorm, o.atol);
+    // FP32 fine level
+    if (c->mg32.empty()) {
+        c->mg32.resize(1);
+        if ((rc = alloc_level(c->mg32[0], c->g))) return rc;
+        if ((rc = dev_alloc(c->r32, c->g)) || (rc = dev_alloc(c->z32, c->g)) ||
+            (rc = dev_alloc(c->p32, c->g)) || (rc = dev_alloc(c->q32, c->g)) ||
+            (rc = dev_alloc(c->x32, c->g)))
+            return rc;
+    }
+    MGLevel<double> F;
+    F.g = c->g; F.tx = c->tx; F.ty = c->ty; F.tz = c->tz; F.invD = c->invD;
+    LV<double> L = view(F, c->mask, true);
+    const double s = c->t_scale;  // t32 = s t64, so A32 e = s r64 solves A64 e = r64
+    // grow the hierarchy first: it appends to mg32, which may move mg32[0]
+    if ((rc = ensure_hierarchy(c, c->mg32, o.mg_levels))) return rc;
+    MGLevel<float>& F32 = c->mg32[0];
+    LAUNCH1D(k_to_f32, n, L, s, F32.tx.p, F32.ty.p, F32.tz.p, F32.invD.p);
+    if ((rc = coarsen_all(c, c->mg32))) return rc;
+    LV<float> L32 = view(F32, c->mask, true);
+    MGOpts mo{o.mg_sweeps, o.mg_omega, o.mg_coarse_scale, 32};
+
+    double* x = c->p.p;
+    float* r = c->r32.p;
+    float* z = c->z32.p;
+    float* p = c->p32.p;
+    float* q = c->q32.p;
+    float* e = c->x32.p;
+    int its = 0, outer = 0;
+    double rn = 0.0, rn0 = -1.0;
+    for (;;) {
+        // FP64 true residual and its FP32 image r32 = s r64
+        {
+            const int kid_ = WPF_KERNEL_TRUE_RESID;
+            LAUNCH1D(k_true_resid, n, L, x, f, (double*)nullptr, (double*)nullptr, r, s,
+                     c->red.partials, c->red.counter, c->red.result + S_T0);
+        }
+        if ((rc = read_results(c, S_T0, 1))) return rc;
+        rn = std::sqrt(c->red.host[S_T0]);
+        if (!std::isfinite(rn)) {
+            set_error("pressure residual is not finite");
+            return WPF_ERR_NAN;
+        }
+        if (rn0 < 0) rn0 = rn;
+        if (rn <= tol || its >= o.maxit || outer >= 50) break;
+        outer++;
+        // inner FP32 MG-PCG on A32 e = r32 from e = 0
+        LAUNCH1D(k_zero<float>, n, n, e);
+        int rzs = S_RZ0;
+        if ((rc = vcycle(c, c->mg32, 0, (const float*)r, z, q, mo))) return rc;
+        LAUNCH1D(k_dot<float>, n, n, (const float*)r, (const float*)z, c->red.partials,
+                 c->red.counter, c->red.result + rzs);
+        LAUNCH1D(k_xpby<float>, n, n, c->red.result, 0, 0, 1, (const float*)z, p);
+        // the inner residual norm in FP32 units equals the FP64 one (invD32 r32 = invD64 r64)
+        const double inner_tol = std::max(o.inner_rtol * rn, 0.5 * tol);
+        int k = 0;
+        while (k < o.inner_maxit && its < o.maxit) {
+            {
+                const int kid_ = WPF_KERNEL_MATVEC;
+                LAUNCH1D((k_matvec_dot<float, true>), n, L32, (const float*)p, q, c->red.partials,
+                         c->red.counter, c->red.result + S_PQ);
+            }
+            {
+                const int kid_ = WPF_KERNEL_CG_UPDATE;
+                LAUNCH1D((k_cg_update<float, float>), n, n, (const double*)c->red.result, rzs, S_PQ,
+                         e, (const float*)p, r, (const float*)q, (const float*)F32.invD.p,
+                         (float*)nullptr, c->red.partials, c->red.counter, c->red.result + S_RN);
+            }
+            k++;
+            its++;
+            if ((rc = read_results(c, S_RN, 1))) return rc;
+            const double rin = std::sqrt(c->red.host[S_RN]);
+            if (!std::isfinite(rin)) break;
+            if (rin <= inner_tol) break;
+            const int rzn = (rzs == S_RZ0) ? S_RZ1 : S_RZ0;
+            if ((rc = vcycle(c, c->mg32, 0, (const float*)r, z, q, mo))) return rc;
+            LAUNCH1D(k_dot<float>, n, n, (const float*)r, (const float*)z, c->red.partials,
+                     c->red.counter, c->red.result + rzn);
+            LAUNCH1D(k_xpby<float>, n, n, c->red.result, rzn, rzs, 0, (const float*)z, p);
+            rzs = rzn;
+        }
+        LAUNCH1D(k_add_f32, n, n, (const float*)e, x);
+    }
+    st->iterations = its;
+    st->outer = outer;
+    st->rnorm0 = rn0;
+    st->rnorm = rn;
+    st->bnorm = bnorm;
+    st->converged = rn <= tol;
+    return st->converged ? WPF_OK : WPF_ERR_NOCONV;
+}
+
+int solve_pressure(wpf_ctx* c, const wpf_solver_opts* in, wpf_solve_stats* st)
+{
+    WPF_CHECK(c->have_perm, WPF_ERR_STATE, "solvePressure before calcPermeability");
+    wpf_solver_opts o;
+    fill_defaults(in, o);
+    wpf_solve_stats local{};
+    if (!st) st = &local;
+    *st = wpf_solve_stats{};
+    auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    switch (o.solver) {
+    case WPF_SOLVER_CG_JACOBI: rc = pcg64(c, o, false, st); break;
+    case WPF_SOLVER_MGCG: rc = pcg64(c, o, true, st); break;
+    case WPF_SOLVER_MIXED: rc = mixed(c, o, st); break;
+    default:
+        set_error("unknown solver %d", o.solver);
+        return WPF_ERR_ARG;
+    }
+    if (rc != WPF_OK && rc != WPF_ERR_NOCONV) return rc;  // keep the first error's text
+    WPF_HIP(hipStreamSynchronize(c->stream));
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    c->have_pressure = true;
+    return rc;
+}
+
+void free_solver(wpf_ctx* c)
+{
+    for (size_t l = 1; l < c->mg64.size(); l++) free_level(c->mg64[l]);
+    c->mg64.clear();
+    for (size_t l = 0; l < c->mg32.size(); l++) free_level(c->mg32[l]);
+    c->mg32.clear();
+    dev_free(c->r32); dev_free(c->z32); dev_free(c->p32); dev_free(c->q32); dev_free(c->x32);
+}
+
+}  // namespace wpf

@@ -1,0 +1,310 @@
+// wpf_internal.h -- device layout and shared helpers of libwpf_hip.so.
+//
+// Layout in HBM (one grid per context, interior cells only):
+//   cell (i,j,k), 0-based, lives at c = i + nx*(j + ny*k)   (x fastest)
+//   fields phi/conc are double-buffered; p, krock, dphidt, dcdt single
+//   face arrays tx/ty/tz hold t = k_face/mu of the face (i+1/2) etc., with
+//   the global-edge face (i = nx-1) stored as 0 (wPFAW3D.f90:843-848)
+//   every face array and solver vector is allocated with a zero pad of
+//   PAD(nx*ny) elements in front and behind, so the -x/-y/-z neighbour of a
+//   boundary cell reads a zero coefficient or a harmless finite value and
+//   the stencil needs no branch
+//   mask: 1 byte per cell, bit0 = well (lw > 0), bit1 = Dirichlet pressure,
+//   bit2 = injection inlet
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+#include <vector>
+
+#include "../../include/wpf.h"
+
+#define WPF_MASK_WELL 1
+#define WPF_MASK_FIXED 2
+#define WPF_MASK_INLET 4
+
+namespace wpf {
+
+void set_error(const char* fmt, ...);
+
+#define WPF_HIP(call)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            ::wpf::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,               \
+                             hipGetErrorString(e_));                                  \
+            return WPF_ERR_HIP;                                                       \
+        }                                                                             \
+    } while (0)
+
+#define WPF_CHECK(cond, code, ...)                                                    \
+    do {                                                                              \
+        if (!(cond)) {                                                                \
+            ::wpf::set_error(__VA_ARGS__);                                            \
+            return (code);                                                            \
+        }                                                                             \
+    } while (0)
+
+// Geometry of one grid level.
+struct Grid {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t n = 0;     // nx*ny*nz
+    int64_t plane = 0; // nx*ny
+    int64_t pad = 0;   // zero pad before and after padded arrays
+};
+
+inline Grid make_grid(int nx, int ny, int nz)
+{
+    Grid g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.plane = (int64_t)nx * ny;
+    g.n = g.plane * nz;
+    g.pad = ((g.plane + 63) / 64) * 64;
+    return g;
+}
+
+// A padded device array: base points at element 0 of the interior.
+template <typename T>
+struct Dev {
+    T* raw = nullptr;
+    T* p = nullptr;
+    int64_t n = 0, pad = 0;
+};
+
+// Multigrid level of the pressure operator in precision R.
+template <typename R>
+struct MGLevel {
+    Grid g;
+    Dev<R> tx, ty, tz;   // face coefficients (coarse: Galerkin sums)
+    Dev<R> D, invD;      // diagonal (coarse levels) and its inverse (0 at fixed)
+    Dev<R> x, b, r;      // correction, right-hand side, scratch
+};
+
+// Reduction scratch: per-block partials + completion counter + results.
+struct Reducer {
+    double* partials = nullptr;  // [slots * max_blocks]
+    unsigned* counter = nullptr; // [slots]
+    double* result = nullptr;    // [slots] device
+    double* host = nullptr;      // pinned mirror of result
+    int max_blocks = 0;
+    int slots = 0;
+};
+
+}  // namespace wpf
+
+struct wpf_learned;  // learned pressure-correction network (k_learned.hip)
+
+struct wpf_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    wpf_params prm{};
+    wpf::Grid g;
+
+    // wells
+    std::vector<wpf_well> wells;
+    std::vector<int32_t> lw_host;   // haloed copy of lw (for export)
+    int ndir = 0, ninj = 0;
+    int32_t* dir_idx = nullptr; double* dir_val = nullptr;
+    int32_t* inj_idx = nullptr; double* inj_val = nullptr;
+    uint8_t* mask = nullptr;
+
+    // state fields (interior)
+    double* phi[2] = {nullptr, nullptr};
+    double* conc[2] = {nullptr, nullptr};
+    int cur = 0;
+    wpf::Dev<double> p;            // padded: the solver reads it with neighbours
+    double* krock = nullptr;
+    double* dphidt = nullptr;
+    double* dcdt = nullptr;
+    double* vx = nullptr; double* vy = nullptr; double* vz = nullptr;  // lazily
+    bool have_perm = false, have_velocity = false, have_pressure = false;
+
+    // FP64 operator and solver vectors (fine level)
+    wpf::Dev<double> tx, ty, tz, invD;
+    wpf::Dev<double> r, z, pp, q, xprev;
+    std::vector<wpf::MGLevel<double>> mg64;   // levels 1.. (coarse)
+    std::vector<wpf::MGLevel<float>> mg32;    // levels 0.. (FP32 copy incl. fine)
+    wpf::Dev<float> r32, z32, p32, q32, x32;
+    bool have_mg64 = false, have_mg32 = false;
+    int mg_levels_built = 0;
+    double t_scale = 1.0;                     // FP32 operator scaling
+    bool have_prev = false;
+
+    // device scalars for the fused step: [0] time [1] dtime [2] taumin_prev
+    // [3] taumin_cur, plus an unsigned-key min slot and the breakthrough step
+    double* scal = nullptr;
+    unsigned long long* tau_key = nullptr;
+    int32_t* bt_step = nullptr;
+    int bt_plane = -1;     // 0-based x index of the breakthrough plane
+    int bt_host = 0;
+
+    wpf::Reducer red;
+    wpf_learned* net = nullptr;
+
+    // kernel timing (wpf_profile_start/stop)
+    int prof_target = -1;
+    std::vector<hipEvent_t> prof_ev;
+    int prof_used = 0;
+    bool prof_open = false;
+};
+
+namespace wpf {
+// Record an event before (begin) or after a launch of kernel `id` when that
+// kernel is the one being timed.
+inline void prof_mark(wpf_ctx* c, int id, bool begin)
+{
+    if (c->prof_target != id) return;
+    const int cap = (int)c->prof_ev.size() / 2;
+    if (begin) {
+        if (c->prof_used < cap) {
+            (void)hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream);
+            c->prof_open = true;
+        }
+    } else if (c->prof_open) {
+        (void)hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream);
+        c->prof_used++;
+        c->prof_open = false;
+    }
+}
+}  // namespace wpf
+
+namespace wpf {
+
+// kernels' host launchers (k_fields.hip)
+int launch_perm(wpf_ctx* c);
+int launch_velocity(wpf_ctx* c);
+int launch_phase_deriv(wpf_ctx* c, double* taumin);
+int launch_concent_deriv(wpf_ctx* c);
+int launch_evolve(wpf_ctx* c, int which, double dtime);
+int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt);
+int launch_export(wpf_ctx* c, int field, double* host);
+int launch_import(wpf_ctx* c, int field, const double* host);
+int launch_diag(wpf_ctx* c, wpf_diag* d);
+int check_breakthrough(wpf_ctx* c, int itime);
+
+// solver (k_solver.hip)
+int solve_pressure(wpf_ctx* c, const wpf_solver_opts* o, wpf_solve_stats* st);
+void free_solver(wpf_ctx* c);
+
+// learned part (k_learned.hip)
+void free_learned(wpf_ctx* c);
+
+// memory helpers
+template <typename T>
+int dev_alloc(Dev<T>& a, const Grid& g)
+{
+    a.n = g.n; a.pad = g.pad;
+    size_t bytes = sizeof(T) * (size_t)(g.n + 2 * g.pad);
+    WPF_HIP(hipMalloc((void**)&a.raw, bytes));
+    WPF_HIP(hipMemset(a.raw, 0, bytes));
+    a.p = a.raw + g.pad;
+    return WPF_OK;
+}
+template <typename T>
+void dev_free(Dev<T>& a)
+{
+    if (a.raw) (void)hipFree(a.raw);
+    a.raw = a.p = nullptr;
+}
+
+int reducer_init(Reducer& r, int slots, int max_blocks);
+void reducer_free(Reducer& r);
+
+}  // namespace wpf
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+#ifdef __HIPCC__
+namespace wpf {
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Order-preserving map of a double onto an unsigned 64-bit key, so that a
+// single atomicMin on the key is a min over doubles (negatives included).
+__device__ __forceinline__ unsigned long long dkey(double v)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dkey_inv(unsigned long long k)
+{
+    unsigned long long u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+// Block-wide sum of NV values per thread, deterministic final reduction by
+// the last block to finish.  blockDim.x*y*z must be a multiple of 64 and at
+// most 1024.  Writes result[slot+v] for v < NV.
+template <int NV>
+__device__ void block_reduce_store(double (&v)[NV], double* partials, unsigned* counter,
+                                   double* result, int nblocks_total, int bid)
+{
+    __shared__ double sh[NV][16];
+    __shared__ bool last;
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthr = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6, nw = nthr >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) sh[k][wid] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int w = 0; w < nw; w++) s += sh[k][w];
+            partials[(size_t)k * nblocks_total + bid] = s;
+        }
+        __threadfence();
+        unsigned done = atomicAdd(counter, 1u);
+        last = (done == (unsigned)nblocks_total - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int b = tid; b < nblocks_total; b += nthr)
+                s += __hip_atomic_load(&partials[(size_t)k * nblocks_total + b],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s = wave_sum(s);
+            if (lane == 0) sh[k][wid] = s;
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                double s = 0.0;
+                for (int w = 0; w < nw; w++) s += sh[k][w];
+                result[k] = s;
+            }
+            atomicExch(counter, 0u);
+        }
+    }
+}
+
+}  // namespace wpf
+#endif

@@ -135,7 +135,8 @@ def main():
     g64.close()
 
     log("headline path (FP32 MG-PCG + FP64 refinement)")
-    fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4)
+    from mizuho_edit_amd._lib import FLAG_EXTRAPOLATE
+    fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, flags=FLAG_EXTRAPOLATE)
     gf, wallf, itsf, state0, prof = run_path(case, fast_opts, a.warmup, a.steps, a.kernel)
     pf = gf.download("pressure")
     gf.close()

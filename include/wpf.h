@@ -196,6 +196,25 @@ int wpf_breakthrough_step(wpf_ctx* ctx, int32_t* step);
  * definition, applied identically to the oracle. */
 int wpf_set_breakthrough_plane(wpf_ctx* ctx, int32_t ix_g);
 
+/* ---- input pipeline (native callers) -------------------------------------
+ * loadConfig (wPFAW3D.f90:264-505) in two calls: wpf_config_load parses the
+ * .cfg and its well file into per-thread staging and returns the parameters,
+ * the well count, the start time (from the field file header when "init"
+ * names one, else 0) and time_total; wpf_config_wells / wpf_config_fields
+ * copy the wells and the initial haloed state (lw, permeab_rock, phase,
+ * concent, as loadConfig + loadField leave them, 1:440-557) into arrays the
+ * caller allocated from those sizes.  Returns WPF_ERR_IO where loadConfig
+ * calls exit(1). */
+int wpf_config_load(const char* cfg_path, wpf_params* p, int32_t* nwell,
+                    double* time_start, double* time_total);
+int wpf_config_wells(wpf_well* wells);
+int wpf_config_fields(int32_t* lw_haloed, double* krock_haloed,
+                      double* phase_haloed, double* concent_haloed);
+/* saveField, wPFAW3D.f90:2096-2141 (haloed host arrays in). */
+int wpf_field_write(const char* path, double time, const wpf_params* p,
+                    const double* krock_haloed, const double* phase_haloed,
+                    const double* concent_haloed);
+
 /* ---- measurement ---------------------------------------------------------
  * Time every launch of one kernel with HIP events on the context's stream
  * between wpf_profile_start and wpf_profile_stop; *total_ms is the summed

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwpf_hip.so")
-SOURCES = ["wpf_api.hip", "k_fields.hip", "k_solver.hip", "k_learned.hip"]
+SOURCES = ["wpf_api.hip", "k_fields.hip", "k_solver.hip", "k_learned.hip", "wpf_io.cpp"]
 ARCH = os.environ.get("WPF_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -42,7 +42,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     procs = []
     for s in SOURCES:
         src = os.path.join(CSRC, s)
-        obj = os.path.join(HERE, "_build", s.replace(".hip", ".o"))
+        obj = os.path.join(HERE, "_build", os.path.splitext(s)[0] + ".o")
         objs.append(obj)
         cmd = ["hipcc", *FLAGS, "-c", src, "-o", obj]
         if verbose:

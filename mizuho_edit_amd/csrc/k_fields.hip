@@ -540,6 +540,9 @@ int launch_diag(wpf_ctx* c, wpf_diag* out)
     hipLaunchKernelGGL(k_diag, grd, blk, 0, c->stream, d, dx, xref, yref, zref, c->phi[c->cur],
                        c->conc[c->cur], c->red.partials, c->red.counter, c->red.result, keys);
     WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, (const double*)c->red.partials,
+                       nb, 2, c->red.result);
+    WPF_HIP(hipGetLastError());
     double sums[2];
     unsigned long long kout[5];
     WPF_HIP(hipMemcpyAsync(sums, c->red.result, sizeof(sums), hipMemcpyDeviceToHost, c->stream));

@@ -28,10 +28,12 @@
 // (blocks are dispatched round-robin over the 8 XCDs) are contiguous and
 // share that XCD's L2.
 #include "wpf_internal.h"
+#include "k_fine.h"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+
 
 namespace wpf {
 
@@ -464,11 +466,6 @@ __global__ void k_set_inj(int n, const int32_t* __restrict__ idx, const double* 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-struct Launch {
-    hipStream_t s;
-    Reducer* red;
-};
-
 // Launch sites that should be timed by wpf_profile_start declare a local
 // `kid_` holding their WPF_KERNEL_* id, hiding this default.
 static const int kid_ = -1;
@@ -479,6 +476,26 @@ static const int kid_ = -1;
         hipLaunchKernelGGL(kern, dim3(blocks_for(n)), dim3(kThreads), 0, c->stream,    \
                            __VA_ARGS__);                                               \
         prof_mark(c, kid_, false);                                                     \
+        WPF_HIP(hipGetLastError());                                                    \
+    } while (0)
+
+#define LAUNCHZ(kern, G, ...)                                                           \
+    do {                                                                               \
+        prof_mark(c, kid_, true);                                                      \
+        hipLaunchKernelGGL(kern, dim3(zg_blocks(G)), dim3(kFX, kFY), 0, c->stream,     \
+                           __VA_ARGS__);                                               \
+        prof_mark(c, kid_, false);                                                     \
+        WPF_HIP(hipGetLastError());                                                    \
+    } while (0)
+
+#define RED(slot) c->red.partials, c->red.counter, c->red.result + (slot)
+
+// sum the per-block partials of the kernel just launched into result[slot..]
+#define FINALIZE(nb, nv, slot)                                                          \
+    do {                                                                               \
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream,              \
+                           (const double*)c->red.partials, (int)(nb), (nv),            \
+                           c->red.result + (slot));                                    \
         WPF_HIP(hipGetLastError());                                                    \
     } while (0)
 
@@ -508,8 +525,6 @@ static void free_level(MGLevel<R>& L)
     dev_free(L.x); dev_free(L.b); dev_free(L.r);
 }
 
-// Build the hierarchy below a fine level: levels[0] is the fine level
-// (its tx/ty/tz/invD point at the fine operator; its D is unused).
 static int target_levels(const Grid& g, int nlevels)
 {
     int nl = 1, nx = g.nx, ny = g.ny, nz = g.nz;
@@ -520,16 +535,17 @@ static int target_levels(const Grid& g, int nlevels)
     return nl;
 }
 
+// Grow or shrink the hierarchy below lv[0] (which the caller owns) to the
+// requested depth.  Appending may move lv[0]: callers take references after.
 template <typename R>
-static int ensure_hierarchy(wpf_ctx* c, std::vector<MGLevel<R>>& lv, int nlevels)
+static int ensure_hierarchy(std::vector<MGLevel<R>>& lv, int nlevels)
 {
-    (void)c;
     const int want = target_levels(lv[0].g, nlevels);
     if ((int)lv.size() == want) return WPF_OK;
     for (size_t l = 1; l < lv.size(); l++) free_level(lv[l]);
     lv.resize(1);
     for (int l = 1; l < want; l++) {
-        const Grid& f = lv[l - 1].g;
+        const Grid f = lv[l - 1].g;
         MGLevel<R> L;
         int rc = alloc_level(L, make_grid((f.nx + 1) / 2, (f.ny + 1) / 2, (f.nz + 1) / 2));
         if (rc) return rc;
@@ -562,54 +578,184 @@ struct MGOpts {
     int coarse_sweeps;
 };
 
-// z = M b on level l (V-cycle).  Level 0 vectors are passed explicitly; on
-// coarse levels the level's own b/x/r are used.
 template <typename R>
-static int vcycle(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, const R* b, R* x, R* tmp,
-                  const MGOpts& o)
+static int coarsest(wpf_ctx* c, MGLevel<R>& L, const R* b, R* x, R* tmp, const MGOpts& o)
 {
-    MGLevel<R>& L = lv[l];
-    const bool fine = (l == 0);
-    LV<R> V = view(L, c->mask, fine);
     const R w = (R)o.omega;
-    if (l + 1 == lv.size()) {
+    const size_t lds = sizeof(R) * 2 * (size_t)(L.g.n + 2 * L.g.plane);
+    if (lds <= 150 * 1024) {
+        static bool attr_set[2] = {false, false};
+        bool& done = attr_set[sizeof(R) == 8];
+        if (!done) {
+            WPF_HIP(hipFuncSetAttribute((const void*)k_coarsest_lds<R>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            WPF_HIP(hipFuncSetAttribute((const void*)k_coarsest_reg<R>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            done = true;
+        }
+        if (L.g.n <= 4 * 1024)
+            hipLaunchKernelGGL(k_coarsest_reg<R>, dim3(1), dim3(1024), lds, c->stream, L.g.nx,
+                               L.g.ny, L.g.nz, (const R*)L.tx.p, (const R*)L.ty.p,
+                               (const R*)L.tz.p, (const R*)L.invD.p, w, o.coarse_sweeps, b, x);
+        else
+            hipLaunchKernelGGL(k_coarsest_lds<R>, dim3(1), dim3(1024), lds, c->stream, L.g.nx,
+                               L.g.ny, L.g.nz, (const R*)L.tx.p, (const R*)L.ty.p,
+                               (const R*)L.tz.p, (const R*)L.invD.p, w, o.coarse_sweeps, b, x);
+    } else {
+        LV<R> V = view(L, nullptr, false);
         hipLaunchKernelGGL((k_coarsest<R>), dim3(1), dim3(1024), 0, c->stream, V, w, o.coarse_sweeps,
                            b, x, tmp);
-        WPF_HIP(hipGetLastError());
-        return WPF_OK;
     }
-    // Damped-Jacobi sweeps ping-pong between two buffers.  The first sweep
-    // (from 0) writes tmp and 2*sweeps-1 more sweeps follow, an odd number
-    // of flips, so the result lands in x.
-    R* cur = tmp;
-    R* oth = x;
-    const int kid_smooth = fine ? WPF_KERNEL_SMOOTH : -1;
+    WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
+// V-cycle on a coarse level l >= 1: x = M_l b, using the level's own arrays.
+template <typename R>
+static int vcycle_coarse(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, const MGOpts& o)
+{
+    MGLevel<R>& L = lv[l];
+    if (l + 1 == lv.size()) return coarsest(c, L, (const R*)L.b.p, L.x.p, L.r.p, o);
+    LV<R> V = view(L, nullptr, false);
+    const R w = (R)o.omega;
+    const R* b = L.b.p;
+    // ping-pong r <-> x; 2*sweeps-1 flips after the first sweep land in x
+    R* cur = L.r.p;
+    R* oth = L.x.p;
     LAUNCH1D(k_smooth0<R>, L.g.n, L.g.n, w, b, L.invD.p, cur);
     for (int s = 1; s < o.sweeps; s++) {
-        const int kid_ = kid_smooth;
-        if (fine) LAUNCH1D((k_smooth<R, true>), L.g.n, V, w, b, (const R*)cur, oth);
-        else LAUNCH1D((k_smooth<R, false>), L.g.n, V, w, b, (const R*)cur, oth);
+        LAUNCH1D((k_smooth<R, false>), L.g.n, V, w, b, (const R*)cur, oth);
         std::swap(cur, oth);
     }
     MGLevel<R>& C = lv[l + 1];
-    {
-        const int kid_ = fine ? WPF_KERNEL_RESTRICT : -1;
-        if (fine) LAUNCH1D((k_resid_restrict<R, true>), C.g.n, V, C.g.nx, C.g.ny, C.g.nz, b, (const R*)cur, C.b.p);
-        else LAUNCH1D((k_resid_restrict<R, false>), C.g.n, V, C.g.nx, C.g.ny, C.g.nz, b, (const R*)cur, C.b.p);
-    }
-    int rc = vcycle(c, lv, l + 1, C.b.p, C.x.p, C.r.p, o);
+    LAUNCH1D((k_resid_restrict<R, false>), C.g.n, V, C.g.nx, C.g.ny, C.g.nz, b, (const R*)cur, C.b.p);
+    int rc = vcycle_coarse(c, lv, l + 1, o);
     if (rc) return rc;
-    {
-        const int kid_ = fine ? WPF_KERNEL_PROLONG : -1;
-        if (fine) LAUNCH1D((k_prolong<R, true>), L.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
-        else LAUNCH1D((k_prolong<R, false>), L.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
-    }
+    LAUNCH1D((k_prolong<R, false>), L.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
     for (int s = 0; s < o.sweeps; s++) {
-        const int kid_ = kid_smooth;
-        if (fine) LAUNCH1D((k_smooth<R, true>), L.g.n, V, w, b, (const R*)cur, oth);
-        else LAUNCH1D((k_smooth<R, false>), L.g.n, V, w, b, (const R*)cur, oth);
+        LAUNCH1D((k_smooth<R, false>), L.g.n, V, w, b, (const R*)cur, oth);
         std::swap(cur, oth);
     }
+    return WPF_OK;
+}
+
+// Fine-level pieces of one PCG solve in precision R.
+template <typename R>
+struct Fine {
+    ZG G;
+    Grid g;
+    const R *tx, *ty, *tz, *invD;
+    R *r, *Z, *T, *pa, *pb, *q;
+};
+
+// z = M r on the fine level, starting from z0 = w invD r already in Z.
+// The result pointer is returned through *zres; (r, z) goes to result slot.
+template <typename R>
+static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R>& F, const MGOpts& o,
+                       int dot_slot, R** zres)
+{
+    const R w = (R)o.omega;
+    R* cur = F.Z;
+    R* oth = F.T;
+    {
+        const int kid_ = WPF_KERNEL_SMOOTH;
+        for (int s = 1; s < o.sweeps; s++) {
+            LAUNCHZ((k_fsmooth<R, false>), F.G, F.G, F.tx, F.ty, F.tz, F.invD, w,
+                    (const R*)F.r, (const R*)cur, oth, RED(dot_slot));
+            std::swap(cur, oth);
+        }
+    }
+    if (lv.size() > 1) {
+        MGLevel<R>& C = lv[1];
+        {
+            const int kid_ = WPF_KERNEL_RESTRICT;
+            LAUNCHZ(k_fresid_restrict<R>, F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask,
+                    (const R*)F.r, (const R*)cur, C.g.nx, C.g.ny, C.b.p);
+        }
+        int rc = vcycle_coarse(c, lv, 1, o);
+        if (rc) return rc;
+        const int kid_ = WPF_KERNEL_PROLONG;
+        LV<R> V;
+        V.tx = F.tx; V.ty = F.ty; V.tz = F.tz; V.D = nullptr; V.invD = F.invD; V.mask = c->mask;
+        V.nx = F.g.nx; V.ny = F.g.ny; V.nz = F.g.nz; V.plane = F.g.plane; V.n = F.g.n;
+        LAUNCH1D((k_prolong<R, true>), F.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
+    }
+    const int kid_ = WPF_KERNEL_SMOOTH;
+    for (int s = 0; s < o.sweeps; s++) {
+        if (s + 1 == o.sweeps) {
+            LAUNCHZ((k_fsmooth<R, true>), F.G, F.G, F.tx, F.ty, F.tz, F.invD, w, (const R*)F.r,
+                    (const R*)cur, oth, RED(dot_slot));
+            FINALIZE(zg_blocks(F.G), 1, dot_slot);
+        } else
+            LAUNCHZ((k_fsmooth<R, false>), F.G, F.G, F.tx, F.ty, F.tz, F.invD, w, (const R*)F.r,
+                    (const R*)cur, oth, RED(dot_slot));
+        std::swap(cur, oth);
+    }
+    *zres = cur;
+    return WPF_OK;
+}
+
+// Preconditioned CG from the residual already in F.r, updating x (type X).
+// mg == nullptr: Jacobi.  Stops when ||invD r|| <= tol or after maxit
+// iterations; returns the iterations done and the last recursive norm.
+template <typename R, typename X>
+static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R>& F, const MGOpts& o,
+                    X* x, double tol, int maxit, int* its_out, double* rn_out)
+{
+    const int64_t n = F.g.n;
+    const R w = mg ? (R)o.omega : (R)1;
+    int rz_cur = S_RZ0, rz_prev = S_RZ1;
+    R* zres = F.Z;
+    LAUNCH1D(k_smooth0<R>, n, n, w, (const R*)F.r, F.invD, F.Z);
+    if (mg) {
+        int rc = vcycle_fine(c, *mg, F, o, rz_cur, &zres);
+        if (rc) return rc;
+    } else {
+        LAUNCH1D(k_dot<R>, n, n, (const R*)F.r, (const R*)F.Z, RED(rz_cur));
+        FINALIZE(blocks_for(n), 1, rz_cur);
+    }
+    R* pc = F.pa;
+    R* pn = F.pb;
+    int its = 0;
+    double rn = -1.0;
+    int first = 1;
+    while (its < maxit) {
+        {
+            const int kid_ = WPF_KERNEL_MATVEC;
+            LAUNCHZ(k_dir_mv<R>, F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (const R*)zres,
+                    (const R*)pc, pn, F.q, (const double*)c->red.result, rz_cur, rz_prev, first,
+                    RED(S_PQ));
+        }
+        FINALIZE(zg_blocks(F.G), 1, S_PQ);
+        std::swap(pc, pn);
+        {
+            const int kid_ = WPF_KERNEL_CG_UPDATE;
+            LAUNCH1D((k_update<R, X>), n, n, (const double*)c->red.result, rz_cur, S_PQ, w, x,
+                     (const R*)pc, F.r, (const R*)F.q, F.invD, F.Z, RED(S_RN));
+        }
+        FINALIZE(blocks_for(n), 2, S_RN);
+        its++;
+        first = 0;
+        int rc = read_results(c, S_RN, 2);
+        if (rc) return rc;
+        rn = std::sqrt(c->red.host[S_RN]);
+        if (!std::isfinite(rn)) {
+            set_error("pressure residual is not finite");
+            return WPF_ERR_NAN;
+        }
+        if (rn <= tol) break;
+        std::swap(rz_cur, rz_prev);
+        if (mg) {
+            rc = vcycle_fine(c, *mg, F, o, rz_cur, &zres);
+            if (rc) return rc;
+        } else {
+            zres = F.Z;
+            WPF_HIP(hipMemcpyAsync(c->red.result + rz_cur, c->red.result + S_RN + 1, sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    *its_out = its;
+    *rn_out = rn;
     return WPF_OK;
 }
 
@@ -627,77 +773,108 @@ static void fill_defaults(const wpf_solver_opts* in, wpf_solver_opts& o)
     if (o.inner_maxit <= 0) o.inner_maxit = 100;
 }
 
-// Prepare x (Dirichlet values) and the right-hand side; returns bnorm.
-static int prepare(wpf_ctx* c, double* bnorm, double** f_out)
+static int ensure_ws(wpf_ctx* c, bool f32)
+{
+    int rc;
+    if (!c->z2.raw) {
+        if ((rc = dev_alloc(c->z2, c->g)) || (rc = dev_alloc(c->pp2, c->g)) ||
+            (rc = dev_alloc(c->psave, c->g)))
+            return rc;
+    }
+    if (c->ninj > 0 && !c->fsrc.raw) {
+        if ((rc = dev_alloc(c->fsrc, c->g))) return rc;
+        hipLaunchKernelGGL(k_set_inj, dim3((c->ninj + 255) / 256), dim3(256), 0, c->stream, c->ninj,
+                           c->inj_idx, c->inj_val, c->fsrc.p);
+        WPF_HIP(hipGetLastError());
+    }
+    if (f32 && c->mg32.empty()) {
+        c->mg32.resize(1);
+        if ((rc = alloc_level(c->mg32[0], c->g))) return rc;
+        if ((rc = dev_alloc(c->r32, c->g)) || (rc = dev_alloc(c->z32, c->g)) ||
+            (rc = dev_alloc(c->p32, c->g)) || (rc = dev_alloc(c->q32, c->g)) ||
+            (rc = dev_alloc(c->x32, c->g)) || (rc = dev_alloc(c->t32, c->g)) ||
+            (rc = dev_alloc(c->p32b, c->g)))
+            return rc;
+    }
+    return WPF_OK;
+}
+
+// Dirichlet values into x, optional extrapolated guess, ||D^-1 b||.
+static int prepare(wpf_ctx* c, const wpf_solver_opts& o, double* bnorm)
 {
     if (c->ndir > 0) {
         hipLaunchKernelGGL(k_set_dirichlet, dim3((c->ndir + 255) / 256), dim3(256), 0, c->stream,
                            c->ndir, c->dir_idx, c->dir_val, c->p.p);
         WPF_HIP(hipGetLastError());
     }
-    double* f = nullptr;
-    if (c->ninj > 0) {
-        // the dense source lives in the fine level's scratch q
-        f = c->q.p;
-        WPF_HIP(hipMemsetAsync(f, 0, sizeof(double) * (size_t)c->g.n, c->stream));
-        hipLaunchKernelGGL(k_set_inj, dim3((c->ninj + 255) / 256), dim3(256), 0, c->stream, c->ninj,
-                           c->inj_idx, c->inj_val, f);
+    if (o.flags & WPF_FLAG_EXTRAPOLATE) {
+        double ratio = 1.0;
+        int valid = c->have_prev ? 1 : 0;
+        if (c->dt_hist[0] > 0 && c->dt_hist[1] > 0) {
+            ratio = c->dt_hist[0] / c->dt_hist[1];
+            if (ratio > 4.0 || ratio < 0.25) valid = 0;
+        }
+        hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)((c->g.n + 255) / 256)), dim3(256), 0,
+                           c->stream, c->g.n, valid, ratio, (const uint8_t*)c->mask, c->p.p,
+                           (const double*)c->xprev.p, c->psave.p);
         WPF_HIP(hipGetLastError());
     }
     MGLevel<double> F;
     F.g = c->g; F.tx = c->tx; F.ty = c->ty; F.tz = c->tz; F.invD = c->invD;
     LV<double> L = view(F, c->mask, true);
-    LAUNCH1D(k_bnorm, c->g.n, L, c->p.p, f, c->red.partials, c->red.counter, c->red.result + S_BN);
+    LAUNCH1D(k_bnorm, c->g.n, L, c->p.p, (const double*)c->fsrc.p, RED(S_BN));
+    FINALIZE(blocks_for(c->g.n), 1, S_BN);
     int rc = read_results(c, S_BN, 1);
     if (rc) return rc;
     *bnorm = std::sqrt(c->red.host[S_BN]);
-    *f_out = f;
     return WPF_OK;
+}
+
+static void finish(wpf_ctx* c, const wpf_solver_opts& o)
+{
+    if (o.flags & WPF_FLAG_EXTRAPOLATE) {
+        std::swap(c->xprev, c->psave);  // xprev now holds the previous solution
+        c->have_prev = true;
+    } else {
+        c->have_prev = false;
+    }
 }
 
 // FP64 PCG with Jacobi (mg == false) or the FP64 V-cycle.
 static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats* st)
 {
     const int64_t n = c->g.n;
-    double bnorm;
-    double* f;
-    int rc = prepare(c, &bnorm, &f);
+    int rc = ensure_ws(c, false);
     if (rc) return rc;
-    if (f) {
-        // q holds f; PCG needs q: move f into xprev (free otherwise)
-        WPF_HIP(hipMemcpyAsync(c->xprev.p, f, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
-                               c->stream));
-        f = c->xprev.p;
-    }
+    double bnorm;
+    if ((rc = prepare(c, o, &bnorm))) return rc;
     const double tol = std::max(o.rtol * bnorm, o.atol);
-    MGLevel<double> F;
-    F.g = c->g; F.tx = c->tx; F.ty = c->ty; F.tz = c->tz; F.invD = c->invD;
-    LV<double> L = view(F, c->mask, true);
-    MGOpts mo{o.mg_sweeps, o.mg_omega, o.mg_coarse_scale, 32};
+    MGLevel<double> F0;
+    F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
+    LV<double> L = view(F0, c->mask, true);
+    MGOpts mo{o.mg_sweeps, o.mg_omega, o.mg_coarse_scale, 48};
     if (mg) {
         if (c->mg64.empty()) c->mg64.resize(1);
-        c->mg64[0].g = c->g;
-        c->mg64[0].tx = c->tx; c->mg64[0].ty = c->ty; c->mg64[0].tz = c->tz;
-        c->mg64[0].invD = c->invD;
-        if ((rc = ensure_hierarchy(c, c->mg64, o.mg_levels))) return rc;
+        c->mg64[0] = F0;  // the fine level is a view of the context's operator
+        if ((rc = ensure_hierarchy(c->mg64, o.mg_levels))) return rc;
         if ((rc = coarsen_all(c, c->mg64))) return rc;
     }
+    Fine<double> F;
+    F.G = make_zg(c->g);
+    F.g = c->g;
+    F.tx = c->tx.p; F.ty = c->ty.p; F.tz = c->tz.p; F.invD = c->invD.p;
+    F.r = c->r.p; F.Z = c->z.p; F.T = c->z2.p; F.pa = c->pp.p; F.pb = c->pp2.p; F.q = c->q.p;
     double* x = c->p.p;
-    double* r = c->r.p;
-    double* z = c->z.p;
-    double* p = c->pp.p;
-    double* q = c->q.p;
-    int its = 0;
+    int its = 0, restarts = 0;
     double rn = 0.0, rn0 = -1.0;
-    int restarts = 0;
     for (;;) {
-        // true residual (and Jacobi z)
         {
             const int kid_ = WPF_KERNEL_TRUE_RESID;
-            LAUNCH1D(k_true_resid, n, L, x, f, r, mg ? nullptr : z, nullptr, 0.0, c->red.partials,
-                     c->red.counter, c->red.result + S_T0);
+            LAUNCH1D(k_true_resid, n, L, (const double*)x, (const double*)c->fsrc.p, F.r,
+                     (double*)nullptr, (float*)nullptr, 0.0, RED(S_T0));
         }
-        if ((rc = read_results(c, S_T0, 2))) return rc;
+        FINALIZE(blocks_for(n), 2, S_T0);
+        if ((rc = read_results(c, S_T0, 1))) return rc;
         rn = std::sqrt(c->red.host[S_T0]);
         if (!std::isfinite(rn)) {
             set_error("pressure residual is not finite");
@@ -705,58 +882,12 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
         }
         if (rn0 < 0) rn0 = rn;
         if (rn <= tol || its >= o.maxit || restarts > 3) break;
-        int rzs = S_RZ0;
-        if (mg) {
-            // z = M r, rz = (r, z)
-            if ((rc = vcycle(c, c->mg64, 0, (const double*)r, z, q, mo))) return rc;
-            LAUNCH1D(k_dot<double>, n, n, (const double*)r, (const double*)z, c->red.partials,
-                     c->red.counter, c->red.result + rzs);
-        } else {
-            WPF_HIP(hipMemcpyAsync(c->red.result + rzs, c->red.result + S_T0 + 1, sizeof(double),
-                                   hipMemcpyDeviceToDevice, c->stream));
-        }
-        LAUNCH1D(k_xpby<double>, n, n, c->red.result, 0, 0, 1, (const double*)z, p);
-        bool conv = false;
-        while (its < o.maxit) {
-            {
-                const int kid_ = WPF_KERNEL_MATVEC;
-                LAUNCH1D((k_matvec_dot<double, true>), n, L, (const double*)p, q, c->red.partials,
-                         c->red.counter, c->red.result + S_PQ);
-            }
-            const int rzn = (rzs == S_RZ0) ? S_RZ1 : S_RZ0;
-            const int kid_ = WPF_KERNEL_CG_UPDATE;
-            if (mg) {
-                LAUNCH1D((k_cg_update<double, double>), n, n, (const double*)c->red.result, rzs, S_PQ,
-                         x, (const double*)p, r, (const double*)q, (const double*)c->invD.p,
-                         (double*)nullptr, c->red.partials, c->red.counter, c->red.result + S_RN);
-            } else {
-                LAUNCH1D((k_cg_update<double, double>), n, n, (const double*)c->red.result, rzs, S_PQ,
-                         x, (const double*)p, r, (const double*)q, (const double*)c->invD.p, z,
-                         c->red.partials, c->red.counter, c->red.result + S_RN);
-            }
-            its++;
-            if ((rc = read_results(c, S_RN, 2))) return rc;
-            rn = std::sqrt(c->red.host[S_RN]);
-            if (!std::isfinite(rn)) {
-                set_error("pressure residual is not finite");
-                return WPF_ERR_NAN;
-            }
-            if (rn <= tol) {
-                conv = true;
-                break;
-            }
-            if (mg) {
-                if ((rc = vcycle(c, c->mg64, 0, (const double*)r, z, q, mo))) return rc;
-                LAUNCH1D(k_dot<double>, n, n, (const double*)r, (const double*)z, c->red.partials,
-                         c->red.counter, c->red.result + rzn);
-            } else {
-                WPF_HIP(hipMemcpyAsync(c->red.result + rzn, c->red.result + S_RN + 1, sizeof(double),
-                                       hipMemcpyDeviceToDevice, c->stream));
-            }
-            LAUNCH1D(k_xpby<double>, n, n, c->red.result, rzn, rzs, 0, (const double*)z, p);
-            rzs = rzn;
-        }
-        (void)conv;
+        int k = 0;
+        double rr = 0.0;
+        rc = pcg_core<double, double>(c, mg ? &c->mg64 : nullptr, F, mo, x, tol, o.maxit - its, &k,
+                                      &rr);
+        if (rc) return rc;
+        its += k;
         restarts++;
     }
     st->iterations = its;
@@ -765,6 +896,7 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
     st->rnorm = rn;
     st->bnorm = bnorm;
     st->converged = rn <= tol;
+    finish(c, o);
     return st->converged ? WPF_OK : WPF_ERR_NOCONV;
 }
 
@@ -772,52 +904,41 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
 static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
 {
     const int64_t n = c->g.n;
-    double bnorm;
-    double* f;
-    int rc = prepare(c, &bnorm, &f);
+    int rc = ensure_ws(c, true);
     if (rc) return rc;
-    if (f) {
-        WPF_HIP(hipMemcpyAsync(c->xprev.p, f, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
-                               c->stream));
-        f = c->xprev.p;
-    }
+    double bnorm;
+    if ((rc = prepare(c, o, &bnorm))) return rc;
     const double tol = std::max(o.rtol * bnorm, o.atol);
-    // FP32 fine level
-    if (c->mg32.empty()) {
-        c->mg32.resize(1);
-        if ((rc = alloc_level(c->mg32[0], c->g))) return rc;
-        if ((rc = dev_alloc(c->r32, c->g)) || (rc = dev_alloc(c->z32, c->g)) ||
-            (rc = dev_alloc(c->p32, c->g)) || (rc = dev_alloc(c->q32, c->g)) ||
-            (rc = dev_alloc(c->x32, c->g)))
-            return rc;
-    }
-    MGLevel<double> F;
-    F.g = c->g; F.tx = c->tx; F.ty = c->ty; F.tz = c->tz; F.invD = c->invD;
-    LV<double> L = view(F, c->mask, true);
+    MGLevel<double> F0;
+    F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
+    LV<double> L = view(F0, c->mask, true);
     const double s = c->t_scale;  // t32 = s t64, so A32 e = s r64 solves A64 e = r64
     // grow the hierarchy first: it appends to mg32, which may move mg32[0]
-    if ((rc = ensure_hierarchy(c, c->mg32, o.mg_levels))) return rc;
-    MGLevel<float>& F32 = c->mg32[0];
-    LAUNCH1D(k_to_f32, n, L, s, F32.tx.p, F32.ty.p, F32.tz.p, F32.invD.p);
+    if ((rc = ensure_hierarchy(c->mg32, o.mg_levels))) return rc;
+    {
+        MGLevel<float>& F32 = c->mg32[0];
+        LAUNCH1D(k_to_f32, n, L, s, F32.tx.p, F32.ty.p, F32.tz.p, F32.invD.p);
+    }
     if ((rc = coarsen_all(c, c->mg32))) return rc;
-    LV<float> L32 = view(F32, c->mask, true);
-    MGOpts mo{o.mg_sweeps, o.mg_omega, o.mg_coarse_scale, 32};
-
-    double* x = c->p.p;
-    float* r = c->r32.p;
-    float* z = c->z32.p;
-    float* p = c->p32.p;
-    float* q = c->q32.p;
+    MGOpts mo{o.mg_sweeps, o.mg_omega, o.mg_coarse_scale, 48};
+    Fine<float> F;
+    F.G = make_zg(c->g);
+    F.g = c->g;
+    F.tx = c->mg32[0].tx.p; F.ty = c->mg32[0].ty.p; F.tz = c->mg32[0].tz.p;
+    F.invD = c->mg32[0].invD.p;
+    F.r = c->r32.p; F.Z = c->z32.p; F.T = c->t32.p; F.pa = c->p32.p; F.pb = c->p32b.p;
+    F.q = c->q32.p;
     float* e = c->x32.p;
+    double* x = c->p.p;
     int its = 0, outer = 0;
     double rn = 0.0, rn0 = -1.0;
     for (;;) {
-        // FP64 true residual and its FP32 image r32 = s r64
         {
             const int kid_ = WPF_KERNEL_TRUE_RESID;
-            LAUNCH1D(k_true_resid, n, L, x, f, (double*)nullptr, (double*)nullptr, r, s,
-                     c->red.partials, c->red.counter, c->red.result + S_T0);
+            LAUNCH1D(k_true_resid, n, L, (const double*)x, (const double*)c->fsrc.p, (double*)nullptr,
+                     (double*)nullptr, F.r, s, RED(S_T0));
         }
+        FINALIZE(blocks_for(n), 2, S_T0);
         if ((rc = read_results(c, S_T0, 1))) return rc;
         rn = std::sqrt(c->red.host[S_T0]);
         if (!std::isfinite(rn)) {
@@ -827,41 +948,15 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
         if (rn0 < 0) rn0 = rn;
         if (rn <= tol || its >= o.maxit || outer >= 50) break;
         outer++;
-        // inner FP32 MG-PCG on A32 e = r32 from e = 0
         LAUNCH1D(k_zero<float>, n, n, e);
-        int rzs = S_RZ0;
-        if ((rc = vcycle(c, c->mg32, 0, (const float*)r, z, q, mo))) return rc;
-        LAUNCH1D(k_dot<float>, n, n, (const float*)r, (const float*)z, c->red.partials,
-                 c->red.counter, c->red.result + rzs);
-        LAUNCH1D(k_xpby<float>, n, n, c->red.result, 0, 0, 1, (const float*)z, p);
-        // the inner residual norm in FP32 units equals the FP64 one (invD32 r32 = invD64 r64)
+        // invD32 r32 = invD64 r64: the inner norm is in the outer units
         const double inner_tol = std::max(o.inner_rtol * rn, 0.5 * tol);
         int k = 0;
-        while (k < o.inner_maxit && its < o.maxit) {
-            {
-                const int kid_ = WPF_KERNEL_MATVEC;
-                LAUNCH1D((k_matvec_dot<float, true>), n, L32, (const float*)p, q, c->red.partials,
-                         c->red.counter, c->red.result + S_PQ);
-            }
-            {
-                const int kid_ = WPF_KERNEL_CG_UPDATE;
-                LAUNCH1D((k_cg_update<float, float>), n, n, (const double*)c->red.result, rzs, S_PQ,
-                         e, (const float*)p, r, (const float*)q, (const float*)F32.invD.p,
-                         (float*)nullptr, c->red.partials, c->red.counter, c->red.result + S_RN);
-            }
-            k++;
-            its++;
-            if ((rc = read_results(c, S_RN, 1))) return rc;
-            const double rin = std::sqrt(c->red.host[S_RN]);
-            if (!std::isfinite(rin)) break;
-            if (rin <= inner_tol) break;
-            const int rzn = (rzs == S_RZ0) ? S_RZ1 : S_RZ0;
-            if ((rc = vcycle(c, c->mg32, 0, (const float*)r, z, q, mo))) return rc;
-            LAUNCH1D(k_dot<float>, n, n, (const float*)r, (const float*)z, c->red.partials,
-                     c->red.counter, c->red.result + rzn);
-            LAUNCH1D(k_xpby<float>, n, n, c->red.result, rzn, rzs, 0, (const float*)z, p);
-            rzs = rzn;
-        }
+        double rr = 0.0;
+        rc = pcg_core<float, float>(c, &c->mg32, F, mo, e, inner_tol,
+                                    std::min(o.inner_maxit, o.maxit - its), &k, &rr);
+        if (rc) return rc;
+        its += k;
         LAUNCH1D(k_add_f32, n, n, (const float*)e, x);
     }
     st->iterations = its;
@@ -870,6 +965,7 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
     st->rnorm = rn;
     st->bnorm = bnorm;
     st->converged = rn <= tol;
+    finish(c, o);
     return st->converged ? WPF_OK : WPF_ERR_NOCONV;
 }
 
@@ -905,6 +1001,8 @@ void free_solver(wpf_ctx* c)
     for (size_t l = 0; l < c->mg32.size(); l++) free_level(c->mg32[l]);
     c->mg32.clear();
     dev_free(c->r32); dev_free(c->z32); dev_free(c->p32); dev_free(c->q32); dev_free(c->x32);
+    dev_free(c->t32); dev_free(c->p32b);
+    dev_free(c->z2); dev_free(c->pp2); dev_free(c->psave); dev_free(c->fsrc);
 }
 
 }  // namespace wpf

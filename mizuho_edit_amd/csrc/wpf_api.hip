@@ -408,6 +408,8 @@ int wpf_evolve_phase(wpf_ctx* c, double dtime)  // :1628-1632
     WPF_CHECK(c, WPF_ERR_ARG, "null context");
     int rc = launch_evolve(c, 0, dtime);
     c->have_perm = false;
+    c->dt_hist[1] = c->dt_hist[0];
+    c->dt_hist[0] = dtime;
     return rc;
 }
 
@@ -464,6 +466,8 @@ int wpf_step(wpf_ctx* c, const wpf_solver_opts* o, int32_t itime, double* time, 
     const int solve_rc = rc;
     rc = launch_step_fields(c, itime, *dtime, false);
     if (rc) return rc;
+    c->dt_hist[1] = c->dt_hist[0];
+    c->dt_hist[0] = *dtime;
     // this step's taumin lives in key (itime % 3)
     unsigned long long k;
     WPF_HIP(hipMemcpyAsync(&k, c->tau_key + (itime % 3), 8, hipMemcpyDeviceToHost, c->stream));

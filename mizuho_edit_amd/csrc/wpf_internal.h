@@ -126,11 +126,13 @@ struct wpf_ctx {
     wpf::Dev<double> r, z, pp, q, xprev;
     std::vector<wpf::MGLevel<double>> mg64;   // levels 1.. (coarse)
     std::vector<wpf::MGLevel<float>> mg32;    // levels 0.. (FP32 copy incl. fine)
-    wpf::Dev<float> r32, z32, p32, q32, x32;
+    wpf::Dev<float> r32, z32, p32, q32, x32, t32, p32b;
+    wpf::Dev<double> z2, pp2, psave, fsrc;    // second buffers, saved p, injection source
     bool have_mg64 = false, have_mg32 = false;
     int mg_levels_built = 0;
     double t_scale = 1.0;                     // FP32 operator scaling
-    bool have_prev = false;
+    bool have_prev = false;                   // xprev holds the previous solution
+    double dt_hist[2] = {0.0, 0.0};           // last two Euler step sizes
 
     // device scalars for the fused step: [0] time [1] dtime [2] taumin_prev
     // [3] taumin_cur, plus an unsigned-key min slot and the breakthrough step
@@ -252,15 +254,19 @@ __device__ __forceinline__ double dkey_inv(unsigned long long k)
     return __longlong_as_double((long long)u);
 }
 
-// Block-wide sum of NV values per thread, deterministic final reduction by
-// the last block to finish.  blockDim.x*y*z must be a multiple of 64 and at
-// most 1024.  Writes result[slot+v] for v < NV.
+// Block-wide sum of NV values per thread, stored as this block's partial
+// partials[v*nblocks_total + bid].  The host follows the kernel with
+// k_finalize, which sums the partials in a fixed order (deterministic) into
+// result[v].  No device-scope fence is issued here: on a multi-XCD part such
+// a fence writes back the whole L2 of the issuing XCD, once per block.
+// blockDim.x*y*z must be a multiple of 64 and at most 1024.
 template <int NV>
 __device__ void block_reduce_store(double (&v)[NV], double* partials, unsigned* counter,
                                    double* result, int nblocks_total, int bid)
 {
+    (void)counter;
+    (void)result;
     __shared__ double sh[NV][16];
-    __shared__ bool last;
     const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
     const int nthr = blockDim.x * blockDim.y * blockDim.z;
     const int lane = tid & 63, wid = tid >> 6, nw = nthr >> 6;
@@ -270,39 +276,31 @@ __device__ void block_reduce_store(double (&v)[NV], double* partials, unsigned* 
         if (lane == 0) sh[k][wid] = s;
     }
     __syncthreads();
-    if (tid == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0.0;
-            for (int w = 0; w < nw; w++) s += sh[k][w];
-            partials[(size_t)k * nblocks_total + bid] = s;
-        }
-        __threadfence();
-        unsigned done = atomicAdd(counter, 1u);
-        last = (done == (unsigned)nblocks_total - 1);
+    if (tid < NV) {
+        double s = 0.0;
+        for (int w = 0; w < nw; w++) s += sh[tid][w];
+        partials[(size_t)tid * nblocks_total + bid] = s;
     }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0.0;
-            for (int b = tid; b < nblocks_total; b += nthr)
-                s += __hip_atomic_load(&partials[(size_t)k * nblocks_total + b],
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s = wave_sum(s);
-            if (lane == 0) sh[k][wid] = s;
-        }
+}
+
+// Sum nv rows of nblocks partials into result[0..nv); one block of 1024.
+static __global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks,
+                                                   int nv, double* __restrict__ result)
+{
+    __shared__ double sh[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int k = 0; k < nv; k++) {
+        double s = 0.0;
+        for (int b = tid; b < nblocks; b += 1024) s += partials[(size_t)k * nblocks + b];
+        s = wave_sum(s);
+        if (lane == 0) sh[wid] = s;
         __syncthreads();
         if (tid == 0) {
-#pragma unroll
-            for (int k = 0; k < NV; k++) {
-                double s = 0.0;
-                for (int w = 0; w < nw; w++) s += sh[k][w];
-                result[k] = s;
-            }
-            atomicExch(counter, 0u);
+            double t = 0.0;
+            for (int w = 0; w < 16; w++) t += sh[w];
+            result[k] = t;
         }
+        __syncthreads();
     }
 }
 

@@ -163,15 +163,19 @@ def test_run_parity_small_workload(gpu, solver):
     tightly: every field within 1e-10 relative, identical dt history and the
     same breakthrough step at plane ix_g = 2 (the first plane past the inlet,
     which the front crosses within the run)."""
+    # 1e-14 sits just above the FP64 floor of the true residual (about 5e-15
+    # relative for this system); solving both sides there keeps the solver
+    # difference well under the 1e-10 bar after 60 coupled steps
+    tight = 1e-14
     case = io.workload("plain-small", seed=1)
     ref = OracleState(case)
-    g = WPFAW3D(case, opts=solver_opts(solver, rtol=TIGHT, maxit=20000, inner_rtol=1e-4))
+    g = WPFAW3D(case, opts=solver_opts(solver, rtol=tight, maxit=20000, inner_rtol=1e-4))
     g.set_breakthrough_plane(2)
     bt_ref = 0
     nsteps = 60
     for s in range(nsteps):
         g.step()
-        ref.run(1, rtol=TIGHT, maxit=20000)
+        ref.run(1, rtol=tight, maxit=20000)
         if not bt_ref and (ref.phase[2, 1:-1, 1:-1][ref.lw[2, 1:-1, 1:-1] == 0] > 0).any():
             bt_ref = ref.itime
         # dt carries taumin, i.e. face velocities, i.e. pressure differences

@@ -131,3 +131,74 @@ def test_no_device_is_an_error_not_a_fallback():
     with pytest.raises(_lib.WpfError) as e:
         WPFAW3D(case)
     assert e.value.status == _lib.WPF_ERR_NODEV
+
+
+def _write_case(tmp_path, grid=(10, 6, 5), seed=4, with_field=True):
+    case = io.workload("plain-small", seed=seed, grid=grid)
+    p = case.params
+    p.time_total = 1.0e30
+    io.write_wells(str(tmp_path / "c.well"), case.wells)
+    if with_field:
+        io.write_field(str(tmp_path / "c.fld"), 0.25, case.krock[1:-1, 1:-1, 1:-1],
+                       case.phase[1:-1, 1:-1, 1:-1], case.concent[1:-1, 1:-1, 1:-1])
+    io.write_config(str(tmp_path / "c.cfg"), p, "c.well", init_file="c.fld" if with_field else None)
+    return case, str(tmp_path / "c.cfg")
+
+
+def test_native_config_loader_matches_python_reader(tmp_path):
+    """wpf_config_load/_wells/_fields (host code of libwpf_hip.so, no GPU
+    needed) against the Python reader and the case that wrote the files."""
+    case, cfg = _write_case(tmp_path)
+    L = _lib.lib()
+    L.wpf_config_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(WpfParams),
+                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]
+    L.wpf_config_wells.argtypes = [ctypes.POINTER(WpfWell)]
+    L.wpf_config_fields.argtypes = [ctypes.c_void_p] * 4
+    cp, nw = WpfParams(), ctypes.c_int32()
+    t0, tt = ctypes.c_double(), ctypes.c_double()
+    assert L.wpf_config_load(cfg.encode(), ctypes.byref(cp), ctypes.byref(nw), ctypes.byref(t0),
+                             ctypes.byref(tt)) == 0
+    ref = case.params.to_c()
+    for name, _ in WpfParams._fields_:
+        assert getattr(cp, name) == getattr(ref, name), name
+    assert nw.value == len(case.wells) and t0.value == 0.25 and tt.value == 1.0e30
+    w = (WpfWell * nw.value)()
+    assert L.wpf_config_wells(w) == 0
+    assert all((w[i].ix_g, w[i].pressure, w[i].concent) ==
+               (case.wells[i].ix_g, case.wells[i].pressure, case.wells[i].concent)
+               for i in range(nw.value))
+    sh = (case.params.nx + 2, case.params.ny + 2, case.params.nz + 2)
+    lw = np.zeros(sh, np.int32, order="F")
+    kr, ph, cc = (np.zeros(sh, order="F") for _ in range(3))
+    assert L.wpf_config_fields(lw.ctypes.data, kr.ctypes.data, ph.ctypes.data, cc.ctypes.data) == 0
+    assert np.array_equal(lw, case.lw)
+    assert np.array_equal(kr, case.krock)
+    assert np.array_equal(ph, case.phase)
+    assert np.array_equal(cc, case.concent)
+    # saveField writes what loadField reads
+    L.wpf_field_write.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.POINTER(WpfParams),
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    out = tmp_path / "o.fld"
+    assert L.wpf_field_write(str(out).encode(), 7.5, ctypes.byref(cp), kr.ctypes.data,
+                             ph.ctypes.data, cc.ctypes.data) == 0
+    t, k2, p2, c2 = io.read_field(str(out), (case.params.nx, case.params.ny, case.params.nz))
+    assert t == 7.5 and np.array_equal(k2, kr[1:-1, 1:-1, 1:-1])
+    # broken input is an error code, not a crash
+    bad = tmp_path / "bad.cfg"
+    bad.write_text("grid = 4 4 4\nnonsense = 3\n")
+    assert L.wpf_config_load(str(bad).encode(), ctypes.byref(cp), ctypes.byref(nw),
+                             ctypes.byref(t0), ctypes.byref(tt)) == 8
+
+
+def test_fortran_driver_reports_missing_gpu(tmp_path):
+    import subprocess
+    drv = os.path.join(os.path.dirname(_lib.LIB_PATH), "fortran", "wpf_driver")
+    if not os.path.exists(drv):
+        pytest.skip("amdflang driver not built")
+    if _lib.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    _, cfg = _write_case(tmp_path)
+    r = subprocess.run([drv, cfg, "2"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "wpf_create" in r.stdout and "no GPU" in r.stdout

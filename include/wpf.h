@@ -84,9 +84,10 @@ typedef struct wpf_well {
 #define WPF_SOLVER_MGCG      1  /* FP64 CG + geometric-multigrid V-cycle     */
 #define WPF_SOLVER_MIXED     2  /* FP32 MG-PCG inside FP64 refinement        */
 
-#define WPF_FLAG_LEARNED_WARM  1  /* learned warm start (needs a net)        */
-#define WPF_FLAG_LEARNED_FCG   2  /* learned stage inside flexible CG        */
+#define WPF_FLAG_LEARNED_WARM  1  /* learned first stage (needs a net)      */
+#define WPF_FLAG_LEARNED_FCG   2  /* same stage, named for flexible CG       */
 #define WPF_FLAG_EXTRAPOLATE   4  /* linear-in-time initial guess            */
+#define WPF_FLAG_CAPTURE       8  /* keep (input, correction) for training   */
 
 typedef struct wpf_solver_opts {
     int32_t solver;       /* WPF_SOLVER_*                                    */
@@ -214,6 +215,38 @@ int wpf_config_fields(int32_t* lw_haloed, double* krock_haloed,
 int wpf_field_write(const char* path, double time, const wpf_params* p,
                     const double* krock_haloed, const double* phase_haloed,
                     const double* concent_haloed);
+
+/* ---- learned pressure correction ------------------------------------------
+ * No counterpart in the reference (its solve is PETSc's).  A 3-D conv
+ * encoder-decoder (4 levels, 16/32/48/64 channels, 3x3x3, BF16 on MFMA with
+ * FP32 accumulation) maps (log k, pressure, Jacobi-scaled residual) at the
+ * start of a solve to a pressure correction.  With WPF_FLAG_LEARNED_WARM and
+ * a network attached, the solver takes that correction as a first search
+ * direction with an exact line search on the FP64 operator, keeps it only if
+ * the FP64 residual drops, and then runs its ordinary iteration to the same
+ * FP64 tolerance: the network can speed a solve up, never decide it.
+ * Grid sizes used with the network must be multiples of 32.
+ * Fields for wpf_net_forward / _gradient / _train are planar FP32
+ * [3][nx*ny*nz] (x fastest); targets [nx*ny*nz]; the output is the level-1
+ * correction at (nx/4, ny/4, nz/4). */
+typedef struct wpf_net wpf_net;
+int64_t wpf_net_num_params(void);
+int wpf_net_create(int32_t seed, wpf_net** out);
+int wpf_net_destroy(wpf_net* net);
+int wpf_net_get_params(wpf_net* net, float* host, int64_t count);
+int wpf_net_set_params(wpf_net* net, const float* host, int64_t count);
+int wpf_net_forward(wpf_net* net, int32_t nx, int32_t ny, int32_t nz,
+                    const float* fields, float* out);
+int wpf_net_gradient(wpf_net* net, int32_t nx, int32_t ny, int32_t nz,
+                     const float* fields, const float* target, double* loss,
+                     float* grad);
+int wpf_net_train(wpf_net* net, int32_t nx, int32_t ny, int32_t nz,
+                  int32_t nsamples, const float* fields, const float* targets,
+                  int32_t epochs, float lr, double* epoch_losses);
+int wpf_attach_net(wpf_ctx* ctx, wpf_net* net);
+/* After a solve with WPF_FLAG_CAPTURE: the network input of its start
+ * state (planar [3][n]) and the correction the solver found (n values). */
+int wpf_capture_get(wpf_ctx* ctx, float* fields, float* target);
 
 /* ---- measurement ---------------------------------------------------------
  * Time every launch of one kernel with HIP events on the context's stream

@@ -32,6 +32,7 @@ KERNEL = {"perm": 0, "step_fields": 1, "smooth": 2, "matvec": 3, "true_resid": 4
 FLAG_LEARNED_WARM = 1
 FLAG_LEARNED_FCG = 2
 FLAG_EXTRAPOLATE = 4
+FLAG_CAPTURE = 8
 
 
 class WpfSolverOpts(ctypes.Structure):
@@ -62,6 +63,7 @@ class WpfError(RuntimeError):
 _lib = None
 P = ctypes.c_void_p
 D = ctypes.POINTER(ctypes.c_double)
+F = ctypes.POINTER(ctypes.c_float)
 I32 = ctypes.c_int32
 
 
@@ -104,6 +106,16 @@ def lib():
         "wpf_set_breakthrough_plane": ([P, I32], I32),
         "wpf_profile_start": ([P, I32], I32),
         "wpf_profile_stop": ([P, D, ctypes.POINTER(I32)], I32),
+        "wpf_net_num_params": ([], ctypes.c_int64),
+        "wpf_net_create": ([I32, ctypes.POINTER(P)], I32),
+        "wpf_net_destroy": ([P], I32),
+        "wpf_net_get_params": ([P, F, ctypes.c_int64], I32),
+        "wpf_net_set_params": ([P, F, ctypes.c_int64], I32),
+        "wpf_net_forward": ([P, I32, I32, I32, F, F], I32),
+        "wpf_net_gradient": ([P, I32, I32, I32, F, F, D, F], I32),
+        "wpf_net_train": ([P, I32, I32, I32, I32, F, F, I32, ctypes.c_float, D], I32),
+        "wpf_attach_net": ([P, P], I32),
+        "wpf_capture_get": ([P, F, F], I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -41,7 +41,7 @@ static constexpr int kThreads = 256;
 static constexpr int kMaxBlocks = 2048;
 
 // result slots: S_RN and S_T0 hold two values each
-enum Slot { S_PQ = 0, S_RN = 1, S_RZ0 = 4, S_RZ1 = 5, S_BN = 6, S_T0 = 8 };
+enum Slot { S_PQ = 0, S_RN = 1, S_RZ0 = 4, S_RZ1 = 5, S_BN = 6, S_T0 = 8, S_LR = 10 };
 
 template <typename R>
 struct LV {
@@ -830,8 +830,108 @@ static int prepare(wpf_ctx* c, const wpf_solver_opts& o, double* bnorm)
     return WPF_OK;
 }
 
+__global__ void __launch_bounds__(256) k_axpy64(int64_t n, double a, const double* __restrict__ d,
+                                                double* __restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + a * d[i];
+}
+__global__ void __launch_bounds__(256) k_cap_target(int64_t n, const double* __restrict__ x,
+                                                    const double* __restrict__ guess,
+                                                    float* __restrict__ t)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) t[i] = (float)(x[i] - guess[i]);
+}
+
+static Fine<double> fine64(wpf_ctx* c)
+{
+    Fine<double> F;
+    F.G = make_zg(c->g);
+    F.g = c->g;
+    F.tx = c->tx.p; F.ty = c->ty.p; F.tz = c->tz.p; F.invD = c->invD.p;
+    F.r = c->r.p; F.Z = c->z.p; F.T = c->z2.p; F.pa = c->pp.p; F.pb = c->pp2.p; F.q = c->q.p;
+    return F;
+}
+
+// FP64 true residual into c->r (norm returned); used by the learned stage and
+// by sample capture.
+static int true_resid64(wpf_ctx* c, double* rn)
+{
+    MGLevel<double> F0;
+    F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
+    LV<double> L = view(F0, c->mask, true);
+    const int64_t n = c->g.n;
+    LAUNCH1D(k_true_resid, n, L, (const double*)c->p.p, (const double*)c->fsrc.p, c->r.p,
+             (double*)nullptr, (float*)nullptr, 0.0, RED(S_T0));
+    FINALIZE(blocks_for(n), 2, S_T0);
+    int rc = read_results(c, S_T0, 1);
+    if (rc) return rc;
+    *rn = std::sqrt(c->red.host[S_T0]);
+    return WPF_OK;
+}
+
+// Start of a solve: optional sample capture and the learned first stage.
+// The stage takes the network's correction d as a search direction with the
+// exact line search alpha = (d, r) / (d, A d) on the FP64 operator and keeps
+// x + alpha d only if the FP64 residual norm drops.
+static int start_stage(wpf_ctx* c, const wpf_solver_opts& o, int* learned)
+{
+    *learned = 0;
+    const bool cap = (o.flags & WPF_FLAG_CAPTURE) != 0;
+    const bool net = (o.flags & (WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG)) && c->net;
+    if (!cap && !net) return WPF_OK;
+    const int64_t n = c->g.n;
+    double rn0;
+    int rc = true_resid64(c, &rn0);
+    if (rc) return rc;
+    if (cap) {
+        if (!c->cap_fields) {
+            WPF_HIP(hipMalloc((void**)&c->cap_fields, sizeof(float) * 3 * (size_t)n));
+            WPF_HIP(hipMalloc((void**)&c->cap_target, sizeof(float) * (size_t)n));
+            WPF_HIP(hipMalloc((void**)&c->cap_guess, sizeof(double) * (size_t)n));
+        }
+        if ((rc = solver_input(c, c->r.p, rn0, c->cap_fields))) return rc;
+        WPF_HIP(hipMemcpyAsync(c->cap_guess, c->p.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
+                               c->stream));
+    }
+    if (!net || !(rn0 > 0.0)) return WPF_OK;
+    Fine<double> F = fine64(c);
+    double* d = c->z2.p;
+    if ((rc = net_correction<double>(c, c->r.p, rn0, d))) return rc;
+    {
+        const int kid_ = WPF_KERNEL_MATVEC;
+        LAUNCHZ(k_dir_mv<double>, F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (const double*)d,
+                (const double*)F.pa, F.pa, F.q, (const double*)c->red.result, S_PQ, S_PQ, 1, RED(S_PQ));
+    }
+    FINALIZE(zg_blocks(F.G), 1, S_PQ);
+    LAUNCH1D(k_dot<double>, n, n, (const double*)c->r.p, (const double*)d, RED(S_LR));
+    FINALIZE(blocks_for(n), 1, S_LR);
+    if ((rc = read_results(c, 0, S_LR + 1))) return rc;
+    const double alpha = c->red.host[S_LR] / c->red.host[S_PQ];
+    if (!std::isfinite(alpha) || !(alpha > 0.0)) return WPF_OK;
+    hipLaunchKernelGGL(k_axpy64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n, alpha,
+                       (const double*)d, c->p.p);
+    WPF_HIP(hipGetLastError());
+    double rn1;
+    if ((rc = true_resid64(c, &rn1))) return rc;
+    if (std::isfinite(rn1) && rn1 < rn0) {
+        *learned = 1;
+    } else {
+        hipLaunchKernelGGL(k_axpy64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n, -alpha,
+                           (const double*)d, c->p.p);
+        WPF_HIP(hipGetLastError());
+    }
+    return WPF_OK;
+}
+
 static void finish(wpf_ctx* c, const wpf_solver_opts& o)
 {
+    if ((o.flags & WPF_FLAG_CAPTURE) && c->cap_target) {
+        hipLaunchKernelGGL(k_cap_target, dim3((unsigned)((c->g.n + 255) / 256)), dim3(256), 0, c->stream,
+                           c->g.n, (const double*)c->p.p, (const double*)c->cap_guess, c->cap_target);
+        c->have_capture = true;
+    }
     if (o.flags & WPF_FLAG_EXTRAPOLATE) {
         std::swap(c->xprev, c->psave);  // xprev now holds the previous solution
         c->have_prev = true;
@@ -848,6 +948,8 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
     if (rc) return rc;
     double bnorm;
     if ((rc = prepare(c, o, &bnorm))) return rc;
+    int learned = 0;
+    if ((rc = start_stage(c, o, &learned))) return rc;
     const double tol = std::max(o.rtol * bnorm, o.atol);
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
@@ -896,6 +998,7 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
     st->rnorm = rn;
     st->bnorm = bnorm;
     st->converged = rn <= tol;
+    st->learned = learned;
     finish(c, o);
     return st->converged ? WPF_OK : WPF_ERR_NOCONV;
 }
@@ -908,6 +1011,8 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
     if (rc) return rc;
     double bnorm;
     if ((rc = prepare(c, o, &bnorm))) return rc;
+    int learned = 0;
+    if ((rc = start_stage(c, o, &learned))) return rc;
     const double tol = std::max(o.rtol * bnorm, o.atol);
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
@@ -965,6 +1070,7 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
     st->rnorm = rn;
     st->bnorm = bnorm;
     st->converged = rn <= tol;
+    st->learned = learned;
     finish(c, o);
     return st->converged ? WPF_OK : WPF_ERR_NOCONV;
 }

@@ -247,6 +247,7 @@ int wpf_destroy(wpf_ctx* c)
     for (int b = 0; b < 2; b++) { F(c->phi[b]); F(c->conc[b]); }
     F(c->krock); F(c->dphidt); F(c->dcdt); F(c->vx); F(c->vy); F(c->vz);
     F(c->scal); F(c->tau_key); F(c->bt_step);
+    F(c->cap_fields); F(c->cap_target); F(c->cap_guess);
     dev_free(c->p); dev_free(c->tx); dev_free(c->ty); dev_free(c->tz); dev_free(c->invD);
     dev_free(c->r); dev_free(c->z); dev_free(c->pp); dev_free(c->q); dev_free(c->xprev);
     reducer_free(c->red);
@@ -508,6 +509,17 @@ int wpf_profile_stop(wpf_ctx* c, double* total_ms, int32_t* count)
     *count = c->prof_used;
     c->prof_target = -1;
     c->prof_used = 0;
+    return WPF_OK;
+}
+
+int wpf_capture_get(wpf_ctx* c, float* fields, float* target)
+{
+    WPF_CHECK(c && fields && target, WPF_ERR_ARG, "null argument");
+    WPF_CHECK(c->have_capture, WPF_ERR_STATE, "no solve with WPF_FLAG_CAPTURE has run");
+    const size_t n = (size_t)c->g.n;
+    WPF_HIP(hipMemcpyAsync(fields, c->cap_fields, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipMemcpyAsync(target, c->cap_target, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipStreamSynchronize(c->stream));
     return WPF_OK;
 }
 

@@ -94,7 +94,8 @@ struct Reducer {
 
 }  // namespace wpf
 
-struct wpf_learned;  // learned pressure-correction network (k_learned.hip)
+// wpf_net (the learned pressure-correction network) is defined in
+// k_learned.hip and declared in wpf.h
 
 struct wpf_ctx {
     int dev = 0;
@@ -133,6 +134,11 @@ struct wpf_ctx {
     double t_scale = 1.0;                     // FP32 operator scaling
     bool have_prev = false;                   // xprev holds the previous solution
     double dt_hist[2] = {0.0, 0.0};           // last two Euler step sizes
+    // training-sample capture (WPF_FLAG_CAPTURE)
+    float* cap_fields = nullptr;              // [3][n]
+    float* cap_target = nullptr;              // [n]
+    double* cap_guess = nullptr;              // [n]
+    bool have_capture = false;
 
     // device scalars for the fused step: [0] time [1] dtime [2] taumin_prev
     // [3] taumin_cur, plus an unsigned-key min slot and the breakthrough step
@@ -143,7 +149,7 @@ struct wpf_ctx {
     int bt_host = 0;
 
     wpf::Reducer red;
-    wpf_learned* net = nullptr;
+    wpf_net* net = nullptr;   // attached network (owned by its creator)
 
     // kernel timing (wpf_profile_start/stop)
     int prof_target = -1;
@@ -192,6 +198,10 @@ void free_solver(wpf_ctx* c);
 
 // learned part (k_learned.hip)
 void free_learned(wpf_ctx* c);
+template <typename R>
+int net_correction(wpf_ctx* c, const double* r64, double rnorm, R* out);
+// network input fields ([3][n] FP32) of the current solver state
+int solver_input(wpf_ctx* c, const double* r64, double rnorm, float* fields);
 
 // memory helpers
 template <typename T>

@@ -1,0 +1,156 @@
+"""The learned pressure correction: host side of the network in
+k_learned.hip (a 3-D conv encoder-decoder, BF16 on MFMA), its training data
+and its training loop.
+
+Training data are (input, correction) pairs the plain solver produces
+itself: with WPF_FLAG_CAPTURE a solve keeps its start state as the network
+input (log k, pressure, Jacobi-scaled residual) and the correction it went on
+to find.  Training runs on the GPU through the library's own forward,
+backward and Adam kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import _lib, io
+from ._lib import check
+
+LAYERS = [("e1", 8, 16), ("e2", 16, 32), ("e3", 32, 48), ("e4", 48, 64),
+          ("d3", 112, 48), ("d2", 80, 32), ("d1", 48, 16), ("out", 16, 1)]
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "pcnet.npz")
+
+
+def _fp(a):
+    return a.ctypes.data_as(_lib.F)
+
+
+def flatten(P: dict) -> np.ndarray:
+    """{name.w: (co, 27, ci), name.b: (co,)} -> the library's flat order."""
+    parts = []
+    for name, ci, co in LAYERS:
+        parts.append(np.asarray(P[name + ".w"], dtype=np.float32).reshape(-1))
+        parts.append(np.asarray(P[name + ".b"], dtype=np.float32).reshape(-1))
+    return np.concatenate(parts)
+
+
+def unflatten(v: np.ndarray) -> dict:
+    P, off = {}, 0
+    for name, ci, co in LAYERS:
+        nw = co * 27 * ci
+        P[name + ".w"] = v[off:off + nw].reshape(co, 27, ci).astype(np.float64)
+        off += nw
+        P[name + ".b"] = v[off:off + co].astype(np.float64)
+        off += co
+    return P
+
+
+class PCNet:
+    def __init__(self, seed: int = 0):
+        h = ctypes.c_void_p()
+        check(_lib.lib().wpf_net_create(seed, ctypes.byref(h)), "wpf_net_create")
+        self._h = h
+        self.nparams = int(_lib.lib().wpf_net_num_params())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().wpf_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_params(self) -> np.ndarray:
+        v = np.zeros(self.nparams, np.float32)
+        check(_lib.lib().wpf_net_get_params(self._h, _fp(v), self.nparams), "get_params")
+        return v
+
+    def set_params(self, v: np.ndarray):
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        check(_lib.lib().wpf_net_set_params(self._h, _fp(v), self.nparams), "set_params")
+
+    def forward(self, fields: np.ndarray) -> np.ndarray:
+        """fields: (3, nx, ny, nz) -> correction at (nx/4, ny/4, nz/4)."""
+        _, nx, ny, nz = fields.shape
+        f = np.ascontiguousarray(np.asarray(fields, np.float32).reshape(3, -1, order="F"))
+        out = np.zeros((nx // 4) * (ny // 4) * (nz // 4), np.float32)
+        check(_lib.lib().wpf_net_forward(self._h, nx, ny, nz, _fp(f), _fp(out)), "net_forward")
+        return out.reshape((nx // 4, ny // 4, nz // 4), order="F")
+
+    def gradient(self, fields, target) -> Tuple[float, np.ndarray]:
+        _, nx, ny, nz = fields.shape
+        f = np.ascontiguousarray(np.asarray(fields, np.float32).reshape(3, -1, order="F"))
+        t = np.ascontiguousarray(np.asarray(target, np.float32).reshape(-1, order="F"))
+        g = np.zeros(self.nparams, np.float32)
+        L = ctypes.c_double()
+        check(_lib.lib().wpf_net_gradient(self._h, nx, ny, nz, _fp(f), _fp(t), ctypes.byref(L), _fp(g)),
+              "net_gradient")
+        return L.value, g
+
+    def train(self, samples: List[Tuple[np.ndarray, np.ndarray]], epochs: int = 10,
+              lr: float = 1e-3) -> np.ndarray:
+        """samples: [(fields (3,nx,ny,nz), target (nx,ny,nz))], one grid size."""
+        _, nx, ny, nz = samples[0][0].shape
+        F = np.ascontiguousarray(np.stack([np.asarray(f, np.float32).reshape(3, -1, order="F")
+                                           for f, _ in samples]))
+        T = np.ascontiguousarray(np.stack([np.asarray(t, np.float32).reshape(-1, order="F")
+                                           for _, t in samples]))
+        losses = np.zeros(epochs)
+        check(_lib.lib().wpf_net_train(self._h, nx, ny, nz, len(samples), _fp(F), _fp(T), epochs,
+                                       ctypes.c_float(lr), losses.ctypes.data_as(_lib.D)), "net_train")
+        return losses
+
+    def save(self, path: str = WEIGHTS):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.savez_compressed(path, params=self.get_params())
+
+    def load(self, path: str = WEIGHTS):
+        with np.load(path, allow_pickle=False) as z:
+            self.set_params(z["params"])
+
+
+def capture_samples(case, nsteps: int, every: int, opts_kw=None, start: int = 3):
+    """Run the plain path on `case`, capturing (input, correction) every
+    `every` steps from step `start` on."""
+    from .wpfaw3d import WPFAW3D, solver_opts
+    kw = dict(rtol=1e-10)
+    kw.update(opts_kw or {})
+    base = kw.pop("flags", _lib.FLAG_EXTRAPOLATE)
+    g = WPFAW3D(case, opts=solver_opts(kw.pop("solver", "mixed"), flags=base, **kw))
+    p = case.params
+    n = p.ncell
+    out = []
+    for s in range(1, nsteps + 1):
+        cap = s >= start and (s - start) % every == 0
+        o = solver_opts("mixed", rtol=1e-10, flags=base | (_lib.FLAG_CAPTURE if cap else 0))
+        g.step(o)
+        if cap:
+            f = np.zeros(3 * n, np.float32)
+            t = np.zeros(n, np.float32)
+            check(_lib.lib().wpf_capture_get(g._ctx, _fp(f), _fp(t)), "capture_get")
+            out.append((f.reshape((3, p.nx, p.ny, p.nz), order="F"),
+                        t.reshape((p.nx, p.ny, p.nz), order="F")))
+    g.close()
+    return out
+
+
+def make_training_set(nreal: int = 8, grid=(64, 64, 64), nsteps: int = 40, every: int = 4,
+                      seed0: int = 100):
+    """Seeded realisations of the plain workload's physics with correlation
+    lengths spread over 2..8 cells."""
+    samples = []
+    for i in range(nreal):
+        corr = 2.0 + 6.0 * (i / max(1, nreal - 1))
+        case = io.workload("plain-small", seed=seed0 + i, grid=grid, corr_len=corr)
+        samples.extend(capture_samples(case, nsteps, every))
+    return samples

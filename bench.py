@@ -66,9 +66,11 @@ def sync():
         pass
 
 
-def run_path(case, opts, warmup, steps, profile_kernel=None):
+def run_path(case, opts, warmup, steps, profile_kernel=None, net=None):
     from mizuho_edit_amd.wpfaw3d import WPFAW3D
     g = WPFAW3D(case, device=0, opts=opts)
+    if net is not None:
+        g.attach_net(net)
     for s in range(warmup):
         rec = g.step()
         log(f"  warmup {s + 1}/{warmup}: its={rec['iterations']} outer={rec['outer']} "
@@ -80,14 +82,17 @@ def run_path(case, opts, warmup, steps, profile_kernel=None):
         g.profile_start(profile_kernel)
     t0 = time.perf_counter()
     its = 0
+    acc = 0
     for s in range(steps):
         rec = g.step()
         its += rec["iterations"]
+        acc += rec["learned"]
         if (s + 1) % max(1, steps // 5) == 0:
             log(f"  step {s + 1}/{steps}: its={rec['iterations']} outer={rec['outer']}")
     sync()
     wall = time.perf_counter() - t0
     prof = g.profile_stop() if profile_kernel else None
+    g.learned_accepted = acc
     return g, wall, its, state, prof
 
 
@@ -114,6 +119,9 @@ def main():
     ap.add_argument("--kernel", default="smooth", help="kernel reported in roofline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--weights", default=None,
+                    help="network weights (.npz); default: the package's trained weights")
+    ap.add_argument("--no-learned", action="store_true", help="fast path without the network")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the hot path on one GPU only; multi-GPU runs are "
@@ -134,11 +142,24 @@ def main():
     p64 = g64.download("pressure")
     g64.close()
 
-    log("headline path (FP32 MG-PCG + FP64 refinement)")
-    from mizuho_edit_amd._lib import FLAG_EXTRAPOLATE
-    fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, flags=FLAG_EXTRAPOLATE)
-    gf, wallf, itsf, state0, prof = run_path(case, fast_opts, a.warmup, a.steps, a.kernel)
+    from mizuho_edit_amd import learned
+    from mizuho_edit_amd._lib import FLAG_EXTRAPOLATE, FLAG_LEARNED_WARM
+    weights = a.weights or learned.WEIGHTS
+    net = None
+    if not a.no_learned:
+        if not os.path.exists(weights):
+            log(f"no network weights at {weights}: run `python -m mizuho_edit_amd.learned train` "
+                "or pass --no-learned")
+            sys.exit(1)
+        net = learned.PCNet()
+        net.load(weights)
+    log("headline path (learned stage + FP32 MG-PCG in FP64 refinement)" if net else
+        "headline path without the network (FP32 MG-PCG in FP64 refinement)")
+    flags = FLAG_EXTRAPOLATE | (FLAG_LEARNED_WARM if net else 0)
+    fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, flags=flags)
+    gf, wallf, itsf, state0, prof = run_path(case, fast_opts, a.warmup, a.steps, a.kernel, net)
     pf = gf.download("pressure")
+    accepted = gf.learned_accepted
     gf.close()
     perr = float(np.abs(pf - p64).max() / np.abs(p64).max())
 
@@ -175,13 +196,16 @@ def main():
         "data": "synthetic: seeded Gaussian random field rock (corr 4 cells), Kozeny-Carman k",
         "config": {"workload": f"wormholing {n}^3, steps {a.warmup + 1}..{a.warmup + a.steps} "
                                f"of the 500-step run", "grid": [n, n, n],
-                   "solver": "FP32 MG-PCG (5 levels, 2 Jacobi sweeps) in FP64 refinement, rtol 1e-10",
+                   "solver": ("learned first stage + " if net else "") +
+                             "FP32 MG-PCG (5 levels, 2 Jacobi sweeps) in FP64 refinement, rtol 1e-10",
                    "parallelism": "1 GPU"},
         "plain_fp64_steps_per_s": sps64,
         "speedup_vs_plain_fp64": sps / sps64,
         "p_rel_err_vs_plain_fp64": perr,
         "cell_updates_per_s": sps * ncell,
         "iterations_per_step": {"plain_fp64": its64 / a.steps, "fast_fp32_inner": itsf / a.steps},
+        "learned_stage": ({"weights": os.path.relpath(weights, ROOT), "accepted_steps": accepted}
+                          if net else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": a.kernel, "launches": count, "avg_us": avg_s * 1e6,

@@ -361,9 +361,12 @@ __global__ void k_loss_grad(int64_t n, double* __restrict__ d, const float* __re
 {
     const double yt = d[0], yy = d[1], tt = d[2];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) d[3] = 1.0 - yt * yt / (yy * tt);
+    // a zero output or target carries no direction: loss 1, no gradient
+    const bool ok = yy > 1e-300 && tt > 1e-300;
+    if (i == 0) d[3] = ok ? 1.0 - yt * yt / (yy * tt) : 1.0;
     if (i < n) {
-        const double a = -2.0 * yt / (yy * tt), b = 2.0 * yt * yt / (yy * yy * tt);
+        const double a = ok ? -2.0 * yt / (yy * tt) : 0.0;
+        const double b = ok ? 2.0 * yt * yt / (yy * yy * tt) : 0.0;
         gy[i] = (float)(a * t[i] + b * y[i]);
     }
 }
@@ -849,9 +852,19 @@ int wpf_net_train(wpf_net* n, int32_t nx, int32_t ny, int32_t nz, int32_t nsampl
 {
     WPF_CHECK(n && fields && targets && nsamples > 0 && epochs > 0, WPF_ERR_ARG, "bad training arguments");
     const size_t nv = (size_t)nx * ny * nz;
+    // a solve that started converged leaves a zero correction: no direction
+    // to learn from, so such samples are left out
+    std::vector<int> use;
+    for (int s = 0; s < nsamples; s++) {
+        const float* t = targets + nv * s;
+        bool any = false;
+        for (size_t i = 0; i < nv && !any; i++) any = t[i] != 0.0f;
+        if (any) use.push_back(s);
+    }
+    WPF_CHECK(!use.empty(), WPF_ERR_ARG, "every training target is zero");
     for (int e = 0; e < epochs; e++) {
         double sum = 0.0;
-        for (int s = 0; s < nsamples; s++) {
+        for (int s : use) {
             int rc = upload_sample(n, nx, ny, nz, fields + 3 * nv * s, targets + nv * s);
             if (rc) return rc;
             double L = 0.0;
@@ -865,7 +878,7 @@ int wpf_net_train(wpf_net* n, int32_t nx, int32_t ny, int32_t nz, int32_t nsampl
             WPF_HIP(hipGetLastError());
             n->packed = false;
         }
-        if (losses) losses[e] = sum / nsamples;
+        if (losses) losses[e] = sum / (double)use.size();
     }
     WPF_HIP(hipStreamSynchronize(n->stream));
     return WPF_OK;

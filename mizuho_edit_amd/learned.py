@@ -138,6 +138,8 @@ def capture_samples(case, nsteps: int, every: int, opts_kw=None, start: int = 3)
             f = np.zeros(3 * n, np.float32)
             t = np.zeros(n, np.float32)
             check(_lib.lib().wpf_capture_get(g._ctx, _fp(f), _fp(t)), "capture_get")
+            if not t.any():
+                continue  # the solve started converged: nothing to learn
             out.append((f.reshape((3, p.nx, p.ny, p.nz), order="F"),
                         t.reshape((p.nx, p.ny, p.nz), order="F")))
     g.close()
@@ -154,3 +156,71 @@ def make_training_set(nreal: int = 8, grid=(64, 64, 64), nsteps: int = 40, every
         case = io.workload("plain-small", seed=seed0 + i, grid=grid, corr_len=corr)
         samples.extend(capture_samples(case, nsteps, every))
     return samples
+
+
+def compare(case, net: Optional["PCNet"], nsteps: int, skip: int = 3, solver: str = "mixed",
+            inner_rtol: float = 1e-4):
+    """Run `case` for nsteps with the extrapolated start, without and with
+    the learned stage.  Returns per-run iterations (steps after `skip`),
+    accepted learned stages, wall seconds and the final pressures."""
+    import time
+    from .wpfaw3d import WPFAW3D, solver_opts
+    out = {}
+    for use in ((False, True) if net is not None else (False,)):
+        g = WPFAW3D(case)
+        if use:
+            g.attach_net(net)
+        flags = _lib.FLAG_EXTRAPOLATE | (_lib.FLAG_LEARNED_WARM if use else 0)
+        o = solver_opts(solver, rtol=1e-10, inner_rtol=inner_rtol, flags=flags)
+        its, acc, wall = [], 0, 0.0
+        for s in range(nsteps):
+            t0 = time.perf_counter()
+            rec = g.step(o)
+            dt = time.perf_counter() - t0
+            if s >= skip:
+                its.append(rec["iterations"])
+                acc += rec["learned"]
+                wall += dt
+        out["net" if use else "plain"] = {"iterations": its, "learned": acc, "seconds": wall,
+                                          "pressure": g.pressure}
+        g.close()
+    return out
+
+
+def main(argv=None):
+    """python -m mizuho_edit_amd.learned train|eval ..."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m mizuho_edit_amd.learned")
+    ap.add_argument("mode", choices=("train", "eval"))
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--nreal", type=int, default=16)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--nsteps", type=int, default=40)
+    ap.add_argument("--every", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args(argv)
+    n = a.grid
+    if a.mode == "train":
+        samples = make_training_set(a.nreal, (n, n, n), a.nsteps, a.every)
+        print(f"{len(samples)} samples at {n}^3", flush=True)
+        net = PCNet(seed=a.seed)
+        losses = net.train(samples, epochs=a.epochs, lr=a.lr)
+        print("loss per epoch:", np.round(losses, 4).tolist(), flush=True)
+        net.save(a.weights)
+        print(f"saved {a.weights}", flush=True)
+    else:
+        net = PCNet()
+        net.load(a.weights)
+        case = io.workload("plain-small", seed=a.seed + 1000, grid=(n, n, n), corr_len=4.0)
+        r = compare(case, net, a.nsteps)
+        p0, p1 = r["plain"].pop("pressure"), r["net"].pop("pressure")
+        r["p_rel_diff"] = float(np.abs(p1 - p0).max() / np.abs(p0).max())
+        print(json.dumps(r), flush=True)
+    net.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -200,6 +200,14 @@ class WPFAW3D:
             self.step(opts)
         return self
 
+    # -- learned pressure correction -------------------------------------------
+    def attach_net(self, net):
+        """Use `net` (a learned.PCNet, kept alive by the caller) for solves
+        whose flags ask for the learned stage; None detaches."""
+        check(_lib.lib().wpf_attach_net(self._ctx, net.handle if net is not None else None),
+              "attach_net")
+        self._net = net
+
     # -- kernel timing ------------------------------------------------------
     def profile_start(self, kernel: str):
         check(_lib.lib().wpf_profile_start(self._ctx, _lib.KERNEL[kernel]), "profile_start")

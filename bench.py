@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--weights", default=None,
                     help="network weights (.npz); default: the package's trained weights")
     ap.add_argument("--no-learned", action="store_true", help="fast path without the network")
+    ap.add_argument("--jacobi", action="store_true",
+                    help="fast path with the plain path's smoother and FP32 faces")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the hot path on one GPU only; multi-GPU runs are "
@@ -143,7 +145,7 @@ def main():
     g64.close()
 
     from mizuho_edit_amd import learned
-    from mizuho_edit_amd._lib import FLAG_EXTRAPOLATE, FLAG_LEARNED_WARM
+    from mizuho_edit_amd._lib import FLAG_CHEBYSHEV, FLAG_EXTRAPOLATE, FLAG_LEARNED_WARM, FLAG_OP16
     weights = a.weights or learned.WEIGHTS
     net = None
     if not a.no_learned:
@@ -156,6 +158,8 @@ def main():
     log("headline path (learned stage + FP32 MG-PCG in FP64 refinement)" if net else
         "headline path without the network (FP32 MG-PCG in FP64 refinement)")
     flags = FLAG_EXTRAPOLATE | (FLAG_LEARNED_WARM if net else 0)
+    if not a.jacobi:
+        flags |= FLAG_CHEBYSHEV | FLAG_OP16
     fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, flags=flags)
     gf, wallf, itsf, state0, prof = run_path(case, fast_opts, a.warmup, a.steps, a.kernel, net)
     pf = gf.download("pressure")
@@ -197,7 +201,10 @@ def main():
         "config": {"workload": f"wormholing {n}^3, steps {a.warmup + 1}..{a.warmup + a.steps} "
                                f"of the 500-step run", "grid": [n, n, n],
                    "solver": ("learned first stage + " if net else "") +
-                             "FP32 MG-PCG (5 levels, 2 Jacobi sweeps) in FP64 refinement, rtol 1e-10",
+                             ("FP32 MG-PCG (5 levels, 2 Jacobi sweeps)" if a.jacobi else
+                              "FP32 MG-PCG (5 levels, degree-2 Chebyshev, FP16-stored fine faces)") +
+                             " in FP64 refinement, rtol 1e-10",
+                   "plain_solver": "FP64 MG-PCG (5 levels, 2 Jacobi sweeps), rtol 1e-10",
                    "parallelism": "1 GPU"},
         "plain_fp64_steps_per_s": sps64,
         "speedup_vs_plain_fp64": sps / sps64,

@@ -88,6 +88,10 @@ typedef struct wpf_well {
 #define WPF_FLAG_LEARNED_FCG   2  /* same stage, named for flexible CG       */
 #define WPF_FLAG_EXTRAPOLATE   4  /* linear-in-time initial guess            */
 #define WPF_FLAG_CAPTURE       8  /* keep (input, correction) for training   */
+#define WPF_FLAG_CHEBYSHEV    16  /* degree-2 Chebyshev smoothing (sweeps 2) */
+#define WPF_FLAG_OP16         32  /* mixed: fine faces stored as FP16        */
+#define WPF_FLAG_F32_STENCIL  64  /* wpf_step: phase/concentration stencils
+                                     in FP32, FP64 fields, update and taumin */
 
 typedef struct wpf_solver_opts {
     int32_t solver;       /* WPF_SOLVER_*                                    */

@@ -33,6 +33,9 @@ FLAG_LEARNED_WARM = 1
 FLAG_LEARNED_FCG = 2
 FLAG_EXTRAPOLATE = 4
 FLAG_CAPTURE = 8
+FLAG_CHEBYSHEV = 16
+FLAG_OP16 = 32
+FLAG_F32_STENCIL = 64
 
 
 class WpfSolverOpts(ctypes.Structure):

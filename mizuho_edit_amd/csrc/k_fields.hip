@@ -375,7 +375,12 @@ struct StepScalars {
     int key_prev, key_cur, key_next;
 };
 
-__global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConsts K, StepScalars S,
+// T is the precision of the phase-field and concentration stencils.  T =
+// double is the reference arithmetic; T = float evaluates the stencils in
+// FP32 (face velocities from the FP64 pressure difference, rounded once)
+// while phi, c, the Euler update and taumin stay FP64.
+template <typename T>
+__global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConstsT<T> K, double dx, StepScalars S,
                                                      const double* __restrict__ phi,
                                                      const double* __restrict__ conc,
                                                      const double* __restrict__ p,
@@ -397,35 +402,38 @@ __global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConsts K, Step
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z * blockDim.z + threadIdx.z;
-    double tau = 1.0e6;
+    T tau = (T)1.0e6;
     if (i < d.nx && j < d.ny && k < d.nz) {
         const Nb n = neighbours(d, i, j, k);
-        const double dx = K.dx;
         const double p0 = p[n.c];
         // face velocities of the six faces (the -face of the first cell of an
         // axis is the zero-gradient boundary face: -(t*0) = -0)
-        const double vxp = face_v(tx[n.c], p[n.xp], p0, dx);
-        const double vyp = face_v(ty[n.c], p[n.yp], p0, dx);
-        const double vzp = face_v(tz[n.c], p[n.zp], p0, dx);
-        const double vxm = (i > 0) ? face_v(tx[n.xm], p0, p[n.xm], dx) : -0.0;
-        const double vym = (j > 0) ? face_v(ty[n.ym], p0, p[n.ym], dx) : -0.0;
-        const double vzm = (k > 0) ? face_v(tz[n.zm], p0, p[n.zm], dx) : -0.0;
-        double s[19];
-        gather19(d, phi, i, j, k, n, s);
+        const T vxp = (T)face_v(tx[n.c], p[n.xp], p0, dx);
+        const T vyp = (T)face_v(ty[n.c], p[n.yp], p0, dx);
+        const T vzp = (T)face_v(tz[n.c], p[n.zp], p0, dx);
+        const T vxm = (T)((i > 0) ? face_v(tx[n.xm], p0, p[n.xm], dx) : -0.0);
+        const T vym = (T)((j > 0) ? face_v(ty[n.ym], p0, p[n.ym], dx) : -0.0);
+        const T vzm = (T)((k > 0) ? face_v(tz[n.zm], p0, p[n.zm], dx) : -0.0);
+        double s64[19];
+        gather19(d, phi, i, j, k, n, s64);
+        T s[19];
+#pragma unroll
+        for (int q = 0; q < 19; q++) s[q] = (T)s64[q];
         const bool well = (mask[n.c] & WPF_MASK_WELL) != 0;
         const double c0 = conc[n.c];
-        const double dph = phase_cell(K, s, c0, vxp, vxm, vyp, vym, vzp, vzm, well, &tau);
-        double dcd = 0.0;
+        const T dph = phase_cell<T>(K, s, (T)c0, vxp, vxm, vyp, vym, vzp, vzm, well, &tau);
+        T dcd = (T)0;
         if (!well) {
-            double cc[7] = {c0, conc[n.xp], conc[n.xm], conc[n.yp], conc[n.ym], conc[n.zp], conc[n.zm]};
-            dcd = concent_cell(K, cc, dph, vxp, vxm, vyp, vym, vzp, vzm);
+            T cc[7] = {(T)c0, (T)conc[n.xp], (T)conc[n.xm], (T)conc[n.yp], (T)conc[n.ym], (T)conc[n.zp],
+                       (T)conc[n.zm]};
+            dcd = concent_cell<T>(K, cc, dph, vxp, vxm, vyp, vym, vzp, vzm);
         }
-        const double phn = s[0] + dph * dt;
+        const double phn = s64[0] + (double)dph * dt;
         phi_out[n.c] = phn;
-        conc_out[n.c] = c0 + dcd * dt;
+        conc_out[n.c] = c0 + (double)dcd * dt;
         if (i == S.bt_plane && !well && phn > 0.0) atomicMin(bt_step, S.itime);
     }
-    block_min_key(tau, &keys[S.key_cur]);
+    block_min_key((double)tau, &keys[S.key_cur]);
     if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
         threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
         // key_next was last read two steps ago: reset it for the next step
@@ -435,7 +443,7 @@ __global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConsts K, Step
     }
 }
 
-int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt)
+int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt, bool f32)
 {
     const Dims d = dims_of(c);
     const PhaseConsts K = make_consts(c->prm);
@@ -452,9 +460,15 @@ int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt)
     S.key_next = (itime + 1) % 3;
     dim3 blk(64, 4, 1);
     prof_mark(c, WPF_KERNEL_STEP_FIELDS, true);
-    hipLaunchKernelGGL(k_step_fields, grid3(d, blk), blk, 0, c->stream, d, K, S, c->phi[c->cur],
-                       c->conc[c->cur], c->p.p, c->tx.p, c->ty.p, c->tz.p, c->mask,
-                       c->phi[1 - c->cur], c->conc[1 - c->cur], c->tau_key, c->scal, c->bt_step);
+    if (f32)
+        hipLaunchKernelGGL(k_step_fields<float>, grid3(d, blk), blk, 0, c->stream, d, consts_as<float>(K),
+                           K.dx, S, c->phi[c->cur], c->conc[c->cur], c->p.p, c->tx.p, c->ty.p, c->tz.p,
+                           c->mask, c->phi[1 - c->cur], c->conc[1 - c->cur], c->tau_key, c->scal,
+                           c->bt_step);
+    else
+        hipLaunchKernelGGL(k_step_fields<double>, grid3(d, blk), blk, 0, c->stream, d, K, K.dx, S,
+                           c->phi[c->cur], c->conc[c->cur], c->p.p, c->tx.p, c->ty.p, c->tz.p, c->mask,
+                           c->phi[1 - c->cur], c->conc[1 - c->cur], c->tau_key, c->scal, c->bt_step);
     prof_mark(c, WPF_KERNEL_STEP_FIELDS, false);
     WPF_HIP(hipGetLastError());
     c->cur = 1 - c->cur;

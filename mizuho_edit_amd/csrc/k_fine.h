@@ -58,11 +58,15 @@ __device__ __forceinline__ bool zm_tile(const ZG& G, int& i, int& j, int& k0, in
     return i < G.nx && j < G.ny;
 }
 
+// The fine operator's face coefficients are stored as C (double, float or
+// _Float16, the last scaled by a power of two so that they are O(1e3)) and
+// used in the vector precision R.
+
 // p_new = z (+ beta p_old), q = A p_new on free cells, reduces (p_new, q).
 // beta = res[num] / res[den]; first != 0 starts a new direction (p = z).
-template <typename R>
-__global__ void __launch_bounds__(256) k_dir_mv(ZG G, const R* __restrict__ tx,
-                                                const R* __restrict__ ty, const R* __restrict__ tz,
+template <typename R, typename C>
+__global__ void __launch_bounds__(256) k_dir_mv(ZG G, const C* __restrict__ tx,
+                                                const C* __restrict__ ty, const C* __restrict__ tz,
                                                 const uint8_t* __restrict__ mask,
                                                 const R* __restrict__ z, const R* __restrict__ pold,
                                                 R* __restrict__ pnew, R* __restrict__ q,
@@ -79,10 +83,11 @@ __global__ void __launch_bounds__(256) k_dir_mv(ZG G, const R* __restrict__ tx,
         auto P = [&](int64_t m) { return first ? z[m] : z[m] + beta * pold[m]; };
         int64_t c = i + (int64_t)G.nx * (j + (int64_t)G.ny * k0);
         R pm = P(c - G.plane), p0 = P(c);
-        R tzm = tz[c - G.plane];
+        R tzm = (R)tz[c - G.plane];
         for (int k = k0; k < k1; k++, c += G.plane) {
             const R pp = P(c + G.plane);
-            const R txp = tx[c], txm = tx[c - 1], typ = ty[c], tym = ty[c - G.nx], tzp = tz[c];
+            const R txp = (R)tx[c], txm = (R)tx[c - 1], typ = (R)ty[c], tym = (R)ty[c - G.nx];
+            const R tzp = (R)tz[c];
             const R d = txp + txm + typ + tym + tzp + tzm;
             const R s = txp * P(c + 1) + txm * P(c - 1) + typ * P(c + G.nx) + tym * P(c - G.nx) +
                         tzp * pp + tzm * pm;
@@ -99,32 +104,74 @@ __global__ void __launch_bounds__(256) k_dir_mv(ZG G, const R* __restrict__ tx,
     block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
 }
 
-// Damped Jacobi on the fine level: y = (1-w) x + w invD (b + N x).  Fixed
-// cells have invD = 0 and x = 0, so they stay 0.  DOT also reduces (b, y).
-template <typename R, bool DOT>
-__global__ void __launch_bounds__(256) k_fsmooth(ZG G, const R* __restrict__ tx,
-                                                 const R* __restrict__ ty, const R* __restrict__ tz,
-                                                 const R* __restrict__ invD, R w,
-                                                 const R* __restrict__ b, const R* __restrict__ x,
-                                                 R* __restrict__ y, double* __restrict__ partials,
-                                                 unsigned* __restrict__ counter,
-                                                 double* __restrict__ result)
+// Coarse-correction operand of a sweep with the prolongation fused in.
+template <typename R>
+struct Pro {
+    const R* xc;      // coarse correction (padded level array)
+    int cnx;
+    int64_t cplane;
+    R cs;             // coarse-correction scale
+};
+
+// v(m) + cs * xc[parent(m)] on free cells; (i, j, k) are m's coordinates.
+// Coordinates one outside the grid map into the pads of xc and mask.
+template <typename R>
+__device__ __forceinline__ R pro_add(const Pro<R>& P, const uint8_t* __restrict__ mask, R v,
+                                     int64_t m, int i, int j, int k)
+{
+    if (!(mask[m] & WPF_MASK_FIXED))
+        v += P.cs * P.xc[(i >> 1) + (int64_t)P.cnx * (j >> 1) + P.cplane * (k >> 1)];
+    return v;
+}
+
+// One smoothing sweep of the fine level,
+//     y = x + a (x - xo) + w invD (b - A x),
+// damped Jacobi for a = 0, a second-order (Chebyshev) step otherwise, with
+// xo the iterate before x (xo == nullptr: 0).  PROX: x is read as
+// x + cs P xc, i.e. the coarse correction of the V-cycle is added on the
+// fly and the prolongation never makes a pass of its own; PROO: the same
+// for xo.  Fixed cells have invD = 0 and x = xo = 0, so they stay 0.  DOT
+// also reduces (b, y).  y may be the array xo points to (each cell reads
+// its own xo before writing y).
+template <typename R, typename C, bool DOT, bool PROX, bool PROO>
+__global__ void __launch_bounds__(256) k_fsweep(ZG G, const C* __restrict__ tx, const C* __restrict__ ty,
+                                                const C* __restrict__ tz, const R* __restrict__ invD,
+                                                const uint8_t* __restrict__ mask, R a, R w,
+                                                const R* __restrict__ b, const R* __restrict__ x,
+                                                const R* xo, Pro<R> P, R* y,
+                                                double* __restrict__ partials,
+                                                unsigned* __restrict__ counter,
+                                                double* __restrict__ result)
 {
     int i, j, k0, k1;
     const bool act = zm_tile(G, i, j, k0, k1);
     double acc[1] = {0.0};
     if (act) {
+        auto X = [&](int64_t m, int ii, int jj, int kk) -> R {
+            const R v = x[m];
+            return PROX ? pro_add(P, mask, v, m, ii, jj, kk) : v;
+        };
         int64_t c = i + (int64_t)G.nx * (j + (int64_t)G.ny * k0);
-        R xm = x[c - G.plane], x0 = x[c];
-        R tzm = tz[c - G.plane];
-        const R w1 = (R)1 - w;
+        R xm = X(c - G.plane, i, j, k0 - 1), x0 = X(c, i, j, k0);
+        R tzm = (R)tz[c - G.plane];
         for (int k = k0; k < k1; k++, c += G.plane) {
-            const R xp = x[c + G.plane];
-            const R tzp = tz[c];
-            const R s = tx[c] * x[c + 1] + tx[c - 1] * x[c - 1] + ty[c] * x[c + G.nx] +
-                        ty[c - G.nx] * x[c - G.nx] + tzp * xp + tzm * xm;
+            const R xp = X(c + G.plane, i, j, k + 1);
+            const R txp = (R)tx[c], txm = (R)tx[c - 1], typ = (R)ty[c], tym = (R)ty[c - G.nx];
+            const R tzp = (R)tz[c];
+            const R d = txp + txm + typ + tym + tzp + tzm;
+            const R s = txp * X(c + 1, i + 1, j, k) + txm * X(c - 1, i - 1, j, k) +
+                        typ * X(c + G.nx, i, j + 1, k) + tym * X(c - G.nx, i, j - 1, k) + tzp * xp +
+                        tzm * xm;
             const R bv = b[c];
-            const R yv = w1 * x0 + w * invD[c] * (bv + s);
+            R yv = x0 + w * invD[c] * (bv - (d * x0 - s));
+            if (a != (R)0) {
+                R ov = (R)0;
+                if (xo) {
+                    ov = xo[c];
+                    if (PROO) ov = pro_add(P, mask, ov, c, i, j, k);
+                }
+                yv += a * (x0 - ov);
+            }
             y[c] = yv;
             if (DOT) acc[0] += (double)bv * (double)yv;
             xm = x0;
@@ -138,10 +185,10 @@ __global__ void __launch_bounds__(256) k_fsmooth(ZG G, const R* __restrict__ tx,
 // Coarse right-hand side bc[I,J,K] = sum over the 2x2x2 children of
 // (b - A x) on free cells.  z pairs are summed in registers, x pairs with a
 // lane shuffle, y pairs through LDS; the (even, even) child writes.
-template <typename R>
-__global__ void __launch_bounds__(256) k_fresid_restrict(ZG G, const R* __restrict__ tx,
-                                                         const R* __restrict__ ty,
-                                                         const R* __restrict__ tz,
+template <typename R, typename C>
+__global__ void __launch_bounds__(256) k_fresid_restrict(ZG G, const C* __restrict__ tx,
+                                                         const C* __restrict__ ty,
+                                                         const C* __restrict__ tz,
                                                          const uint8_t* __restrict__ mask,
                                                          const R* __restrict__ b,
                                                          const R* __restrict__ x, int cnx, int cny,
@@ -152,14 +199,15 @@ __global__ void __launch_bounds__(256) k_fresid_restrict(ZG G, const R* __restri
     const bool act = zm_tile(G, i, j, k0, k1);
     int64_t c = (int64_t)min(i, G.nx - 1) + (int64_t)G.nx * (min(j, G.ny - 1) + (int64_t)G.ny * k0);
     R xm = act ? x[c - G.plane] : (R)0, x0 = act ? x[c] : (R)0;
-    R tzm = act ? tz[c - G.plane] : (R)0;
+    R tzm = act ? (R)tz[c - G.plane] : (R)0;
     const int64_t cplane = (int64_t)cnx * cny;
     for (int k = k0; k < k1; k += 2) {
         R v = (R)0;
         for (int dz = 0; dz < 2; dz++) {
             if (act && k + dz < k1) {
                 const R xp = x[c + G.plane];
-                const R txp = tx[c], txm = tx[c - 1], typ = ty[c], tym = ty[c - G.nx], tzp = tz[c];
+                const R txp = (R)tx[c], txm = (R)tx[c - 1], typ = (R)ty[c], tym = (R)ty[c - G.nx];
+                const R tzp = (R)tz[c];
                 const R d = txp + txm + typ + tym + tzp + tzm;
                 const R s = txp * x[c + 1] + txm * x[c - 1] + typ * x[c + G.nx] + tym * x[c - G.nx] +
                             tzp * xp + tzm * xm;
@@ -175,8 +223,8 @@ __global__ void __launch_bounds__(256) k_fresid_restrict(ZG G, const R* __restri
         __syncthreads();
         if (act && !(i & 1) && !(j & 1)) {
             const R w2 = (threadIdx.y + 1 < kFY) ? sh[threadIdx.y + 1][threadIdx.x] : (R)0;
-            const int64_t C = (i >> 1) + (int64_t)cnx * (j >> 1) + cplane * (k >> 1);
-            bc[C] = v + w2;
+            const int64_t Ci = (i >> 1) + (int64_t)cnx * (j >> 1) + cplane * (k >> 1);
+            bc[Ci] = v + w2;
         }
         __syncthreads();
     }

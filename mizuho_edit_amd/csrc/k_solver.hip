@@ -268,18 +268,22 @@ __global__ void __launch_bounds__(kThreads) k_smooth0(int64_t n, R w, const R* _
     for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = w * invD[c] * b[c];
 }
 
-// damped Jacobi: y = (1-w) x + w invD (b + N x)   (fixed cells: invD = 0,
-// x = 0, so y stays 0)
+// smoothing sweep y = x + a (x - xo) + w invD (b - A x): damped Jacobi for
+// a = 0, a second-order (Chebyshev) step otherwise (xo == nullptr: 0).
+// Cells with invD = 0 (fixed, or empty aggregates) keep x = 0.  y may be
+// the array xo points to.
 template <typename R, bool FINE>
-__global__ void __launch_bounds__(kThreads) k_smooth(LV<R> L, R w, const R* __restrict__ b,
-                                                     const R* __restrict__ x, R* __restrict__ y)
+__global__ void __launch_bounds__(kThreads) k_smooth(LV<R> L, R a, R w, const R* __restrict__ b,
+                                                     const R* __restrict__ x, const R* xo, R* y)
 {
     const Chunk ch = my_chunk(L.n);
     for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
         R d, s;
         row_parts<R, FINE>(L, x, c, d, s);
-        (void)d;
-        y[c] = ((R)1 - w) * x[c] + w * L.invD[c] * (b[c] + s);
+        const R xv = x[c];
+        R yv = xv + w * L.invD[c] * (b[c] - (d * xv - s));
+        if (a != (R)0) yv += a * (xv - (xo ? xo[c] : (R)0));
+        y[c] = yv;
     }
 }
 
@@ -432,6 +436,32 @@ __global__ void __launch_bounds__(kThreads) k_to_f32(LV<double> L, double s, flo
     }
 }
 
+// FP16 fine faces of the inner solve: h = half(float(t64 * s)).  The FP32
+// copy and invD hold exactly the values the FP16 faces decode to, so the
+// coarse levels are the Galerkin operators of the operator the fine level
+// applies.
+__global__ void __launch_bounds__(kThreads) k_to_f16(LV<double> L, double s, _Float16* __restrict__ htx,
+                                                     _Float16* __restrict__ hty,
+                                                     _Float16* __restrict__ htz, float* __restrict__ tx,
+                                                     float* __restrict__ ty, float* __restrict__ tz,
+                                                     float* __restrict__ invD)
+{
+    const Chunk ch = my_chunk(L.n);
+    auto h = [&](double t) { return (_Float16)(float)(t * s); };
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        const _Float16 a = h(L.tx[c]), b = h(L.ty[c]), d = h(L.tz[c]);
+        htx[c] = a;
+        hty[c] = b;
+        htz[c] = d;
+        tx[c] = (float)a;
+        ty[c] = (float)b;
+        tz[c] = (float)d;
+        const float D = (float)a + (float)h(L.tx[c - 1]) + (float)b + (float)h(L.ty[c - L.nx]) + (float)d +
+                        (float)h(L.tz[c - L.plane]);
+        invD[c] = (L.invD[c] == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
+    }
+}
+
 // x64 += e32 (scaled back)
 __global__ void __launch_bounds__(kThreads) k_add_f32(int64_t n, const float* __restrict__ e,
                                                       double* __restrict__ x)
@@ -571,12 +601,39 @@ static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv)
     return WPF_OK;
 }
 
+// Smoother of every level: damped Jacobi (w1 = w2 = omega, a2 = 0) or the
+// degree-2 Chebyshev polynomial in D^-1 A on [0.5, 2] (the spectrum of
+// D^-1 A of these M-matrices lies in [0, 2]): step 1 x1 = x0 + w1 D^-1 r0,
+// step 2 x2 = x1 + a2 (x1 - x0) + w2 D^-1 r1.  The same polynomial before
+// and after the coarse correction keeps the V-cycle symmetric.
 struct MGOpts {
     int sweeps;
-    double omega;
     double cscale;
     int coarse_sweeps;
+    double omega;      // coarsest-level Jacobi weight
+    double w1, a2, w2;
 };
+
+static MGOpts mg_opts(const wpf_solver_opts& o)
+{
+    MGOpts m{o.mg_sweeps, o.mg_coarse_scale, 48, o.mg_omega, o.mg_omega, 0.0, o.mg_omega};
+    if (o.flags & WPF_FLAG_CHEBYSHEV) {
+        const double hi = 2.0, lo = 0.5;
+        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+        const double rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+        m.w1 = 1.0 / theta;
+        m.a2 = rho1 * rho0;
+        m.w2 = 2.0 * rho1 / delta;
+    }
+    return m;
+}
+
+// (a, w) of smoothing step s (0-based) of a pre- or post-smoothing sequence
+static inline void step_coef(const MGOpts& o, int s, double* a, double* w)
+{
+    *a = (s == 1) ? o.a2 : 0.0;
+    *w = (s == 0) ? o.w1 : (s == 1 ? o.w2 : o.omega);
+}
 
 template <typename R>
 static int coarsest(wpf_ctx* c, MGLevel<R>& L, const R* b, R* x, R* tmp, const MGOpts& o)
@@ -617,14 +674,16 @@ static int vcycle_coarse(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, cons
     MGLevel<R>& L = lv[l];
     if (l + 1 == lv.size()) return coarsest(c, L, (const R*)L.b.p, L.x.p, L.r.p, o);
     LV<R> V = view(L, nullptr, false);
-    const R w = (R)o.omega;
     const R* b = L.b.p;
     // ping-pong r <-> x; 2*sweeps-1 flips after the first sweep land in x
     R* cur = L.r.p;
     R* oth = L.x.p;
-    LAUNCH1D(k_smooth0<R>, L.g.n, L.g.n, w, b, L.invD.p, cur);
+    double a, w;
+    LAUNCH1D(k_smooth0<R>, L.g.n, L.g.n, (R)o.w1, b, L.invD.p, cur);
     for (int s = 1; s < o.sweeps; s++) {
-        LAUNCH1D((k_smooth<R, false>), L.g.n, V, w, b, (const R*)cur, oth);
+        step_coef(o, s, &a, &w);
+        // the iterate before step 1 is x0 = 0
+        LAUNCH1D((k_smooth<R, false>), L.g.n, V, (R)a, (R)w, b, (const R*)cur, (const R*)nullptr, oth);
         std::swap(cur, oth);
     }
     MGLevel<R>& C = lv[l + 1];
@@ -633,62 +692,96 @@ static int vcycle_coarse(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, cons
     if (rc) return rc;
     LAUNCH1D((k_prolong<R, false>), L.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
     for (int s = 0; s < o.sweeps; s++) {
-        LAUNCH1D((k_smooth<R, false>), L.g.n, V, w, b, (const R*)cur, oth);
+        step_coef(o, s, &a, &w);
+        // step 1 reads its x0 from oth and overwrites it cell by cell
+        LAUNCH1D((k_smooth<R, false>), L.g.n, V, (R)a, (R)w, b, (const R*)cur, (const R*)oth, oth);
         std::swap(cur, oth);
     }
     return WPF_OK;
 }
 
-// Fine-level pieces of one PCG solve in precision R.
-template <typename R>
+// Fine-level pieces of one PCG solve: vectors in precision R, face
+// coefficients stored as C.
+template <typename R, typename C>
 struct Fine {
     ZG G;
     Grid g;
-    const R *tx, *ty, *tz, *invD;
+    const C *tx, *ty, *tz;
+    const R* invD;
     R *r, *Z, *T, *pa, *pb, *q;
 };
 
-// z = M r on the fine level, starting from z0 = w invD r already in Z.
+#define FINE_OP(F) F.G, F.tx, F.ty, F.tz, F.invD, (const uint8_t*)c->mask
+
+// z = M r on the fine level, starting from z0 = w1 invD r already in Z.
 // The result pointer is returned through *zres; (r, z) goes to result slot.
-template <typename R>
-static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R>& F, const MGOpts& o,
+template <typename R, typename C>
+static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>& F, const MGOpts& o,
                        int dot_slot, R** zres)
 {
-    const R w = (R)o.omega;
     R* cur = F.Z;
     R* oth = F.T;
+    double a, w;
+    Pro<R> none{nullptr, 0, 0, (R)0};
     {
         const int kid_ = WPF_KERNEL_SMOOTH;
         for (int s = 1; s < o.sweeps; s++) {
-            LAUNCHZ((k_fsmooth<R, false>), F.G, F.G, F.tx, F.ty, F.tz, F.invD, w,
-                    (const R*)F.r, (const R*)cur, oth, RED(dot_slot));
+            step_coef(o, s, &a, &w);
+            LAUNCHZ((k_fsweep<R, C, false, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                    (const R*)cur, (const R*)nullptr, none, oth, RED(dot_slot));
             std::swap(cur, oth);
         }
     }
-    if (lv.size() > 1) {
-        MGLevel<R>& C = lv[1];
+    const bool coarse = lv.size() > 1;
+    Pro<R> pro = none;
+    if (coarse) {
+        MGLevel<R>& C1 = lv[1];
         {
             const int kid_ = WPF_KERNEL_RESTRICT;
-            LAUNCHZ(k_fresid_restrict<R>, F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask,
-                    (const R*)F.r, (const R*)cur, C.g.nx, C.g.ny, C.b.p);
+            LAUNCHZ((k_fresid_restrict<R, C>), F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask,
+                    (const R*)F.r, (const R*)cur, C1.g.nx, C1.g.ny, C1.b.p);
         }
         int rc = vcycle_coarse(c, lv, 1, o);
         if (rc) return rc;
-        const int kid_ = WPF_KERNEL_PROLONG;
-        LV<R> V;
-        V.tx = F.tx; V.ty = F.ty; V.tz = F.tz; V.D = nullptr; V.invD = F.invD; V.mask = c->mask;
-        V.nx = F.g.nx; V.ny = F.g.ny; V.nz = F.g.nz; V.plane = F.g.plane; V.n = F.g.n;
-        LAUNCH1D((k_prolong<R, true>), F.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
+        pro = Pro<R>{C1.x.p, C1.g.nx, C1.g.plane, (R)o.cscale};
     }
+    // post-smoothing; the first step adds the coarse correction as it reads
+    // x, the Chebyshev step 1 re-adds it to its x0
     const int kid_ = WPF_KERNEL_SMOOTH;
     for (int s = 0; s < o.sweeps; s++) {
-        if (s + 1 == o.sweeps) {
-            LAUNCHZ((k_fsmooth<R, true>), F.G, F.G, F.tx, F.ty, F.tz, F.invD, w, (const R*)F.r,
-                    (const R*)cur, oth, RED(dot_slot));
-            FINALIZE(zg_blocks(F.G), 1, dot_slot);
-        } else
-            LAUNCHZ((k_fsmooth<R, false>), F.G, F.G, F.tx, F.ty, F.tz, F.invD, w, (const R*)F.r,
-                    (const R*)cur, oth, RED(dot_slot));
+        step_coef(o, s, &a, &w);
+        const bool last = s + 1 == o.sweeps;
+        const R* xo = (const R*)oth;
+        if (s == 0 && last) {
+            if (coarse)
+                LAUNCHZ((k_fsweep<R, C, true, true, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                        (const R*)cur, xo, pro, oth, RED(dot_slot));
+            else
+                LAUNCHZ((k_fsweep<R, C, true, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                        (const R*)cur, xo, none, oth, RED(dot_slot));
+        } else if (s == 0) {
+            if (coarse)
+                LAUNCHZ((k_fsweep<R, C, false, true, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                        (const R*)cur, xo, pro, oth, RED(dot_slot));
+            else
+                LAUNCHZ((k_fsweep<R, C, false, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                        (const R*)cur, xo, none, oth, RED(dot_slot));
+        } else if (s == 1 && a != 0.0 && coarse) {
+            // x0 of this step is the pre-smoothed iterate plus the correction
+            if (last)
+                LAUNCHZ((k_fsweep<R, C, true, false, true>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                        (const R*)cur, xo, pro, oth, RED(dot_slot));
+            else
+                LAUNCHZ((k_fsweep<R, C, false, false, true>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                        (const R*)cur, xo, pro, oth, RED(dot_slot));
+        } else if (last) {
+            LAUNCHZ((k_fsweep<R, C, true, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                    (const R*)cur, xo, none, oth, RED(dot_slot));
+        } else {
+            LAUNCHZ((k_fsweep<R, C, false, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
+                    (const R*)cur, xo, none, oth, RED(dot_slot));
+        }
+        if (last) FINALIZE(zg_blocks(F.G), 1, dot_slot);
         std::swap(cur, oth);
     }
     *zres = cur;
@@ -698,12 +791,12 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R>& F
 // Preconditioned CG from the residual already in F.r, updating x (type X).
 // mg == nullptr: Jacobi.  Stops when ||invD r|| <= tol or after maxit
 // iterations; returns the iterations done and the last recursive norm.
-template <typename R, typename X>
-static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R>& F, const MGOpts& o,
+template <typename R, typename X, typename C>
+static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F, const MGOpts& o,
                     X* x, double tol, int maxit, int* its_out, double* rn_out)
 {
     const int64_t n = F.g.n;
-    const R w = mg ? (R)o.omega : (R)1;
+    const R w = mg ? (R)o.w1 : (R)1;
     int rz_cur = S_RZ0, rz_prev = S_RZ1;
     R* zres = F.Z;
     LAUNCH1D(k_smooth0<R>, n, n, w, (const R*)F.r, F.invD, F.Z);
@@ -722,7 +815,7 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R>& F, c
     while (its < maxit) {
         {
             const int kid_ = WPF_KERNEL_MATVEC;
-            LAUNCHZ(k_dir_mv<R>, F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (const R*)zres,
+            LAUNCHZ((k_dir_mv<R, C>), F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (const R*)zres,
                     (const R*)pc, pn, F.q, (const double*)c->red.result, rz_cur, rz_prev, first,
                     RED(S_PQ));
         }
@@ -844,9 +937,9 @@ __global__ void __launch_bounds__(256) k_cap_target(int64_t n, const double* __r
     if (i < n) t[i] = (float)(x[i] - guess[i]);
 }
 
-static Fine<double> fine64(wpf_ctx* c)
+static Fine<double, double> fine64(wpf_ctx* c)
 {
-    Fine<double> F;
+    Fine<double, double> F;
     F.G = make_zg(c->g);
     F.g = c->g;
     F.tx = c->tx.p; F.ty = c->ty.p; F.tz = c->tz.p; F.invD = c->invD.p;
@@ -896,12 +989,12 @@ static int start_stage(wpf_ctx* c, const wpf_solver_opts& o, int* learned)
                                c->stream));
     }
     if (!net || !(rn0 > 0.0)) return WPF_OK;
-    Fine<double> F = fine64(c);
+    Fine<double, double> F = fine64(c);
     double* d = c->z2.p;
     if ((rc = net_correction<double>(c, c->r.p, rn0, d))) return rc;
     {
         const int kid_ = WPF_KERNEL_MATVEC;
-        LAUNCHZ(k_dir_mv<double>, F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (const double*)d,
+        LAUNCHZ((k_dir_mv<double, double>), F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (const double*)d,
                 (const double*)F.pa, F.pa, F.q, (const double*)c->red.result, S_PQ, S_PQ, 1, RED(S_PQ));
     }
     FINALIZE(zg_blocks(F.G), 1, S_PQ);
@@ -954,18 +1047,14 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
     LV<double> L = view(F0, c->mask, true);
-    MGOpts mo{o.mg_sweeps, o.mg_omega, o.mg_coarse_scale, 48};
+    const MGOpts mo = mg_opts(o);
     if (mg) {
         if (c->mg64.empty()) c->mg64.resize(1);
         c->mg64[0] = F0;  // the fine level is a view of the context's operator
         if ((rc = ensure_hierarchy(c->mg64, o.mg_levels))) return rc;
         if ((rc = coarsen_all(c, c->mg64))) return rc;
     }
-    Fine<double> F;
-    F.G = make_zg(c->g);
-    F.g = c->g;
-    F.tx = c->tx.p; F.ty = c->ty.p; F.tz = c->tz.p; F.invD = c->invD.p;
-    F.r = c->r.p; F.Z = c->z.p; F.T = c->z2.p; F.pa = c->pp.p; F.pb = c->pp2.p; F.q = c->q.p;
+    Fine<double, double> F = fine64(c);
     double* x = c->p.p;
     int its = 0, restarts = 0;
     double rn = 0.0, rn0 = -1.0;
@@ -986,8 +1075,8 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
         if (rn <= tol || its >= o.maxit || restarts > 3) break;
         int k = 0;
         double rr = 0.0;
-        rc = pcg_core<double, double>(c, mg ? &c->mg64 : nullptr, F, mo, x, tol, o.maxit - its, &k,
-                                      &rr);
+        rc = pcg_core<double, double, double>(c, mg ? &c->mg64 : nullptr, F, mo, x, tol, o.maxit - its,
+                                              &k, &rr);
         if (rc) return rc;
         its += k;
         restarts++;
@@ -1017,22 +1106,45 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
     LV<double> L = view(F0, c->mask, true);
-    const double s = c->t_scale;  // t32 = s t64, so A32 e = s r64 solves A64 e = r64
+    // The FP32 system is A' e = r' with A' = s A64 and r' = s r64, which has
+    // the same solution.  s = mu/kwh makes the faces O(1); with FP16 faces a
+    // further power of two puts the largest face near 4096 (sums of six stay
+    // below FP16's maximum, the smallest faces of these rocks stay normal).
+    double s = c->t_scale;
+    const bool op16 = (o.flags & WPF_FLAG_OP16) != 0;
+    if (op16) {
+        const double tmax = std::max(c->prm.permeab_hole, c->krock_max) / c->prm.viscosity * s;
+        if (tmax > 0.0) s *= std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / tmax)));
+        if (!c->h_tx.raw) {
+            if ((rc = dev_alloc(c->h_tx, c->g)) || (rc = dev_alloc(c->h_ty, c->g)) ||
+                (rc = dev_alloc(c->h_tz, c->g)))
+                return rc;
+        }
+    }
     // grow the hierarchy first: it appends to mg32, which may move mg32[0]
     if ((rc = ensure_hierarchy(c->mg32, o.mg_levels))) return rc;
     {
         MGLevel<float>& F32 = c->mg32[0];
-        LAUNCH1D(k_to_f32, n, L, s, F32.tx.p, F32.ty.p, F32.tz.p, F32.invD.p);
+        if (op16)
+            LAUNCH1D(k_to_f16, n, L, s, c->h_tx.p, c->h_ty.p, c->h_tz.p, F32.tx.p, F32.ty.p, F32.tz.p,
+                     F32.invD.p);
+        else
+            LAUNCH1D(k_to_f32, n, L, s, F32.tx.p, F32.ty.p, F32.tz.p, F32.invD.p);
     }
     if ((rc = coarsen_all(c, c->mg32))) return rc;
-    MGOpts mo{o.mg_sweeps, o.mg_omega, o.mg_coarse_scale, 48};
-    Fine<float> F;
+    const MGOpts mo = mg_opts(o);
+    Fine<float, float> F;
     F.G = make_zg(c->g);
     F.g = c->g;
     F.tx = c->mg32[0].tx.p; F.ty = c->mg32[0].ty.p; F.tz = c->mg32[0].tz.p;
     F.invD = c->mg32[0].invD.p;
     F.r = c->r32.p; F.Z = c->z32.p; F.T = c->t32.p; F.pa = c->p32.p; F.pb = c->p32b.p;
     F.q = c->q32.p;
+    Fine<float, _Float16> H;
+    H.G = F.G; H.g = F.g;
+    H.tx = c->h_tx.p; H.ty = c->h_ty.p; H.tz = c->h_tz.p;
+    H.invD = F.invD;
+    H.r = F.r; H.Z = F.Z; H.T = F.T; H.pa = F.pa; H.pb = F.pb; H.q = F.q;
     float* e = c->x32.p;
     double* x = c->p.p;
     int its = 0, outer = 0;
@@ -1058,8 +1170,9 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
         const double inner_tol = std::max(o.inner_rtol * rn, 0.5 * tol);
         int k = 0;
         double rr = 0.0;
-        rc = pcg_core<float, float>(c, &c->mg32, F, mo, e, inner_tol,
-                                    std::min(o.inner_maxit, o.maxit - its), &k, &rr);
+        const int cap = std::min(o.inner_maxit, o.maxit - its);
+        rc = op16 ? pcg_core<float, float, _Float16>(c, &c->mg32, H, mo, e, inner_tol, cap, &k, &rr)
+                  : pcg_core<float, float, float>(c, &c->mg32, F, mo, e, inner_tol, cap, &k, &rr);
         if (rc) return rc;
         its += k;
         LAUNCH1D(k_add_f32, n, n, (const float*)e, x);
@@ -1080,6 +1193,9 @@ int solve_pressure(wpf_ctx* c, const wpf_solver_opts* in, wpf_solve_stats* st)
     WPF_CHECK(c->have_perm, WPF_ERR_STATE, "solvePressure before calcPermeability");
     wpf_solver_opts o;
     fill_defaults(in, o);
+    WPF_CHECK(!(o.flags & WPF_FLAG_CHEBYSHEV) || o.mg_sweeps == 2, WPF_ERR_ARG,
+              "Chebyshev smoothing is the degree-2 polynomial: mg_sweeps must be 2 (got %d)",
+              o.mg_sweeps);
     wpf_solve_stats local{};
     if (!st) st = &local;
     *st = wpf_solve_stats{};
@@ -1108,6 +1224,7 @@ void free_solver(wpf_ctx* c)
     c->mg32.clear();
     dev_free(c->r32); dev_free(c->z32); dev_free(c->p32); dev_free(c->q32); dev_free(c->x32);
     dev_free(c->t32); dev_free(c->p32b);
+    dev_free(c->h_tx); dev_free(c->h_ty); dev_free(c->h_tz);
     dev_free(c->z2); dev_free(c->pp2); dev_free(c->psave); dev_free(c->fsrc);
 }
 

@@ -19,81 +19,100 @@
 
 namespace wpf {
 
-struct PhaseConsts {
-    double dx, two_dx, dx2, four_dx2;
-    double w0, maxdphi, eps, lambda, diffusion, ral, rat, tau_num, alpha;
-    double kwh, inv_mu_unused;
+template <typename T>
+struct PhaseConstsT {
+    T dx, two_dx, dx2, four_dx2;
+    T w0, maxdphi, eps, lambda, diffusion, ral, rat, tau_num, alpha;
+    T kwh, inv_mu_unused;
 };
+using PhaseConsts = PhaseConstsT<double>;
+
+// the same constants rounded to another precision
+template <typename T>
+inline PhaseConstsT<T> consts_as(const PhaseConsts& K)
+{
+    PhaseConstsT<T> k;
+    k.dx = (T)K.dx; k.two_dx = (T)K.two_dx; k.dx2 = (T)K.dx2; k.four_dx2 = (T)K.four_dx2;
+    k.w0 = (T)K.w0; k.maxdphi = (T)K.maxdphi; k.eps = (T)K.eps; k.lambda = (T)K.lambda;
+    k.diffusion = (T)K.diffusion; k.ral = (T)K.ral; k.rat = (T)K.rat; k.tau_num = (T)K.tau_num;
+    k.alpha = (T)K.alpha; k.kwh = (T)K.kwh; k.inv_mu_unused = (T)K.inv_mu_unused;
+    return k;
+}
+
+// |u| beyond which 1 + u*u is not representable: cos(atan u) = 0
+template <typename T> __device__ __forceinline__ T atan_big();
+template <> __device__ __forceinline__ double atan_big<double>() { return 1.0e150; }
+template <> __device__ __forceinline__ float atan_big<float>() { return 1.0e18f; }
 
 // s: 0 c, 1 x+, 2 x-, 3 y+, 4 y-, 5 z+, 6 z-, 7 (x+,y+) 8 (x+,y-)
 // 9 (x-,y+) 10 (x-,y-), 11 (y+,z+) 12 (y-,z+) 13 (y+,z-) 14 (y-,z-),
 // 15 (x+,z+) 16 (x-,z+) 17 (x+,z-) 18 (x-,z-).
 // Returns dphasedt (0 at wells) and tau through *tau.
-__device__ __forceinline__ double phase_cell(const PhaseConsts& K, const double* s, double conc,
-                                             double vxp, double vxm, double vyp, double vym,
-                                             double vzp, double vzm, bool is_well, double* tau_out)
+template <typename T>
+__device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T conc, T vxp, T vxm,
+                                        T vyp, T vym, T vzp, T vzm, bool is_well, T* tau_out)
 {
-    const double gx = (s[1] - s[2]) / K.two_dx;
-    const double gy = (s[3] - s[4]) / K.two_dx;
-    const double gz = (s[5] - s[6]) / K.two_dx;
-    const double dpf = sqrt(gx * gx + gy * gy + gz * gz);
-    double vs = 0.0;
-    if (dpf > 1.0e-6) {
-        const double n1 = gx / dpf, n2 = gy / dpf, n3 = gz / dpf;
-        const double vxc = 0.5 * (vxp + vxm);
-        const double vyc = 0.5 * (vyp + vym);
-        const double vzc = 0.5 * (vzp + vzm);
+    const T gx = (s[1] - s[2]) / K.two_dx;
+    const T gy = (s[3] - s[4]) / K.two_dx;
+    const T gz = (s[5] - s[6]) / K.two_dx;
+    const T dpf = sqrt(gx * gx + gy * gy + gz * gz);
+    T vs = T(0.0);
+    if (dpf > T(1.0e-6)) {
+        const T n1 = gx / dpf, n2 = gy / dpf, n3 = gz / dpf;
+        const T vxc = T(0.5) * (vxp + vxm);
+        const T vyc = T(0.5) * (vyp + vym);
+        const T vzc = T(0.5) * (vzp + vzm);
         vs = vxc * n1 + vyc * n2 + vzc * n3;
     }
-    const double vt = 0.0;
-    double kxx;
-    if (vs >= 0.0)
+    const T vt = T(0.0);
+    T kxx;
+    if (vs >= T(0.0))
         kxx = K.diffusion;
     else
         kxx = K.diffusion + fabs(conc) * K.eps * (K.ral * (vs * vs) + K.rat * (vt * vt))
                                 / sqrt(vs * vs + vt * vt);
-    const double tau = K.tau_num / kxx;
+    const T tau = K.tau_num / kxx;
     *tau_out = tau;
-    if (is_well) return 0.0;
+    if (is_well) return T(0.0);
 
-    const double phix = gx, phiy = gy, phiz = gz;
-    const double phixx = (s[1] - 2.0 * s[0] + s[2]) / K.dx2;
-    const double phiyy = (s[3] - 2.0 * s[0] + s[4]) / K.dx2;
-    const double phizz = (s[5] - 2.0 * s[0] + s[6]) / K.dx2;
-    const double phixy = (s[7] - s[8] - s[9] + s[10]) / K.four_dx2;
-    const double phiyz = (s[11] - s[12] - s[13] + s[14]) / K.four_dx2;
-    const double phizx = (s[15] - s[16] - s[17] + s[18]) / K.four_dx2;
-    if (phix == 0.0 && phiy == 0.0 && phiz == 0.0 && phixx == 0.0 && phiyy == 0.0 &&
-        phizz == 0.0 && phixy == 0.0 && phiyz == 0.0 && phizx == 0.0)
-        return 0.0;
+    const T phix = gx, phiy = gy, phiz = gz;
+    const T phixx = (s[1] - T(2.0) * s[0] + s[2]) / K.dx2;
+    const T phiyy = (s[3] - T(2.0) * s[0] + s[4]) / K.dx2;
+    const T phizz = (s[5] - T(2.0) * s[0] + s[6]) / K.dx2;
+    const T phixy = (s[7] - s[8] - s[9] + s[10]) / K.four_dx2;
+    const T phiyz = (s[11] - s[12] - s[13] + s[14]) / K.four_dx2;
+    const T phizx = (s[15] - s[16] - s[17] + s[18]) / K.four_dx2;
+    if (phix == T(0.0) && phiy == T(0.0) && phiz == T(0.0) && phixx == T(0.0) && phiyy == T(0.0) &&
+        phizz == T(0.0) && phixy == T(0.0) && phiyz == T(0.0) && phizx == T(0.0))
+        return T(0.0);
 
     // theta = atan(u), phi_ = acos(v)
-    const double u = phiy / (phix + 1.0e-20);
-    double c, sn;
-    if (fabs(u) < 1.0e150) {
-        c = 1.0 / sqrt(1.0 + u * u);
+    const T u = phiy / (phix + T(1.0e-20));
+    T c, sn;
+    if (fabs(u) < atan_big<T>()) {
+        c = T(1.0) / sqrt(T(1.0) + u * u);
         sn = u * c;
     } else {
-        c = 0.0;
-        sn = u > 0.0 ? 1.0 : -1.0;
+        c = T(0.0);
+        sn = u > T(0.0) ? T(1.0) : -T(1.0);
     }
-    double v = phiz / (dpf + 1.0e-20);
-    v = fmin(fmax(v, -1.0), 1.0);
-    const double sphi = sqrt((1.0 - v) * (1.0 + v));
-    const double c2p = 2.0 * (v * v) - 1.0;          // cos(2 phi_)
-    const double c4phi = 2.0 * (c2p * c2p) - 1.0;    // cos(4 phi_)
-    const double s4phi = 4.0 * v * sphi * c2p;       // sin(4 phi_)
+    T v = phiz / (dpf + T(1.0e-20));
+    v = fmin(fmax(v, -T(1.0)), T(1.0));
+    const T sphi = sqrt((T(1.0) - v) * (T(1.0) + v));
+    const T c2p = T(2.0) * (v * v) - T(1.0);          // cos(2 phi_)
+    const T c4phi = T(2.0) * (c2p * c2p) - T(1.0);    // cos(4 phi_)
+    const T s4phi = T(4.0) * v * sphi * c2p;       // sin(4 phi_)
 
-    const double eps = K.eps, w0 = K.w0;
-    const double c2 = c * c, s2 = sn * sn, c4 = c2 * c2, s4 = s2 * s2;
-    const double w = w0 * (1.0 + eps * (7.0 * c4 - 6.0 * c2 + s4 * c4phi));
+    const T eps = K.eps, w0 = K.w0;
+    const T c2 = c * c, s2 = sn * sn, c4 = c2 * c2, s4 = s2 * s2;
+    const T w = w0 * (T(1.0) + eps * (T(7.0) * c4 - T(6.0) * c2 + s4 * c4phi));
 
-    const double ph = s[0];
-    const double f1 = (1.0 - ph * ph) * (ph - K.lambda * conc);
-    const double lap = (+s[1] + s[2] + s[3] + s[4] + s[5] + s[6] - 6.0 * s[0]) / K.dx2;
-    const double f3 = w * w * lap;
+    const T ph = s[0];
+    const T f1 = (T(1.0) - ph * ph) * (ph - K.lambda * conc);
+    const T lap = (+s[1] + s[2] + s[3] + s[4] + s[5] + s[6] - T(6.0) * s[0]) / K.dx2;
+    const T f3 = w * w * lap;
 
-    const double md = K.maxdphi;
+    const T md = K.maxdphi;
     const bool gate_t = (phix > md || phiy > md);
     const bool gate_p = gate_t || phiz > md;
     if (!gate_p) {
@@ -102,77 +121,77 @@ __device__ __forceinline__ double phase_cell(const PhaseConsts& K, const double*
         return (f1 + f3) / tau;
     }
 
-    const double cos2t = c2 - s2;  // cos(2 theta)
-    const double dwdtheta = w0 * (4.0 * eps * sn * c * (-7.0 * c2 + 3.0 + s2 * c4phi));
-    const double ddwdtheta =
-        w0 * (4.0 * eps * (cos2t * (-7.0 * c2 + 3.0 + s2 * c4phi) + 2.0 * s2 * c2 * (7.0 + c4phi)));
-    const double dwdphi_ = w0 * (-4.0 * eps * s4 * s4phi);
-    const double ddwdphi_ = w0 * (-16.0 * eps * s4 * c4phi);
-    const double ddwdthetadphi_ = w0 * (-16.0 * eps * (s2 * sn) * c * s4phi);
+    const T cos2t = c2 - s2;  // cos(2 theta)
+    const T dwdtheta = w0 * (T(4.0) * eps * sn * c * (-T(7.0) * c2 + T(3.0) + s2 * c4phi));
+    const T ddwdtheta =
+        w0 * (T(4.0) * eps * (cos2t * (-T(7.0) * c2 + T(3.0) + s2 * c4phi) + T(2.0) * s2 * c2 * (T(7.0) + c4phi)));
+    const T dwdphi_ = w0 * (-T(4.0) * eps * s4 * s4phi);
+    const T ddwdphi_ = w0 * (-T(16.0) * eps * s4 * c4phi);
+    const T ddwdthetadphi_ = w0 * (-T(16.0) * eps * (s2 * sn) * c * s4phi);
 
-    double dthetadx = 0.0, dthetady = 0.0, dthetadz = 0.0;
-    double dthetaddpdx = 0.0, dthetaddpdy = 0.0;
+    T dthetadx = T(0.0), dthetady = T(0.0), dthetadz = T(0.0);
+    T dthetaddpdx = T(0.0), dthetaddpdy = T(0.0);
     if (gate_t) {
-        const double den = phix * phix + phiy * phiy + 1.0e-20;
+        const T den = phix * phix + phiy * phiy + T(1.0e-20);
         dthetadx = (phixy * phix - phiy * phixx) / den;
         dthetady = (phiyy * phix - phiy * phixy) / den;
         dthetadz = (phiyz * phix - phiy * phizx) / den;
-        const double g2 = phix * phix + phiy * phiy + phiz * phiz;
-        const double r2 = (phix * phix + phiy * phiy) + 1.0e-20;
+        const T g2 = phix * phix + phiy * phiy + phiz * phiz;
+        const T r2 = (phix * phix + phiy * phiy) + T(1.0e-20);
         dthetaddpdx = -(phiy * g2 / r2);
         dthetaddpdy = phix * g2 / r2;
     }
-    const double rxy = sqrt(phix * phix + phiy * phiy);
-    const double den3 = (phix * phix + phiy * phiy + phiz * phiz) * rxy + 1.0e-20;
-    const double pxy2 = phix * phix + phiy * phiy;
-    const double dphi_dx = (phiz * (phix * phixx + phiy * phixy) - pxy2 * phizx) / den3;
-    const double dphi_dy = (phiz * (phix * phixy + phiy * phiyy) - pxy2 * phiyz) / den3;
-    const double dphi_dz = (phiz * (phix * phizx + phiy * phiyz) - pxy2 * phizz) / den3;
-    const double dthetaddpdz = 0.0;
-    const double dphi_ddpdx = phix * phiz / (rxy * dpf + 1.0e-20);
-    const double dphi_ddpdy = phiy * phiz / (rxy * dpf + 1.0e-20);
-    const double dphi_ddpdz = -(rxy / (dpf + 1.0e-20));
+    const T rxy = sqrt(phix * phix + phiy * phiy);
+    const T den3 = (phix * phix + phiy * phiy + phiz * phiz) * rxy + T(1.0e-20);
+    const T pxy2 = phix * phix + phiy * phiy;
+    const T dphi_dx = (phiz * (phix * phixx + phiy * phixy) - pxy2 * phizx) / den3;
+    const T dphi_dy = (phiz * (phix * phixy + phiy * phiyy) - pxy2 * phiyz) / den3;
+    const T dphi_dz = (phiz * (phix * phizx + phiy * phiyz) - pxy2 * phizz) / den3;
+    const T dthetaddpdz = T(0.0);
+    const T dphi_ddpdx = phix * phiz / (rxy * dpf + T(1.0e-20));
+    const T dphi_ddpdy = phiy * phiz / (rxy * dpf + T(1.0e-20));
+    const T dphi_ddpdz = -(rxy / (dpf + T(1.0e-20)));
 
-    const double dwdx = dwdtheta * dthetadx + dwdphi_ * dphi_dx;
-    const double dwdy = dwdtheta * dthetady + dwdphi_ * dphi_dy;
-    const double dwdz = dwdtheta * dthetadz + dwdphi_ * dphi_dz;
-    const double ddwdthetax = ddwdtheta * dthetadx + ddwdthetadphi_ * dphi_dx;
-    const double ddwdthetay = ddwdtheta * dthetady + ddwdthetadphi_ * dphi_dy;
-    const double ddwdthetaz = ddwdtheta * dthetadz + ddwdthetadphi_ * dphi_dz;
-    const double ddwdphi_x = ddwdphi_ * dphi_dx + ddwdthetadphi_ * dthetadx;
-    const double ddwdphi_y = ddwdphi_ * dphi_dy + ddwdthetadphi_ * dthetady;
-    const double ddwdphi_z = ddwdphi_ * dphi_dz + ddwdthetadphi_ * dthetadz;
+    const T dwdx = dwdtheta * dthetadx + dwdphi_ * dphi_dx;
+    const T dwdy = dwdtheta * dthetady + dwdphi_ * dphi_dy;
+    const T dwdz = dwdtheta * dthetadz + dwdphi_ * dphi_dz;
+    const T ddwdthetax = ddwdtheta * dthetadx + ddwdthetadphi_ * dphi_dx;
+    const T ddwdthetay = ddwdtheta * dthetady + ddwdthetadphi_ * dphi_dy;
+    const T ddwdthetaz = ddwdtheta * dthetadz + ddwdthetadphi_ * dphi_dz;
+    const T ddwdphi_x = ddwdphi_ * dphi_dx + ddwdthetadphi_ * dthetadx;
+    const T ddwdphi_y = ddwdphi_ * dphi_dy + ddwdthetadphi_ * dthetady;
+    const T ddwdphi_z = ddwdphi_ * dphi_dz + ddwdthetadphi_ * dthetadz;
 
-    const double f2 = 2.0 * w * (dwdx * phix + dwdy * phiy + dwdz * phiz);
-    const double dpf2 = dpf * dpf;
-    const double f4 = dwdx * (dwdtheta * dthetaddpdx * dpf2 + dwdphi_ * dphi_ddpdx * dpf2)
+    const T f2 = T(2.0) * w * (dwdx * phix + dwdy * phiy + dwdz * phiz);
+    const T dpf2 = dpf * dpf;
+    const T f4 = dwdx * (dwdtheta * dthetaddpdx * dpf2 + dwdphi_ * dphi_ddpdx * dpf2)
                     + w * (ddwdthetax * dthetaddpdx * dpf2 + ddwdphi_x * dphi_ddpdx * dpf2);
-    const double f5 = dwdy * (dwdtheta * dthetaddpdy * dpf2 + dwdphi_ * dphi_ddpdy * dpf2)
+    const T f5 = dwdy * (dwdtheta * dthetaddpdy * dpf2 + dwdphi_ * dphi_ddpdy * dpf2)
                     + w * (ddwdthetay * dthetaddpdy * dpf2 + ddwdphi_y * dphi_ddpdy * dpf2);
-    const double f6 = dwdz * (dwdtheta * dthetaddpdz * dpf2 + dwdphi_ * dphi_ddpdz * dpf2)
+    const T f6 = dwdz * (dwdtheta * dthetaddpdz * dpf2 + dwdphi_ * dphi_ddpdz * dpf2)
                     + w * (ddwdthetaz * dthetaddpdz * dpf2 + ddwdphi_z * dphi_ddpdz * dpf2);
-    const double f7 = 0.0;
+    const T f7 = T(0.0);
     return (f1 + f2 + f3 + f4 + f5 + f6 + f7) / tau;
 }
 
 // calcConcentDeriv for one non-well cell, wPFAW3D.f90:1566-1622.
 // cc: 0 c, 1 x+, 2 x-, 3 y+, 4 y-, 5 z+, 6 z-.
-__device__ __forceinline__ double concent_cell(const PhaseConsts& K, const double* cc,
-                                               double dphidt, double vxp, double vxm, double vyp,
-                                               double vym, double vzp, double vzm)
+template <typename T>
+__device__ __forceinline__ T concent_cell(const PhaseConstsT<T>& K, const T* cc, T dphidt, T vxp,
+                                          T vxm, T vyp, T vym, T vzp, T vzm)
 {
-    const double lap = (+cc[1] + cc[2] + cc[3] + cc[4] + cc[5] + cc[6] - 6.0 * cc[0]) / K.dx2;
-    const double c1 = K.diffusion * lap;
-    const double c2 = K.alpha * dphidt;
-    const double c3 = 0.0;
-    const double fxp = vxp * (vxp > 0.0 ? cc[0] : cc[1]);
-    const double fxm = vxm * (vxm > 0.0 ? cc[2] : cc[0]);
-    const double fyp = vyp * (vyp > 0.0 ? cc[0] : cc[3]);
-    const double fym = vym * (vym > 0.0 ? cc[4] : cc[0]);
-    const double fzp = vzp * (vzp > 0.0 ? cc[0] : cc[5]);
-    const double fzm = vzm * (vzm > 0.0 ? cc[6] : cc[0]);
-    const double div = (fxp - fxm) / K.dx + (fyp - fym) / K.dx + (fzp - fzm) / K.dx;
-    const double c7 = -div;
+    const T lap = (+cc[1] + cc[2] + cc[3] + cc[4] + cc[5] + cc[6] - T(6.0) * cc[0]) / K.dx2;
+    const T c1 = K.diffusion * lap;
+    const T c2 = K.alpha * dphidt;
+    const T c3 = T(0.0);
+    const T fxp = vxp * (vxp > T(0.0) ? cc[0] : cc[1]);
+    const T fxm = vxm * (vxm > T(0.0) ? cc[2] : cc[0]);
+    const T fyp = vyp * (vyp > T(0.0) ? cc[0] : cc[3]);
+    const T fym = vym * (vym > T(0.0) ? cc[4] : cc[0]);
+    const T fzp = vzp * (vzp > T(0.0) ? cc[0] : cc[5]);
+    const T fzm = vzm * (vzm > T(0.0) ? cc[6] : cc[0]);
+    const T div = (fxp - fxm) / K.dx + (fyp - fym) / K.dx + (fzp - fzm) / K.dx;
+    const T c7 = -div;
     return c1 + c2 + c3 + c7;
 }
 

@@ -151,7 +151,9 @@ int wpf_create(const wpf_params* p, const int32_t* lw_haloed, const wpf_well* we
                 }
     }
     int rc;
-    WPF_HIP(hipMalloc((void**)&c->mask, g.n));
+    WPF_HIP(hipMalloc((void**)&c->mask_raw, g.n + 2 * g.pad));
+    WPF_HIP(hipMemset(c->mask_raw, 0, g.n + 2 * g.pad));
+    c->mask = c->mask_raw + g.pad;
     WPF_HIP(hipMemcpy(c->mask, mask.data(), g.n, hipMemcpyHostToDevice));
     c->ndir = (int)didx.size();
     c->ninj = (int)iidx.size();
@@ -243,7 +245,7 @@ int wpf_destroy(wpf_ctx* c)
     free_learned(c);
     free_solver(c);
     auto F = [](void* q) { if (q) (void)hipFree(q); };
-    F(c->mask); F(c->dir_idx); F(c->dir_val); F(c->inj_idx); F(c->inj_val);
+    F(c->mask_raw); F(c->dir_idx); F(c->dir_val); F(c->inj_idx); F(c->inj_val);
     for (int b = 0; b < 2; b++) { F(c->phi[b]); F(c->conc[b]); }
     F(c->krock); F(c->dphidt); F(c->dcdt); F(c->vx); F(c->vy); F(c->vz);
     F(c->scal); F(c->tau_key); F(c->bt_step);
@@ -285,6 +287,11 @@ int wpf_upload(wpf_ctx* c, int32_t field, const double* h)
         }
     WPF_HIP(hipMemcpyAsync(d, tmp.data(), sizeof(double) * g.n, hipMemcpyHostToDevice, c->stream));
     WPF_HIP(hipStreamSynchronize(c->stream));
+    if (field == WPF_FIELD_PERMEAB_ROCK) {
+        double m = 0.0;
+        for (double v : tmp) m = std::max(m, v);
+        c->krock_max = m;
+    }
     if (field == WPF_FIELD_PHASE || field == WPF_FIELD_PERMEAB_ROCK) c->have_perm = false;
     if (field == WPF_FIELD_PRESSURE) c->have_velocity = false;
     return WPF_OK;
@@ -465,7 +472,7 @@ int wpf_step(wpf_ctx* c, const wpf_solver_opts* o, int32_t itime, double* time, 
     rc = solve_pressure(c, o, st);
     if (rc && rc != WPF_ERR_NOCONV) return rc;
     const int solve_rc = rc;
-    rc = launch_step_fields(c, itime, *dtime, false);
+    rc = launch_step_fields(c, itime, *dtime, false, (o->flags & WPF_FLAG_F32_STENCIL) != 0);
     if (rc) return rc;
     c->dt_hist[1] = c->dt_hist[0];
     c->dt_hist[0] = *dtime;

@@ -109,7 +109,8 @@ struct wpf_ctx {
     int ndir = 0, ninj = 0;
     int32_t* dir_idx = nullptr; double* dir_val = nullptr;
     int32_t* inj_idx = nullptr; double* inj_val = nullptr;
-    uint8_t* mask = nullptr;
+    uint8_t* mask = nullptr;        // padded like the solver vectors (pad bytes 0)
+    uint8_t* mask_raw = nullptr;
 
     // state fields (interior)
     double* phi[2] = {nullptr, nullptr};
@@ -128,6 +129,8 @@ struct wpf_ctx {
     std::vector<wpf::MGLevel<double>> mg64;   // levels 1.. (coarse)
     std::vector<wpf::MGLevel<float>> mg32;    // levels 0.. (FP32 copy incl. fine)
     wpf::Dev<float> r32, z32, p32, q32, x32, t32, p32b;
+    wpf::Dev<_Float16> h_tx, h_ty, h_tz;      // FP16 fine faces of the inner solve
+    double krock_max = 0.0;                   // largest uploaded rock permeability
     wpf::Dev<double> z2, pp2, psave, fsrc;    // second buffers, saved p, injection source
     bool have_mg64 = false, have_mg32 = false;
     int mg_levels_built = 0;
@@ -186,7 +189,7 @@ int launch_velocity(wpf_ctx* c);
 int launch_phase_deriv(wpf_ctx* c, double* taumin);
 int launch_concent_deriv(wpf_ctx* c);
 int launch_evolve(wpf_ctx* c, int which, double dtime);
-int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt);
+int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt, bool f32);
 int launch_export(wpf_ctx* c, int field, double* host);
 int launch_import(wpf_ctx* c, int field, const double* host);
 int launch_diag(wpf_ctx* c, wpf_diag* d);

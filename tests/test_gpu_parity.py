@@ -15,7 +15,7 @@ into the GPU context.  Tolerances:
 import numpy as np
 import pytest
 
-from mizuho_edit_amd import io
+from mizuho_edit_amd import _lib, io
 from mizuho_edit_amd.wpfaw3d import WPFAW3D, solver_opts
 from oracle.oracle import OracleState
 
@@ -120,15 +120,20 @@ def test_evolve_bit_exact(gpu):
     g.close()
 
 
-@pytest.mark.parametrize("solver", ["cg", "mgcg", "mixed"])
-def test_pressure_solve(gpu, solver):
+CHEB, OP16 = _lib.FLAG_CHEBYSHEV, _lib.FLAG_OP16
+
+
+@pytest.mark.parametrize("solver,flags", [("cg", 0), ("mgcg", 0), ("mixed", 0), ("mgcg", CHEB),
+                                          ("mixed", CHEB), ("mixed", OP16), ("mixed", CHEB | OP16)])
+def test_pressure_solve(gpu, solver, flags):
     case, st = _state(grid=(32, 16, 16), steps=4)
     g = _gpu_from(case, st)
     # perturb the initial guess so every solver has work to do
     p0 = st.pressure.copy()
     g.upload("pressure", p0 * 0.5)
     g.calcPermeability()
-    info = g.solvePressure(solver_opts(solver, rtol=TIGHT, maxit=20000, inner_rtol=1e-4))
+    info = g.solvePressure(solver_opts(solver, rtol=TIGHT, maxit=20000, inner_rtol=1e-4,
+                                       flags=flags))
     assert info["converged"] == 1
     st.calc_permeability()
     st.pressure[...] = p0 * 0.5
@@ -157,8 +162,8 @@ def test_fused_step_matches_procedures(gpu):
     b.close()
 
 
-@pytest.mark.parametrize("solver", ["mgcg", "mixed"])
-def test_run_parity_small_workload(gpu, solver):
+@pytest.mark.parametrize("solver,flags", [("mgcg", 0), ("mixed", 0), ("mixed", CHEB | OP16)])
+def test_run_parity_small_workload(gpu, solver, flags):
     """64x32x32 workload, 60 steps from the loaded state, both sides solving
     tightly: every field within 1e-10 relative, identical dt history and the
     same breakthrough step at plane ix_g = 2 (the first plane past the inlet,
@@ -169,7 +174,8 @@ def test_run_parity_small_workload(gpu, solver):
     tight = 1e-14
     case = io.workload("plain-small", seed=1)
     ref = OracleState(case)
-    g = WPFAW3D(case, opts=solver_opts(solver, rtol=tight, maxit=20000, inner_rtol=1e-4))
+    g = WPFAW3D(case, opts=solver_opts(solver, rtol=tight, maxit=20000, inner_rtol=1e-4,
+                                       flags=flags))
     g.set_breakthrough_plane(2)
     bt_ref = 0
     nsteps = 60

@@ -39,14 +39,25 @@ METRIC = "time-steps/s at 256³: learned+FP32-refined vs plain FP64 GPU, ≤1e-6
 HBM_PEAK_GBS = 8000.0
 
 # algorithmic bytes per cell (each array element the math must read or
-# write once per launch; mask = 1 byte)
-def bytes_per_cell(kernel: str, r: int) -> float:
+# write once per launch; mask = 1 byte); r = vector precision, h = stored
+# face precision of the fine operator
+def smoother_bytes(r: int, h: int, cheb: bool = True) -> float:
+    """Mean over the three sweeps of one V-cycle on the fine level (degree-2
+    Chebyshev, coarse correction fused into the post-smoothing):
+      pre-smoothing step 2  x, b, invD, 3 faces -> y          4r + 3h
+      post step 1           + mask + coarse correction        4r + 3h + 1 + r/8
+      post step 2           + mask + coarse correction + x0   5r + 3h + 1 + r/8"""
+    base = 4 * r + 3 * h
+    last = base + r + 1 + r / 8 if cheb else base  # Jacobi's step 2 needs no x0
+    return (base + (base + 1 + r / 8) + last) / 3
+
+
+def bytes_per_cell(kernel: str, r: int, h: int, cheb: bool = True) -> float:
     return {
-        "smooth": 7 * r + 1,        # x, b, tx, ty, tz, invD, mask -> y
-        "matvec": 5 * r + 1,        # p, tx, ty, tz, mask -> q
-        "cg_update": 7 * r,         # x, p, r, q, invD -> x, r
-        "restrict": 5 * r + 1 + r / 8,
-        "prolong": 2 * r + 1 + r / 8,
+        "smooth": smoother_bytes(r, h, cheb),
+        "matvec": 4 * r + 3 * h + 1,  # z, p_old, 3 faces, mask -> p_new, q
+        "cg_update": 8 * r,           # x, p, r, q, invD -> x, r, z
+        "restrict": 2 * r + 3 * h + 1 + r / 8,
         "true_resid": 5 * 8 + 1 + r,  # x, t x3, invD (FP64), mask -> r (precision r)
         "step_fields": 6 * 8 + 1 + 2 * 8,  # phi, c, p, t x3, mask -> phi', c'
         "perm": 2 * 8 + 1 + 4 * 8,  # phi, krock, mask -> tx, ty, tz, invD
@@ -110,6 +121,28 @@ def cpu_baseline(case, state, nthreads):
     return wall, int(st.hist[-1][2])
 
 
+def calibrate_copy(reps=20):
+    """A plain 1 GiB device-to-device copy (torch's copy kernel), timed with
+    events: the reference point for HBM bandwidth and for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run."""
+    import torch
+    n = (1 << 30) // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30,
+                      "us_per_copy": 1e3 * ms, "GBps_read_plus_write": 2 * (1 << 30) / (ms * 1e-3) / 1e9}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +157,12 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="fast path without the network")
     ap.add_argument("--jacobi", action="store_true",
                     help="fast path with the plain path's smoother and FP32 faces")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="time a plain 1 GiB copy kernel and exit")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the hot path on one GPU only; multi-GPU runs are "
               "not part of it (see DESIGN.md for how the grid would shard)", file=sys.stderr)
@@ -171,8 +209,7 @@ def main():
     sps64 = a.steps / wall64
     total_ms, count = prof
     avg_s = total_ms / 1e3 / max(count, 1)
-    bpc = bytes_per_cell(a.kernel, 4 if a.kernel in ("smooth", "matvec", "cg_update", "restrict",
-                                                      "prolong", "true_resid") else 8)
+    bpc = bytes_per_cell(a.kernel, 4, 4 if a.jacobi else 2, not a.jacobi)
     achieved = bpc * ncell / avg_s / 1e9 if count else 0.0
 
     cpu = None

@@ -684,6 +684,22 @@ namespace wpf {
 // Net input from the solver state: planar [log10 k scaled, p / p_scale,
 // invD r / rms] at full resolution, then the forward pass; the correction d
 // (0 on fixed cells) is written to out at full resolution.
+// the same from the scaled FP32 residual of the mixed solver: invD32 r32 =
+// invD64 r64, so the third channel is identical up to rounding
+__global__ void k_solver_input32(int64_t n, double kwh, double rms_inv, double p_scale,
+                                 const double* __restrict__ phi, const double* __restrict__ kr,
+                                 const double* __restrict__ p, const float* __restrict__ r,
+                                 const float* __restrict__ invD, float* __restrict__ fields)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double f = 0.5 * (1.0 + phi[c]);
+    const double k0 = kwh * f + kr[c] * (1.0 - f);
+    fields[c] = (float)((log10(fmax(k0, 1e-300)) + 13.0) * 0.5);
+    fields[n + c] = (float)(p[c] / p_scale);
+    fields[2 * n + c] = (float)((double)r[c] * (double)invD[c] * rms_inv);
+}
+
 __global__ void k_solver_input(int64_t n, double kwh, double rms_inv, double p_scale,
                                const double* __restrict__ phi, const double* __restrict__ kr,
                                const double* __restrict__ p, const double* __restrict__ r,
@@ -736,6 +752,33 @@ int net_correction(wpf_ctx* c, const double* r64, double rnorm, R* out)
     n->stream = saved;
     return rc;
 }
+int net_correction32(wpf_ctx* c, const float* r32, const float* invD32, double rnorm, float* out)
+{
+    wpf_net* n = c->net;
+    WPF_CHECK(n, WPF_ERR_STATE, "no network attached");
+    const Grid& g = c->g;
+    int rc = ensure_grid(n, g.nx, g.ny, g.nz);
+    if (rc) return rc;
+    if (!n->fields) {
+        if ((rc = alloc(&n->fields, 3 * n->lv[0].nv))) return rc;
+    }
+    hipStream_t saved = n->stream;
+    n->stream = c->stream;
+    const double rms_inv = rnorm > 0 ? std::sqrt((double)g.n) / rnorm : 0.0;
+    hipLaunchKernelGGL(k_solver_input32, dim3(nblk(g.n)), dim3(256), 0, c->stream, g.n, c->prm.permeab_hole,
+                       rms_inv, 1.0e5, (const double*)c->phi[c->cur], (const double*)c->krock,
+                       (const double*)c->p.p, r32, invD32, n->fields);
+    WPF_HIP(hipGetLastError());
+    rc = forward(n, n->fields);
+    if (!rc) {
+        hipLaunchKernelGGL(k_upsample_out<float>, dim3(nblk(g.n)), dim3(256), 0, c->stream, n->lv[0], n->lv[1],
+                           (const float*)n->y, (const uint8_t*)c->mask, out);
+        if (hipGetLastError() != hipSuccess) rc = WPF_ERR_HIP;
+    }
+    n->stream = saved;
+    return rc;
+}
+
 template int net_correction<double>(wpf_ctx*, const double*, double, double*);
 template int net_correction<float>(wpf_ctx*, const double*, double, float*);
 

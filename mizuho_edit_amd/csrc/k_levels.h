@@ -1,0 +1,497 @@
+// k_levels.h -- the pressure solver's 1-D kernels (every multigrid level;
+// the FP64 residual, norms and conversions of the fine level), shared by the
+// single-grid solver and its batched ensemble form.
+//
+// Launch shape: 1-D over cells, 256 threads, a fixed grid of at most
+// kMaxBlocks blocks, each block owning a contiguous chunk of cells; block b
+// is mapped to chunk (b % 8) * (nb / 8) + b / 8 so the chunks of one XCD
+// (blocks are dispatched round-robin over the 8 XCDs) are contiguous and
+// share that XCD's L2.  Each kernel's body is a tile function of (block b
+// of nb) so that the ensemble runs the same chunks, with the same partial
+// sums, for every member.
+#pragma once
+
+#include "k_fine.h"
+
+namespace wpf {
+
+static constexpr int kThreads = 256;
+static constexpr int kMaxBlocks = 2048;
+
+// result slots: S_RN and S_T0 hold two values each
+enum Slot { S_PQ = 0, S_RN = 1, S_RZ0 = 4, S_RZ1 = 5, S_BN = 6, S_T0 = 8, S_LR = 10 };
+
+template <typename R>
+struct LV {
+    const R* tx;
+    const R* ty;
+    const R* tz;
+    const R* D;     // nullptr on the fine level (summed from t)
+    const R* invD;
+    const uint8_t* mask;  // fine level only
+    int nx, ny, nz;
+    int64_t plane, n;
+};
+
+template <typename R>
+static LV<R> view(const MGLevel<R>& L, const uint8_t* mask, bool fine)
+{
+    LV<R> v;
+    v.tx = L.tx.p; v.ty = L.ty.p; v.tz = L.tz.p;
+    v.D = fine ? nullptr : L.D.p;
+    v.invD = L.invD.p;
+    v.mask = fine ? mask : nullptr;
+    v.nx = L.g.nx; v.ny = L.g.ny; v.nz = L.g.nz;
+    v.plane = L.g.plane; v.n = L.g.n;
+    return v;
+}
+
+struct Chunk {
+    int64_t begin, end;
+};
+
+// chunk of block b of nb over n cells
+__device__ __forceinline__ Chunk my_chunk_at(int64_t n, int b, int nb)
+{
+    const int lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int64_t per = ((n + nb - 1) / nb + kThreads - 1) / kThreads * kThreads;
+    Chunk ch;
+    ch.begin = (int64_t)lb * per;
+    ch.end = min(n, ch.begin + per);
+    return ch;
+}
+
+__device__ __forceinline__ Chunk my_chunk(int64_t n) { return my_chunk_at(n, blockIdx.x, gridDim.x); }
+
+static int blocks_for(int64_t n)
+{
+    int64_t b = (n + kThreads - 1) / kThreads;
+    if (b > kMaxBlocks) b = kMaxBlocks;
+    if (b >= 8) b = (b / 8) * 8;
+    return (int)std::max<int64_t>(1, b);
+}
+
+// off-diagonal sum and diagonal of row c
+template <typename R, bool FINE>
+__device__ __forceinline__ void row_parts(const LV<R>& L, const R* __restrict__ x, int64_t c, R& d,
+                                          R& s)
+{
+    const R txp = L.tx[c], txm = L.tx[c - 1];
+    const R typ = L.ty[c], tym = L.ty[c - L.nx];
+    const R tzp = L.tz[c], tzm = L.tz[c - L.plane];
+    if (FINE)
+        d = txp + txm + typ + tym + tzp + tzm;
+    else
+        d = L.D[c];
+    s = txp * x[c + 1] + txm * x[c - 1] + typ * x[c + L.nx] + tym * x[c - L.nx] +
+        tzp * x[c + L.plane] + tzm * x[c - L.plane];
+}
+
+template <typename R>
+__device__ __forceinline__ bool is_fixed(const LV<R>& L, int64_t c, bool fine)
+{
+    return fine && (L.mask[c] & WPF_MASK_FIXED);
+}
+
+// ---------------------------------------------------------------------------
+// true residual r = f - A x in FP64 (fine level, x holds Dirichlet values),
+// reduces ||invD r||^2 into result[0]; optionally also writes z = invD r and
+// reduces (r, z) into result[1]; optionally writes a scaled FP32 copy.
+// ---------------------------------------------------------------------------
+static __device__ __forceinline__ void t_true_resid(LV<double> L, const double* __restrict__ x,
+                                                         const double* __restrict__ f,
+                                                         double* __restrict__ r,
+                                                         double* __restrict__ z, float* __restrict__ r32,
+                                                         double s32, double* __restrict__ partials,
+                                                         unsigned* __restrict__ counter,
+                                                         double* __restrict__ result, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(L.n, b_, nb_);
+    double acc[2] = {0.0, 0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        double d, s;
+        row_parts<double, true>(L, x, c, d, s);
+        double rv = (f ? f[c] : 0.0) - (d * x[c] - s);
+        if (L.mask[c] & WPF_MASK_FIXED) rv = 0.0;
+        if (r) r[c] = rv;
+        const double zv = rv * L.invD[c];
+        acc[0] += zv * zv;
+        if (z) {
+            z[c] = zv;
+            acc[1] += rv * zv;
+        }
+        if (r32) r32[c] = (float)(rv * s32);
+    }
+    block_reduce_store<2>(acc, partials, counter, result, nb_, b_);
+}
+static __global__ void __launch_bounds__(kThreads) k_true_resid(LV<double> L, const double* __restrict__ x,
+                                                         const double* __restrict__ f,
+                                                         double* __restrict__ r,
+                                                         double* __restrict__ z, float* __restrict__ r32,
+                                                         double s32, double* __restrict__ partials,
+                                                         unsigned* __restrict__ counter,
+                                                         double* __restrict__ result)
+{
+    t_true_resid(L, x, f, r, z, r32, s32, partials, counter, result, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// ||D^-1 b|| of the reference system: Dirichlet rows carry p_D; free rows
+// carry (f + sum over Dirichlet neighbours t p_D) / D.
+static __device__ __forceinline__ void t_bnorm(LV<double> L, const double* __restrict__ x,
+                                                    const double* __restrict__ f,
+                                                    double* __restrict__ partials,
+                                                    unsigned* __restrict__ counter,
+                                                    double* __restrict__ result, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(L.n, b_, nb_);
+    double acc[1] = {0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        double v;
+        if (L.mask[c] & WPF_MASK_FIXED) {
+            v = x[c];
+        } else {
+            auto fx = [&](int64_t m) { return (L.mask[m] & WPF_MASK_FIXED) ? x[m] : 0.0; };
+            // masks outside the grid are never read: their coefficient is 0
+            double s = 0.0;
+            if (L.tx[c] != 0.0) s += L.tx[c] * fx(c + 1);
+            if (L.tx[c - 1] != 0.0) s += L.tx[c - 1] * fx(c - 1);
+            if (L.ty[c] != 0.0) s += L.ty[c] * fx(c + L.nx);
+            if (L.ty[c - L.nx] != 0.0) s += L.ty[c - L.nx] * fx(c - L.nx);
+            if (L.tz[c] != 0.0) s += L.tz[c] * fx(c + L.plane);
+            if (L.tz[c - L.plane] != 0.0) s += L.tz[c - L.plane] * fx(c - L.plane);
+            v = ((f ? f[c] : 0.0) + s) * L.invD[c];
+        }
+        acc[0] += v * v;
+    }
+    block_reduce_store<1>(acc, partials, counter, result, nb_, b_);
+}
+static __global__ void __launch_bounds__(kThreads) k_bnorm(LV<double> L, const double* __restrict__ x,
+                                                    const double* __restrict__ f,
+                                                    double* __restrict__ partials,
+                                                    unsigned* __restrict__ counter,
+                                                    double* __restrict__ result)
+{
+    t_bnorm(L, x, f, partials, counter, result, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// (a, b) dot product
+template <typename R>
+__device__ __forceinline__ void t_dot(int64_t n, const R* __restrict__ a,
+                                                  const R* __restrict__ b,
+                                                  double* __restrict__ partials,
+                                                  unsigned* __restrict__ counter,
+                                                  double* __restrict__ result, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(n, b_, nb_);
+    double acc[1] = {0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads)
+        acc[0] += (double)a[c] * (double)b[c];
+    block_reduce_store<1>(acc, partials, counter, result, nb_, b_);
+}
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_dot(int64_t n, const R* __restrict__ a,
+                                                  const R* __restrict__ b,
+                                                  double* __restrict__ partials,
+                                                  unsigned* __restrict__ counter,
+                                                  double* __restrict__ result)
+{
+    t_dot<R>(n, a, b, partials, counter, result, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// multigrid
+// ---------------------------------------------------------------------------
+// first sweep from x = 0: x = w invD b
+template <typename R>
+__device__ __forceinline__ void t_smooth0(int64_t n, R w, const R* __restrict__ b,
+                                                      const R* __restrict__ invD, R* __restrict__ x, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(n, b_, nb_);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = w * invD[c] * b[c];
+}
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_smooth0(int64_t n, R w, const R* __restrict__ b,
+                                                      const R* __restrict__ invD, R* __restrict__ x)
+{
+    t_smooth0<R>(n, w, b, invD, x, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// smoothing sweep y = x + a (x - xo) + w invD (b - A x): damped Jacobi for
+// a = 0, a second-order (Chebyshev) step otherwise (xo == nullptr: 0).
+// Cells with invD = 0 (fixed, or empty aggregates) keep x = 0.  y may be
+// the array xo points to.
+template <typename R, bool FINE>
+__device__ __forceinline__ void t_smooth(LV<R> L, R a, R w, const R* __restrict__ b,
+                                                     const R* __restrict__ x, const R* xo, R* y, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(L.n, b_, nb_);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        R d, s;
+        row_parts<R, FINE>(L, x, c, d, s);
+        const R xv = x[c];
+        R yv = xv + w * L.invD[c] * (b[c] - (d * xv - s));
+        if (a != (R)0) yv += a * (xv - (xo ? xo[c] : (R)0));
+        y[c] = yv;
+    }
+}
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_smooth(LV<R> L, R a, R w, const R* __restrict__ b,
+                                                     const R* __restrict__ x, const R* xo, R* y)
+{
+    t_smooth<R, FINE>(L, a, w, b, x, xo, y, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// coarse right-hand side: bc[I] = sum over children of (b - A x)
+template <typename R, bool FINE>
+__device__ __forceinline__ void t_resid_restrict(LV<R> L, int cnx, int cny, int cnz,
+                                                             const R* __restrict__ b,
+                                                             const R* __restrict__ x,
+                                                             R* __restrict__ bc, int b_, int nb_)
+{
+    const int64_t nc = (int64_t)cnx * cny * cnz;
+    const Chunk ch = my_chunk_at(nc, b_, nb_);
+    for (int64_t C = ch.begin + threadIdx.x; C < ch.end; C += kThreads) {
+        const int I = (int)(C % cnx);
+        const int J = (int)((C / cnx) % cny);
+        const int K = (int)(C / ((int64_t)cnx * cny));
+        R acc = (R)0;
+        for (int dk = 0; dk < 2; dk++) {
+            const int k = 2 * K + dk;
+            if (k >= L.nz) break;
+            for (int dj = 0; dj < 2; dj++) {
+                const int j = 2 * J + dj;
+                if (j >= L.ny) break;
+                for (int di = 0; di < 2; di++) {
+                    const int i = 2 * I + di;
+                    if (i >= L.nx) break;
+                    const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
+                    if (is_fixed(L, c, FINE)) continue;
+                    R d, s;
+                    row_parts<R, FINE>(L, x, c, d, s);
+                    acc += b[c] - (d * x[c] - s);
+                }
+            }
+        }
+        bc[C] = acc;
+    }
+}
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_resid_restrict(LV<R> L, int cnx, int cny, int cnz,
+                                                             const R* __restrict__ b,
+                                                             const R* __restrict__ x,
+                                                             R* __restrict__ bc)
+{
+    t_resid_restrict<R, FINE>(L, cnx, cny, cnz, b, x, bc, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// x += scale * P e  (piecewise constant; fixed fine cells untouched)
+template <typename R, bool FINE>
+__device__ __forceinline__ void t_prolong(LV<R> L, int cnx, int cny, R scale,
+                                                      const R* __restrict__ e, R* __restrict__ x, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(L.n, b_, nb_);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        const int i = (int)(c % L.nx);
+        const int j = (int)((c / L.nx) % L.ny);
+        const int k = (int)(c / L.plane);
+        const int64_t C = (i >> 1) + (int64_t)cnx * ((j >> 1) + (int64_t)cny * (k >> 1));
+        if (is_fixed(L, c, FINE)) continue;
+        x[c] = x[c] + scale * e[C];
+    }
+}
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_prolong(LV<R> L, int cnx, int cny, R scale,
+                                                      const R* __restrict__ e, R* __restrict__ x)
+{
+    t_prolong<R, FINE>(L, cnx, cny, scale, e, x, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// Galerkin aggregation A_c = P^T A P on 2x2x2 aggregates.
+template <typename R, bool FINE>
+__device__ __forceinline__ void t_coarsen(LV<R> L, int cnx, int cny, int cnz,
+                                                      int64_t cplane, R* __restrict__ ctx,
+                                                      R* __restrict__ cty, R* __restrict__ ctz,
+                                                      R* __restrict__ cD, R* __restrict__ cinvD, int b_, int nb_)
+{
+    const int64_t nc = (int64_t)cnx * cny * cnz;
+    const Chunk ch = my_chunk_at(nc, b_, nb_);
+    for (int64_t C = ch.begin + threadIdx.x; C < ch.end; C += kThreads) {
+        const int I = (int)(C % cnx);
+        const int J = (int)((C / cnx) % cny);
+        const int K = (int)(C / cplane);
+        R D = 0, tx = 0, ty = 0, tz = 0;
+        for (int dk = 0; dk < 2; dk++) {
+            const int k = 2 * K + dk;
+            if (k >= L.nz) break;
+            for (int dj = 0; dj < 2; dj++) {
+                const int j = 2 * J + dj;
+                if (j >= L.ny) break;
+                for (int di = 0; di < 2; di++) {
+                    const int i = 2 * I + di;
+                    if (i >= L.nx) break;
+                    const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
+                    if (is_fixed(L, c, FINE)) continue;
+                    if (FINE)
+                        D += L.tx[c] + L.tx[c - 1] + L.ty[c] + L.ty[c - L.nx] + L.tz[c] +
+                             L.tz[c - L.plane];
+                    else
+                        D += L.D[c];
+                    // +x face
+                    if (i + 1 < L.nx && !is_fixed(L, c + 1, FINE)) {
+                        if (di == 0) D -= (R)2 * L.tx[c];
+                        else tx += L.tx[c];
+                    }
+                    if (j + 1 < L.ny && !is_fixed(L, c + L.nx, FINE)) {
+                        if (dj == 0) D -= (R)2 * L.ty[c];
+                        else ty += L.ty[c];
+                    }
+                    if (k + 1 < L.nz && !is_fixed(L, c + L.plane, FINE)) {
+                        if (dk == 0) D -= (R)2 * L.tz[c];
+                        else tz += L.tz[c];
+                    }
+                }
+            }
+        }
+        // faces on the global edge of the coarse grid stay 0
+        if (I == cnx - 1) tx = 0;
+        if (J == cny - 1) ty = 0;
+        if (K == cnz - 1) tz = 0;
+        ctx[C] = tx;
+        cty[C] = ty;
+        ctz[C] = tz;
+        cD[C] = D;
+        cinvD[C] = D > (R)0 ? (R)1 / D : (R)0;
+    }
+}
+template <typename R, bool FINE>
+__global__ void __launch_bounds__(kThreads) k_coarsen(LV<R> L, int cnx, int cny, int cnz,
+                                                      int64_t cplane, R* __restrict__ ctx,
+                                                      R* __restrict__ cty, R* __restrict__ ctz,
+                                                      R* __restrict__ cD, R* __restrict__ cinvD)
+{
+    t_coarsen<R, FINE>(L, cnx, cny, cnz, cplane, ctx, cty, ctz, cD, cinvD, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// coarsest level: nsweep damped-Jacobi sweeps from 0, one workgroup,
+// ping-pong between x and y (both global, L2-resident)
+template <typename R>
+__global__ void __launch_bounds__(1024) k_coarsest(LV<R> L, R w, int nsweep, const R* __restrict__ b,
+                                                   R* __restrict__ x, R* __restrict__ y)
+{
+    for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) x[c] = w * L.invD[c] * b[c];
+    __syncthreads();
+    for (int it = 1; it < nsweep; it++) {
+        for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) {
+            R d, s;
+            row_parts<R, false>(L, x, c, d, s);
+            (void)d;
+            y[c] = ((R)1 - w) * x[c] + w * L.invD[c] * (b[c] + s);
+        }
+        __syncthreads();
+        for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) x[c] = y[c];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// precision conversion of the fine operator: t32 = t64 * s, invD32 = 1/D32
+// ---------------------------------------------------------------------------
+static __device__ __forceinline__ void t_to_f32(LV<double> L, double s, float* __restrict__ tx,
+                                                     float* __restrict__ ty, float* __restrict__ tz,
+                                                     float* __restrict__ invD, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(L.n, b_, nb_);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        const float a = (float)(L.tx[c] * s), b = (float)(L.ty[c] * s), d = (float)(L.tz[c] * s);
+        tx[c] = a;
+        ty[c] = b;
+        tz[c] = d;
+        const float D = a + (float)(L.tx[c - 1] * s) + b + (float)(L.ty[c - L.nx] * s) + d +
+                        (float)(L.tz[c - L.plane] * s);
+        invD[c] = (L.invD[c] == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
+    }
+}
+static __global__ void __launch_bounds__(kThreads) k_to_f32(LV<double> L, double s, float* __restrict__ tx,
+                                                     float* __restrict__ ty, float* __restrict__ tz,
+                                                     float* __restrict__ invD)
+{
+    t_to_f32(L, s, tx, ty, tz, invD, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// FP16 fine faces of the inner solve: h = half(float(t64 * s)).  The FP32
+// copy and invD hold exactly the values the FP16 faces decode to, so the
+// coarse levels are the Galerkin operators of the operator the fine level
+// applies.
+static __device__ __forceinline__ void t_to_f16(LV<double> L, double s, _Float16* __restrict__ htx,
+                                                     _Float16* __restrict__ hty,
+                                                     _Float16* __restrict__ htz, float* __restrict__ tx,
+                                                     float* __restrict__ ty, float* __restrict__ tz,
+                                                     float* __restrict__ invD, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(L.n, b_, nb_);
+    auto h = [&](double t) { return (_Float16)(float)(t * s); };
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        const _Float16 a = h(L.tx[c]), b = h(L.ty[c]), d = h(L.tz[c]);
+        htx[c] = a;
+        hty[c] = b;
+        htz[c] = d;
+        tx[c] = (float)a;
+        ty[c] = (float)b;
+        tz[c] = (float)d;
+        const float D = (float)a + (float)h(L.tx[c - 1]) + (float)b + (float)h(L.ty[c - L.nx]) + (float)d +
+                        (float)h(L.tz[c - L.plane]);
+        invD[c] = (L.invD[c] == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
+    }
+}
+static __global__ void __launch_bounds__(kThreads) k_to_f16(LV<double> L, double s, _Float16* __restrict__ htx,
+                                                     _Float16* __restrict__ hty,
+                                                     _Float16* __restrict__ htz, float* __restrict__ tx,
+                                                     float* __restrict__ ty, float* __restrict__ tz,
+                                                     float* __restrict__ invD)
+{
+    t_to_f16(L, s, htx, hty, htz, tx, ty, tz, invD, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// x64 += e32 (scaled back)
+static __device__ __forceinline__ void t_add_f32(int64_t n, const float* __restrict__ e,
+                                                      double* __restrict__ x, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(n, b_, nb_);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = x[c] + (double)e[c];
+}
+static __global__ void __launch_bounds__(kThreads) k_add_f32(int64_t n, const float* __restrict__ e,
+                                                      double* __restrict__ x)
+{
+    t_add_f32(n, e, x, (int)blockIdx.x, (int)gridDim.x);
+}
+
+template <typename R>
+__device__ __forceinline__ void t_zero(int64_t n, R* __restrict__ x, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(n, b_, nb_);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = (R)0;
+}
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_zero(int64_t n, R* __restrict__ x)
+{
+    t_zero<R>(n, x, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// enforce Dirichlet values (sparse list)
+static __global__ void k_set_dirichlet(int n, const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                double* __restrict__ x)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[idx[t]] = val[t];
+}
+
+// dense injection right-hand side f = q dx^2/dx^3 (1:817, sign of the SPD form)
+static __global__ void k_set_inj(int n, const int32_t* __restrict__ idx, const double* __restrict__ val,
+                          double* __restrict__ f)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) f[idx[t]] = val[t];
+}
+
+
+}  // namespace wpf

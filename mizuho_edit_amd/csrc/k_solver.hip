@@ -28,7 +28,8 @@
 // (blocks are dispatched round-robin over the 8 XCDs) are contiguous and
 // share that XCD's L2.
 #include "wpf_internal.h"
-#include "k_fine.h"
+#include "k_levels.h"
+#include "k_vfine.h"
 
 #include <algorithm>
 #include <chrono>
@@ -36,462 +37,6 @@
 
 
 namespace wpf {
-
-static constexpr int kThreads = 256;
-static constexpr int kMaxBlocks = 2048;
-
-// result slots: S_RN and S_T0 hold two values each
-enum Slot { S_PQ = 0, S_RN = 1, S_RZ0 = 4, S_RZ1 = 5, S_BN = 6, S_T0 = 8, S_LR = 10 };
-
-template <typename R>
-struct LV {
-    const R* tx;
-    const R* ty;
-    const R* tz;
-    const R* D;     // nullptr on the fine level (summed from t)
-    const R* invD;
-    const uint8_t* mask;  // fine level only
-    int nx, ny, nz;
-    int64_t plane, n;
-};
-
-template <typename R>
-static LV<R> view(const MGLevel<R>& L, const uint8_t* mask, bool fine)
-{
-    LV<R> v;
-    v.tx = L.tx.p; v.ty = L.ty.p; v.tz = L.tz.p;
-    v.D = fine ? nullptr : L.D.p;
-    v.invD = L.invD.p;
-    v.mask = fine ? mask : nullptr;
-    v.nx = L.g.nx; v.ny = L.g.ny; v.nz = L.g.nz;
-    v.plane = L.g.plane; v.n = L.g.n;
-    return v;
-}
-
-struct Chunk {
-    int64_t begin, end;
-};
-
-__device__ __forceinline__ Chunk my_chunk(int64_t n)
-{
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
-    const int lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
-    const int64_t per = ((n + nb - 1) / nb + kThreads - 1) / kThreads * kThreads;
-    Chunk ch;
-    ch.begin = (int64_t)lb * per;
-    ch.end = min(n, ch.begin + per);
-    return ch;
-}
-
-static int blocks_for(int64_t n)
-{
-    int64_t b = (n + kThreads - 1) / kThreads;
-    if (b > kMaxBlocks) b = kMaxBlocks;
-    if (b >= 8) b = (b / 8) * 8;
-    return (int)std::max<int64_t>(1, b);
-}
-
-// off-diagonal sum and diagonal of row c
-template <typename R, bool FINE>
-__device__ __forceinline__ void row_parts(const LV<R>& L, const R* __restrict__ x, int64_t c, R& d,
-                                          R& s)
-{
-    const R txp = L.tx[c], txm = L.tx[c - 1];
-    const R typ = L.ty[c], tym = L.ty[c - L.nx];
-    const R tzp = L.tz[c], tzm = L.tz[c - L.plane];
-    if (FINE)
-        d = txp + txm + typ + tym + tzp + tzm;
-    else
-        d = L.D[c];
-    s = txp * x[c + 1] + txm * x[c - 1] + typ * x[c + L.nx] + tym * x[c - L.nx] +
-        tzp * x[c + L.plane] + tzm * x[c - L.plane];
-}
-
-template <typename R>
-__device__ __forceinline__ bool is_fixed(const LV<R>& L, int64_t c, bool fine)
-{
-    return fine && (L.mask[c] & WPF_MASK_FIXED);
-}
-
-// ---------------------------------------------------------------------------
-// true residual r = f - A x in FP64 (fine level, x holds Dirichlet values),
-// reduces ||invD r||^2 into result[0]; optionally also writes z = invD r and
-// reduces (r, z) into result[1]; optionally writes a scaled FP32 copy.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_true_resid(LV<double> L, const double* __restrict__ x,
-                                                         const double* __restrict__ f,
-                                                         double* __restrict__ r,
-                                                         double* __restrict__ z, float* __restrict__ r32,
-                                                         double s32, double* __restrict__ partials,
-                                                         unsigned* __restrict__ counter,
-                                                         double* __restrict__ result)
-{
-    const Chunk ch = my_chunk(L.n);
-    double acc[2] = {0.0, 0.0};
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        double d, s;
-        row_parts<double, true>(L, x, c, d, s);
-        double rv = (f ? f[c] : 0.0) - (d * x[c] - s);
-        if (L.mask[c] & WPF_MASK_FIXED) rv = 0.0;
-        if (r) r[c] = rv;
-        const double zv = rv * L.invD[c];
-        acc[0] += zv * zv;
-        if (z) {
-            z[c] = zv;
-            acc[1] += rv * zv;
-        }
-        if (r32) r32[c] = (float)(rv * s32);
-    }
-    block_reduce_store<2>(acc, partials, counter, result, gridDim.x, blockIdx.x);
-}
-
-// ||D^-1 b|| of the reference system: Dirichlet rows carry p_D; free rows
-// carry (f + sum over Dirichlet neighbours t p_D) / D.
-__global__ void __launch_bounds__(kThreads) k_bnorm(LV<double> L, const double* __restrict__ x,
-                                                    const double* __restrict__ f,
-                                                    double* __restrict__ partials,
-                                                    unsigned* __restrict__ counter,
-                                                    double* __restrict__ result)
-{
-    const Chunk ch = my_chunk(L.n);
-    double acc[1] = {0.0};
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        double v;
-        if (L.mask[c] & WPF_MASK_FIXED) {
-            v = x[c];
-        } else {
-            auto fx = [&](int64_t m) { return (L.mask[m] & WPF_MASK_FIXED) ? x[m] : 0.0; };
-            // masks outside the grid are never read: their coefficient is 0
-            double s = 0.0;
-            if (L.tx[c] != 0.0) s += L.tx[c] * fx(c + 1);
-            if (L.tx[c - 1] != 0.0) s += L.tx[c - 1] * fx(c - 1);
-            if (L.ty[c] != 0.0) s += L.ty[c] * fx(c + L.nx);
-            if (L.ty[c - L.nx] != 0.0) s += L.ty[c - L.nx] * fx(c - L.nx);
-            if (L.tz[c] != 0.0) s += L.tz[c] * fx(c + L.plane);
-            if (L.tz[c - L.plane] != 0.0) s += L.tz[c - L.plane] * fx(c - L.plane);
-            v = ((f ? f[c] : 0.0) + s) * L.invD[c];
-        }
-        acc[0] += v * v;
-    }
-    block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
-}
-
-// q = A p (masked), reduces (p, q).
-template <typename R, bool FINE>
-__global__ void __launch_bounds__(kThreads) k_matvec_dot(LV<R> L, const R* __restrict__ p,
-                                                         R* __restrict__ q,
-                                                         double* __restrict__ partials,
-                                                         unsigned* __restrict__ counter,
-                                                         double* __restrict__ result)
-{
-    const Chunk ch = my_chunk(L.n);
-    double acc[1] = {0.0};
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        R d, s;
-        row_parts<R, FINE>(L, p, c, d, s);
-        R qv = d * p[c] - s;
-        if (is_fixed(L, c, FINE)) qv = (R)0;
-        q[c] = qv;
-        acc[0] += (double)p[c] * (double)qv;
-    }
-    block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
-}
-
-// x += alpha p; r -= alpha q; alpha = res[num]/res[den].  Reduces
-// ||invD r||^2 (slot 0) and, when z != nullptr, writes z = invD r and
-// reduces (r, z) (slot 1).
-template <typename R, typename X>
-__global__ void __launch_bounds__(kThreads) k_cg_update(int64_t n, const double* __restrict__ res,
-                                                        int num, int den, X* __restrict__ x,
-                                                        const R* __restrict__ p, R* __restrict__ r,
-                                                        const R* __restrict__ q,
-                                                        const R* __restrict__ invD, R* __restrict__ z,
-                                                        double* __restrict__ partials,
-                                                        unsigned* __restrict__ counter,
-                                                        double* __restrict__ result)
-{
-    const double alpha = res[num] / res[den];
-    const R a = (R)alpha;
-    const Chunk ch = my_chunk(n);
-    double acc[2] = {0.0, 0.0};
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        x[c] = x[c] + (X)(a * p[c]);
-        const R rv = r[c] - a * q[c];
-        r[c] = rv;
-        const R zv = rv * invD[c];
-        acc[0] += (double)zv * (double)zv;
-        if (z) {
-            z[c] = zv;
-            acc[1] += (double)rv * (double)zv;
-        }
-    }
-    block_reduce_store<2>(acc, partials, counter, result, gridDim.x, blockIdx.x);
-}
-
-// p = z + beta p, beta = res[num]/res[den] (first iteration: p = z).
-template <typename R>
-__global__ void __launch_bounds__(kThreads) k_xpby(int64_t n, const double* __restrict__ res, int num,
-                                                   int den, int first, const R* __restrict__ z,
-                                                   R* __restrict__ p)
-{
-    const R beta = first ? (R)0 : (R)(res[num] / res[den]);
-    const Chunk ch = my_chunk(n);
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads)
-        p[c] = first ? z[c] : z[c] + beta * p[c];
-}
-
-// (a, b) dot product
-template <typename R>
-__global__ void __launch_bounds__(kThreads) k_dot(int64_t n, const R* __restrict__ a,
-                                                  const R* __restrict__ b,
-                                                  double* __restrict__ partials,
-                                                  unsigned* __restrict__ counter,
-                                                  double* __restrict__ result)
-{
-    const Chunk ch = my_chunk(n);
-    double acc[1] = {0.0};
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads)
-        acc[0] += (double)a[c] * (double)b[c];
-    block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
-}
-
-// ---------------------------------------------------------------------------
-// multigrid
-// ---------------------------------------------------------------------------
-// first sweep from x = 0: x = w invD b
-template <typename R>
-__global__ void __launch_bounds__(kThreads) k_smooth0(int64_t n, R w, const R* __restrict__ b,
-                                                      const R* __restrict__ invD, R* __restrict__ x)
-{
-    const Chunk ch = my_chunk(n);
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = w * invD[c] * b[c];
-}
-
-// smoothing sweep y = x + a (x - xo) + w invD (b - A x): damped Jacobi for
-// a = 0, a second-order (Chebyshev) step otherwise (xo == nullptr: 0).
-// Cells with invD = 0 (fixed, or empty aggregates) keep x = 0.  y may be
-// the array xo points to.
-template <typename R, bool FINE>
-__global__ void __launch_bounds__(kThreads) k_smooth(LV<R> L, R a, R w, const R* __restrict__ b,
-                                                     const R* __restrict__ x, const R* xo, R* y)
-{
-    const Chunk ch = my_chunk(L.n);
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        R d, s;
-        row_parts<R, FINE>(L, x, c, d, s);
-        const R xv = x[c];
-        R yv = xv + w * L.invD[c] * (b[c] - (d * xv - s));
-        if (a != (R)0) yv += a * (xv - (xo ? xo[c] : (R)0));
-        y[c] = yv;
-    }
-}
-
-// coarse right-hand side: bc[I] = sum over children of (b - A x)
-template <typename R, bool FINE>
-__global__ void __launch_bounds__(kThreads) k_resid_restrict(LV<R> L, int cnx, int cny, int cnz,
-                                                             const R* __restrict__ b,
-                                                             const R* __restrict__ x,
-                                                             R* __restrict__ bc)
-{
-    const int64_t nc = (int64_t)cnx * cny * cnz;
-    const Chunk ch = my_chunk(nc);
-    for (int64_t C = ch.begin + threadIdx.x; C < ch.end; C += kThreads) {
-        const int I = (int)(C % cnx);
-        const int J = (int)((C / cnx) % cny);
-        const int K = (int)(C / ((int64_t)cnx * cny));
-        R acc = (R)0;
-        for (int dk = 0; dk < 2; dk++) {
-            const int k = 2 * K + dk;
-            if (k >= L.nz) break;
-            for (int dj = 0; dj < 2; dj++) {
-                const int j = 2 * J + dj;
-                if (j >= L.ny) break;
-                for (int di = 0; di < 2; di++) {
-                    const int i = 2 * I + di;
-                    if (i >= L.nx) break;
-                    const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
-                    if (is_fixed(L, c, FINE)) continue;
-                    R d, s;
-                    row_parts<R, FINE>(L, x, c, d, s);
-                    acc += b[c] - (d * x[c] - s);
-                }
-            }
-        }
-        bc[C] = acc;
-    }
-}
-
-// x += scale * P e  (piecewise constant; fixed fine cells untouched)
-template <typename R, bool FINE>
-__global__ void __launch_bounds__(kThreads) k_prolong(LV<R> L, int cnx, int cny, R scale,
-                                                      const R* __restrict__ e, R* __restrict__ x)
-{
-    const Chunk ch = my_chunk(L.n);
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        const int i = (int)(c % L.nx);
-        const int j = (int)((c / L.nx) % L.ny);
-        const int k = (int)(c / L.plane);
-        const int64_t C = (i >> 1) + (int64_t)cnx * ((j >> 1) + (int64_t)cny * (k >> 1));
-        if (is_fixed(L, c, FINE)) continue;
-        x[c] = x[c] + scale * e[C];
-    }
-}
-
-// Galerkin aggregation A_c = P^T A P on 2x2x2 aggregates.
-template <typename R, bool FINE>
-__global__ void __launch_bounds__(kThreads) k_coarsen(LV<R> L, int cnx, int cny, int cnz,
-                                                      int64_t cplane, R* __restrict__ ctx,
-                                                      R* __restrict__ cty, R* __restrict__ ctz,
-                                                      R* __restrict__ cD, R* __restrict__ cinvD)
-{
-    const int64_t nc = (int64_t)cnx * cny * cnz;
-    const Chunk ch = my_chunk(nc);
-    for (int64_t C = ch.begin + threadIdx.x; C < ch.end; C += kThreads) {
-        const int I = (int)(C % cnx);
-        const int J = (int)((C / cnx) % cny);
-        const int K = (int)(C / cplane);
-        R D = 0, tx = 0, ty = 0, tz = 0;
-        for (int dk = 0; dk < 2; dk++) {
-            const int k = 2 * K + dk;
-            if (k >= L.nz) break;
-            for (int dj = 0; dj < 2; dj++) {
-                const int j = 2 * J + dj;
-                if (j >= L.ny) break;
-                for (int di = 0; di < 2; di++) {
-                    const int i = 2 * I + di;
-                    if (i >= L.nx) break;
-                    const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
-                    if (is_fixed(L, c, FINE)) continue;
-                    if (FINE)
-                        D += L.tx[c] + L.tx[c - 1] + L.ty[c] + L.ty[c - L.nx] + L.tz[c] +
-                             L.tz[c - L.plane];
-                    else
-                        D += L.D[c];
-                    // +x face
-                    if (i + 1 < L.nx && !is_fixed(L, c + 1, FINE)) {
-                        if (di == 0) D -= (R)2 * L.tx[c];
-                        else tx += L.tx[c];
-                    }
-                    if (j + 1 < L.ny && !is_fixed(L, c + L.nx, FINE)) {
-                        if (dj == 0) D -= (R)2 * L.ty[c];
-                        else ty += L.ty[c];
-                    }
-                    if (k + 1 < L.nz && !is_fixed(L, c + L.plane, FINE)) {
-                        if (dk == 0) D -= (R)2 * L.tz[c];
-                        else tz += L.tz[c];
-                    }
-                }
-            }
-        }
-        // faces on the global edge of the coarse grid stay 0
-        if (I == cnx - 1) tx = 0;
-        if (J == cny - 1) ty = 0;
-        if (K == cnz - 1) tz = 0;
-        ctx[C] = tx;
-        cty[C] = ty;
-        ctz[C] = tz;
-        cD[C] = D;
-        cinvD[C] = D > (R)0 ? (R)1 / D : (R)0;
-    }
-}
-
-// coarsest level: nsweep damped-Jacobi sweeps from 0, one workgroup,
-// ping-pong between x and y (both global, L2-resident)
-template <typename R>
-__global__ void __launch_bounds__(1024) k_coarsest(LV<R> L, R w, int nsweep, const R* __restrict__ b,
-                                                   R* __restrict__ x, R* __restrict__ y)
-{
-    for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) x[c] = w * L.invD[c] * b[c];
-    __syncthreads();
-    for (int it = 1; it < nsweep; it++) {
-        for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) {
-            R d, s;
-            row_parts<R, false>(L, x, c, d, s);
-            (void)d;
-            y[c] = ((R)1 - w) * x[c] + w * L.invD[c] * (b[c] + s);
-        }
-        __syncthreads();
-        for (int64_t c = threadIdx.x; c < L.n; c += blockDim.x) x[c] = y[c];
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// precision conversion of the fine operator: t32 = t64 * s, invD32 = 1/D32
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_to_f32(LV<double> L, double s, float* __restrict__ tx,
-                                                     float* __restrict__ ty, float* __restrict__ tz,
-                                                     float* __restrict__ invD)
-{
-    const Chunk ch = my_chunk(L.n);
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        const float a = (float)(L.tx[c] * s), b = (float)(L.ty[c] * s), d = (float)(L.tz[c] * s);
-        tx[c] = a;
-        ty[c] = b;
-        tz[c] = d;
-        const float D = a + (float)(L.tx[c - 1] * s) + b + (float)(L.ty[c - L.nx] * s) + d +
-                        (float)(L.tz[c - L.plane] * s);
-        invD[c] = (L.invD[c] == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
-    }
-}
-
-// FP16 fine faces of the inner solve: h = half(float(t64 * s)).  The FP32
-// copy and invD hold exactly the values the FP16 faces decode to, so the
-// coarse levels are the Galerkin operators of the operator the fine level
-// applies.
-__global__ void __launch_bounds__(kThreads) k_to_f16(LV<double> L, double s, _Float16* __restrict__ htx,
-                                                     _Float16* __restrict__ hty,
-                                                     _Float16* __restrict__ htz, float* __restrict__ tx,
-                                                     float* __restrict__ ty, float* __restrict__ tz,
-                                                     float* __restrict__ invD)
-{
-    const Chunk ch = my_chunk(L.n);
-    auto h = [&](double t) { return (_Float16)(float)(t * s); };
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        const _Float16 a = h(L.tx[c]), b = h(L.ty[c]), d = h(L.tz[c]);
-        htx[c] = a;
-        hty[c] = b;
-        htz[c] = d;
-        tx[c] = (float)a;
-        ty[c] = (float)b;
-        tz[c] = (float)d;
-        const float D = (float)a + (float)h(L.tx[c - 1]) + (float)b + (float)h(L.ty[c - L.nx]) + (float)d +
-                        (float)h(L.tz[c - L.plane]);
-        invD[c] = (L.invD[c] == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
-    }
-}
-
-// x64 += e32 (scaled back)
-__global__ void __launch_bounds__(kThreads) k_add_f32(int64_t n, const float* __restrict__ e,
-                                                      double* __restrict__ x)
-{
-    const Chunk ch = my_chunk(n);
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = x[c] + (double)e[c];
-}
-
-template <typename R>
-__global__ void __launch_bounds__(kThreads) k_zero(int64_t n, R* __restrict__ x)
-{
-    const Chunk ch = my_chunk(n);
-    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) x[c] = (R)0;
-}
-
-// enforce Dirichlet values (sparse list)
-__global__ void k_set_dirichlet(int n, const int32_t* __restrict__ idx, const double* __restrict__ val,
-                                double* __restrict__ x)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) x[idx[t]] = val[t];
-}
-
-// dense injection right-hand side f = q dx^2/dx^3 (1:817, sign of the SPD form)
-__global__ void k_set_inj(int n, const int32_t* __restrict__ idx, const double* __restrict__ val,
-                          double* __restrict__ f)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) f[idx[t]] = val[t];
-}
 
 // ---------------------------------------------------------------------------
 // host side
@@ -705,13 +250,129 @@ static int vcycle_coarse(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, cons
 template <typename R, typename C>
 struct Fine {
     ZG G;
+    VZ V;       // vector form (used when vec)
+    bool vec;
     Grid g;
     const C *tx, *ty, *tz;
     const R* invD;
     R *r, *Z, *T, *pa, *pb, *q;
 };
 
-#define FINE_OP(F) F.G, F.tx, F.ty, F.tz, F.invD, (const uint8_t*)c->mask
+// cells per thread of the vector kernels (one 16-byte vector)
+template <typename R>
+constexpr int vwidth() { return 16 / (int)sizeof(R); }
+
+// Vector form when x rows split into whole vectors; zc planes per tile.
+template <typename R, typename C>
+static void set_shape(Fine<R, C>& F, int zc)
+{
+    constexpr int V = vwidth<R>();
+    F.G = make_zg(F.g);
+    F.vec = F.g.nx % V == 0;
+    F.V = make_vz<V>(F.g, zc);
+}
+
+template <typename R, typename C>
+static int fine_blocks(const Fine<R, C>& F)
+{
+    return F.vec ? vz_blocks(F.V) : zg_blocks(F.G);
+}
+
+template <typename R, typename C, bool DOT, bool PROX, bool PROO>
+static int sweep(wpf_ctx* c, const Fine<R, C>& F, double a, double w, const R* x, const R* xo,
+                 const Pro<R>& P, R* y, int slot);
+
+// Autotuner of the vector kernels' tile depth: times one smoothing sweep
+// over the grid for each candidate number of planes per tile and keeps the
+// fastest (once per context and precision; *zc caches the choice).
+template <typename R, typename C>
+static int tune_shape(wpf_ctx* c, Fine<R, C>& F, int* zc)
+{
+    if (*zc > 0) {
+        set_shape(F, *zc);
+        return WPF_OK;
+    }
+    set_shape(F, 8);
+    if (!F.vec) {
+        *zc = 8;
+        return WPF_OK;
+    }
+    const int cand[] = {2, 4, 8, 16, 32};
+    hipEvent_t e0, e1;
+    WPF_HIP(hipEventCreate(&e0));
+    WPF_HIP(hipEventCreate(&e1));
+    const Pro<R> none{nullptr, 0, 0, (R)0};
+    int best = 8;
+    float best_ms = 1e30f;
+    int rc = WPF_OK;
+    for (int zcv : cand) {
+        if (zcv > 2 * F.g.nz) break;
+        set_shape(F, zcv);
+        // scratch vectors only: Z is read, T written
+        if ((rc = sweep<R, C, false, false, false>(c, F, 0.0, 0.5, F.Z, nullptr, none, F.T, S_T0))) break;
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < 3 && !rc; r++)
+            rc = sweep<R, C, false, false, false>(c, F, 0.0, 0.5, F.Z, nullptr, none, F.T, S_T0);
+        (void)hipEventRecord(e1, c->stream);
+        if (rc || hipEventSynchronize(e1) != hipSuccess) {
+            if (!rc) rc = WPF_ERR_HIP;
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = zcv;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *zc = best;
+    set_shape(F, best);
+    return WPF_OK;
+}
+
+// One fine-level smoothing sweep (see k_fsweep / k_vsweep).
+template <typename R, typename C, bool DOT, bool PROX, bool PROO>
+static int sweep(wpf_ctx* c, const Fine<R, C>& F, double a, double w, const R* x, const R* xo,
+                 const Pro<R>& P, R* y, int slot)
+{
+    constexpr int V = vwidth<R>();
+    prof_mark(c, WPF_KERNEL_SMOOTH, true);
+    if (F.vec)
+        hipLaunchKernelGGL((k_vsweep<R, C, V, DOT, PROX, PROO>), dim3(vz_blocks(F.V)), dim3(256), 0,
+                           c->stream, F.V, F.tx, F.ty, F.tz, F.invD, (const uint8_t*)c->mask, (R)a, (R)w,
+                           (const R*)F.r, x, xo, P, y, RED(slot));
+    else
+        hipLaunchKernelGGL((k_fsweep<R, C, DOT, PROX, PROO>), dim3(zg_blocks(F.G)), dim3(kFX, kFY), 0,
+                           c->stream, F.G, F.tx, F.ty, F.tz, F.invD, (const uint8_t*)c->mask, (R)a, (R)w,
+                           (const R*)F.r, x, xo, P, y, RED(slot));
+    prof_mark(c, WPF_KERNEL_SMOOTH, false);
+    WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
+// p_new = z (+ beta p_old), q = A p_new, reduces (p_new, q) into S_PQ.
+template <typename R, typename C>
+static int dir_mv(wpf_ctx* c, const Fine<R, C>& F, const R* z, const R* pold, R* pnew, int num, int den,
+                  int first)
+{
+    constexpr int V = vwidth<R>();
+    prof_mark(c, WPF_KERNEL_MATVEC, true);
+    if (F.vec)
+        hipLaunchKernelGGL((k_vdir_mv<R, C, V>), dim3(vz_blocks(F.V)), dim3(256), 0, c->stream, F.V, F.tx,
+                           F.ty, F.tz, (const uint8_t*)c->mask, z, pold, pnew, F.q,
+                           (const double*)c->red.result, num, den, first, RED(S_PQ));
+    else
+        hipLaunchKernelGGL((k_dir_mv<R, C>), dim3(zg_blocks(F.G)), dim3(kFX, kFY), 0, c->stream, F.G, F.tx,
+                           F.ty, F.tz, (const uint8_t*)c->mask, z, pold, pnew, F.q,
+                           (const double*)c->red.result, num, den, first, RED(S_PQ));
+    prof_mark(c, WPF_KERNEL_MATVEC, false);
+    WPF_HIP(hipGetLastError());
+    FINALIZE(fine_blocks(F), 1, S_PQ);
+    return WPF_OK;
+}
 
 // z = M r on the fine level, starting from z0 = w1 invD r already in Z.
 // The result pointer is returned through *zres; (r, z) goes to result slot.
@@ -723,14 +384,11 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>
     R* oth = F.T;
     double a, w;
     Pro<R> none{nullptr, 0, 0, (R)0};
-    {
-        const int kid_ = WPF_KERNEL_SMOOTH;
-        for (int s = 1; s < o.sweeps; s++) {
-            step_coef(o, s, &a, &w);
-            LAUNCHZ((k_fsweep<R, C, false, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                    (const R*)cur, (const R*)nullptr, none, oth, RED(dot_slot));
-            std::swap(cur, oth);
-        }
+    int rc;
+    for (int s = 1; s < o.sweeps; s++) {
+        step_coef(o, s, &a, &w);
+        if ((rc = sweep<R, C, false, false, false>(c, F, a, w, cur, nullptr, none, oth, dot_slot))) return rc;
+        std::swap(cur, oth);
     }
     const bool coarse = lv.size() > 1;
     Pro<R> pro = none;
@@ -741,47 +399,30 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>
             LAUNCHZ((k_fresid_restrict<R, C>), F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask,
                     (const R*)F.r, (const R*)cur, C1.g.nx, C1.g.ny, C1.b.p);
         }
-        int rc = vcycle_coarse(c, lv, 1, o);
-        if (rc) return rc;
+        if ((rc = vcycle_coarse(c, lv, 1, o))) return rc;
         pro = Pro<R>{C1.x.p, C1.g.nx, C1.g.plane, (R)o.cscale};
     }
     // post-smoothing; the first step adds the coarse correction as it reads
     // x, the Chebyshev step 1 re-adds it to its x0
-    const int kid_ = WPF_KERNEL_SMOOTH;
     for (int s = 0; s < o.sweeps; s++) {
         step_coef(o, s, &a, &w);
         const bool last = s + 1 == o.sweeps;
         const R* xo = (const R*)oth;
-        if (s == 0 && last) {
-            if (coarse)
-                LAUNCHZ((k_fsweep<R, C, true, true, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                        (const R*)cur, xo, pro, oth, RED(dot_slot));
-            else
-                LAUNCHZ((k_fsweep<R, C, true, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                        (const R*)cur, xo, none, oth, RED(dot_slot));
-        } else if (s == 0) {
-            if (coarse)
-                LAUNCHZ((k_fsweep<R, C, false, true, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                        (const R*)cur, xo, pro, oth, RED(dot_slot));
-            else
-                LAUNCHZ((k_fsweep<R, C, false, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                        (const R*)cur, xo, none, oth, RED(dot_slot));
-        } else if (s == 1 && a != 0.0 && coarse) {
+        if (s == 0 && last)
+            rc = coarse ? sweep<R, C, true, true, false>(c, F, a, w, cur, xo, pro, oth, dot_slot)
+                        : sweep<R, C, true, false, false>(c, F, a, w, cur, xo, none, oth, dot_slot);
+        else if (s == 0)
+            rc = coarse ? sweep<R, C, false, true, false>(c, F, a, w, cur, xo, pro, oth, dot_slot)
+                        : sweep<R, C, false, false, false>(c, F, a, w, cur, xo, none, oth, dot_slot);
+        else if (s == 1 && a != 0.0 && coarse)
             // x0 of this step is the pre-smoothed iterate plus the correction
-            if (last)
-                LAUNCHZ((k_fsweep<R, C, true, false, true>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                        (const R*)cur, xo, pro, oth, RED(dot_slot));
-            else
-                LAUNCHZ((k_fsweep<R, C, false, false, true>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                        (const R*)cur, xo, pro, oth, RED(dot_slot));
-        } else if (last) {
-            LAUNCHZ((k_fsweep<R, C, true, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                    (const R*)cur, xo, none, oth, RED(dot_slot));
-        } else {
-            LAUNCHZ((k_fsweep<R, C, false, false, false>), F.G, FINE_OP(F), (R)a, (R)w, (const R*)F.r,
-                    (const R*)cur, xo, none, oth, RED(dot_slot));
-        }
-        if (last) FINALIZE(zg_blocks(F.G), 1, dot_slot);
+            rc = last ? sweep<R, C, true, false, true>(c, F, a, w, cur, xo, pro, oth, dot_slot)
+                      : sweep<R, C, false, false, true>(c, F, a, w, cur, xo, pro, oth, dot_slot);
+        else
+            rc = last ? sweep<R, C, true, false, false>(c, F, a, w, cur, xo, none, oth, dot_slot)
+                      : sweep<R, C, false, false, false>(c, F, a, w, cur, xo, none, oth, dot_slot);
+        if (rc) return rc;
+        if (last) FINALIZE(fine_blocks(F), 1, dot_slot);
         std::swap(cur, oth);
     }
     *zres = cur;
@@ -813,13 +454,7 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
     double rn = -1.0;
     int first = 1;
     while (its < maxit) {
-        {
-            const int kid_ = WPF_KERNEL_MATVEC;
-            LAUNCHZ((k_dir_mv<R, C>), F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (const R*)zres,
-                    (const R*)pc, pn, F.q, (const double*)c->red.result, rz_cur, rz_prev, first,
-                    RED(S_PQ));
-        }
-        FINALIZE(zg_blocks(F.G), 1, S_PQ);
+        if (int rc = dir_mv(c, F, (const R*)zres, (const R*)pc, pn, rz_cur, rz_prev, first)) return rc;
         std::swap(pc, pn);
         {
             const int kid_ = WPF_KERNEL_CG_UPDATE;
@@ -911,6 +546,9 @@ static int prepare(wpf_ctx* c, const wpf_solver_opts& o, double* bnorm)
                            c->stream, c->g.n, valid, ratio, (const uint8_t*)c->mask, c->p.p,
                            (const double*)c->xprev.p, c->psave.p);
         WPF_HIP(hipGetLastError());
+        c->extrapolated = valid != 0;
+    } else {
+        c->extrapolated = false;
     }
     MGLevel<double> F;
     F.g = c->g; F.tx = c->tx; F.ty = c->ty; F.tz = c->tz; F.invD = c->invD;
@@ -940,8 +578,8 @@ __global__ void __launch_bounds__(256) k_cap_target(int64_t n, const double* __r
 static Fine<double, double> fine64(wpf_ctx* c)
 {
     Fine<double, double> F;
-    F.G = make_zg(c->g);
     F.g = c->g;
+    set_shape(F, c->tuned_zc[1] > 0 ? c->tuned_zc[1] : 8);
     F.tx = c->tx.p; F.ty = c->ty.p; F.tz = c->tz.p; F.invD = c->invD.p;
     F.r = c->r.p; F.Z = c->z.p; F.T = c->z2.p; F.pa = c->pp.p; F.pb = c->pp2.p; F.q = c->q.p;
     return F;
@@ -968,11 +606,11 @@ static int true_resid64(wpf_ctx* c, double* rn)
 // The stage takes the network's correction d as a search direction with the
 // exact line search alpha = (d, r) / (d, A d) on the FP64 operator and keeps
 // x + alpha d only if the FP64 residual norm drops.
-static int start_stage(wpf_ctx* c, const wpf_solver_opts& o, int* learned)
+static int start_stage(wpf_ctx* c, const wpf_solver_opts& o, int* learned, bool allow_net)
 {
     *learned = 0;
     const bool cap = (o.flags & WPF_FLAG_CAPTURE) != 0;
-    const bool net = (o.flags & (WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG)) && c->net;
+    const bool net = allow_net && (o.flags & (WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG)) && c->net;
     if (!cap && !net) return WPF_OK;
     const int64_t n = c->g.n;
     double rn0;
@@ -1018,6 +656,76 @@ static int start_stage(wpf_ctx* c, const wpf_solver_opts& o, int* learned)
     return WPF_OK;
 }
 
+// The learned stage pays only if the solves that use it need fewer inner
+// iterations than those that do not.  The first kGateProbe solves alternate
+// with and without it; afterwards it is used while its mean count stays 3%
+// below the other's, and every kGatePeriod solves the two are probed again
+// (kGateProbe solves, fresh means), so a change of regime is followed.
+static constexpr int kGateProbe = 8, kGatePeriod = 40;
+
+static bool gate_use(wpf_ctx* c)
+{
+    const int k = c->gate_calls % kGatePeriod;
+    if (k == 0) {
+        c->gate_on_n = c->gate_off_n = 0;
+        c->gate_on_its = c->gate_off_its = 0.0;
+    }
+    if (k < kGateProbe) return (k & 1) == 0;
+    if (c->gate_on_n == 0 || c->gate_off_n == 0) return true;
+    return c->gate_on_its / c->gate_on_n < 0.97 * (c->gate_off_its / c->gate_off_n);
+}
+
+static void gate_record(wpf_ctx* c, bool used, int its)
+{
+    if (c->gate_calls % kGatePeriod < kGateProbe) {
+        if (used) {
+            c->gate_on_n++;
+            c->gate_on_its += its;
+        } else {
+            c->gate_off_n++;
+            c->gate_off_its += its;
+        }
+    }
+    c->gate_calls++;
+}
+
+// e = alpha d, r -= alpha q with alpha = (d, r) / (d, A d) from the device
+// results (alpha = 0 unless positive and finite)
+__global__ void __launch_bounds__(kThreads) k_lin_start(int64_t n, const double* __restrict__ res, int num,
+                                                        int den, const float* __restrict__ d,
+                                                        const float* __restrict__ q, float* __restrict__ e,
+                                                        float* __restrict__ r)
+{
+    double al = res[num] / res[den];
+    if (!(al > 0.0) || !(al < 1e300)) al = 0.0;
+    const float a = (float)al;
+    const Chunk ch = my_chunk(n);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        e[c] = a * d[c];
+        r[c] = r[c] - a * q[c];
+    }
+}
+
+// Learned first stage of the mixed solver: the network's correction d of
+// the FP32 residual in F.r becomes the starting guess e = alpha d of the
+// first inner solve, alpha the exact line search (d, r) / (d, A d), which
+// lowers the A-norm of the error whenever d is not A-orthogonal to it.  The
+// inner and outer stopping tests that follow are unchanged.
+template <typename C>
+static int learned_start(wpf_ctx* c, const Fine<float, C>& F, double rn, float* e)
+{
+    const int64_t n = F.g.n;
+    float* d = F.T;
+    int rc = net_correction32(c, F.r, F.invD, rn, d);
+    if (rc) return rc;
+    if ((rc = dir_mv(c, F, (const float*)d, (const float*)d, F.pa, 0, 0, 1))) return rc;
+    LAUNCH1D(k_dot<float>, n, n, (const float*)F.r, (const float*)d, RED(S_LR));
+    FINALIZE(blocks_for(n), 1, S_LR);
+    LAUNCH1D(k_lin_start, n, n, (const double*)c->red.result, S_LR, S_PQ, (const float*)d,
+             (const float*)F.q, e, F.r);
+    return WPF_OK;
+}
+
 static void finish(wpf_ctx* c, const wpf_solver_opts& o)
 {
     if ((o.flags & WPF_FLAG_CAPTURE) && c->cap_target) {
@@ -1042,7 +750,7 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
     double bnorm;
     if ((rc = prepare(c, o, &bnorm))) return rc;
     int learned = 0;
-    if ((rc = start_stage(c, o, &learned))) return rc;
+    if ((rc = start_stage(c, o, &learned, true))) return rc;
     const double tol = std::max(o.rtol * bnorm, o.atol);
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
@@ -1055,6 +763,7 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
         if ((rc = coarsen_all(c, c->mg64))) return rc;
     }
     Fine<double, double> F = fine64(c);
+    if ((rc = tune_shape(c, F, &c->tuned_zc[1]))) return rc;
     double* x = c->p.p;
     int its = 0, restarts = 0;
     double rn = 0.0, rn0 = -1.0;
@@ -1101,7 +810,13 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
     double bnorm;
     if ((rc = prepare(c, o, &bnorm))) return rc;
     int learned = 0;
-    if ((rc = start_stage(c, o, &learned))) return rc;
+    if ((rc = start_stage(c, o, &learned, false))) return rc;
+    const bool want_net = (o.flags & (WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG)) && c->net;
+    // without an extrapolated guess the network is the only warm start and
+    // is always used; otherwise the probe decides
+    const bool gated = want_net && c->extrapolated;
+    const bool use_net = want_net && (!gated || gate_use(c));
+    bool learned_pending = use_net;
     const double tol = std::max(o.rtol * bnorm, o.atol);
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
@@ -1134,17 +849,20 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
     if ((rc = coarsen_all(c, c->mg32))) return rc;
     const MGOpts mo = mg_opts(o);
     Fine<float, float> F;
-    F.G = make_zg(c->g);
     F.g = c->g;
+    set_shape(F, 8);
     F.tx = c->mg32[0].tx.p; F.ty = c->mg32[0].ty.p; F.tz = c->mg32[0].tz.p;
     F.invD = c->mg32[0].invD.p;
     F.r = c->r32.p; F.Z = c->z32.p; F.T = c->t32.p; F.pa = c->p32.p; F.pb = c->p32b.p;
     F.q = c->q32.p;
     Fine<float, _Float16> H;
-    H.G = F.G; H.g = F.g;
+    H.G = F.G; H.V = F.V; H.vec = F.vec; H.g = F.g;
     H.tx = c->h_tx.p; H.ty = c->h_ty.p; H.tz = c->h_tz.p;
     H.invD = F.invD;
     H.r = F.r; H.Z = F.Z; H.T = F.T; H.pa = F.pa; H.pb = F.pb; H.q = F.q;
+    if ((rc = op16 ? tune_shape(c, H, &c->tuned_zc[0]) : tune_shape(c, F, &c->tuned_zc[0]))) return rc;
+    set_shape(F, c->tuned_zc[0]);
+    set_shape(H, c->tuned_zc[0]);
     float* e = c->x32.p;
     double* x = c->p.p;
     int its = 0, outer = 0;
@@ -1166,6 +884,12 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
         if (rn <= tol || its >= o.maxit || outer >= 50) break;
         outer++;
         LAUNCH1D(k_zero<float>, n, n, e);
+        if (learned_pending) {
+            learned_pending = false;
+            rc = op16 ? learned_start(c, H, rn, e) : learned_start(c, F, rn, e);
+            if (rc) return rc;
+            learned = 1;
+        }
         // invD32 r32 = invD64 r64: the inner norm is in the outer units
         const double inner_tol = std::max(o.inner_rtol * rn, 0.5 * tol);
         int k = 0;
@@ -1177,6 +901,7 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
         its += k;
         LAUNCH1D(k_add_f32, n, n, (const float*)e, x);
     }
+    if (gated) gate_record(c, use_net, its);
     st->iterations = its;
     st->outer = outer;
     st->rnorm0 = rn0;

@@ -131,11 +131,18 @@ struct wpf_ctx {
     wpf::Dev<float> r32, z32, p32, q32, x32, t32, p32b;
     wpf::Dev<_Float16> h_tx, h_ty, h_tz;      // FP16 fine faces of the inner solve
     double krock_max = 0.0;                   // largest uploaded rock permeability
+    // probe of the learned stage: solves with and without it, compared by
+    // their inner iteration counts
+    int gate_calls = 0, gate_on_n = 0, gate_off_n = 0;
+    double gate_on_its = 0.0, gate_off_its = 0.0;
+    int tuned_zc[2] = {0, 0};                 // planes per tile of the vector fine
+                                              // kernels (FP32, FP64); 0 = not tuned
     wpf::Dev<double> z2, pp2, psave, fsrc;    // second buffers, saved p, injection source
     bool have_mg64 = false, have_mg32 = false;
     int mg_levels_built = 0;
     double t_scale = 1.0;                     // FP32 operator scaling
     bool have_prev = false;                   // xprev holds the previous solution
+    bool extrapolated = false;                // this solve started from an extrapolation
     double dt_hist[2] = {0.0, 0.0};           // last two Euler step sizes
     // training-sample capture (WPF_FLAG_CAPTURE)
     float* cap_fields = nullptr;              // [3][n]
@@ -203,6 +210,8 @@ void free_solver(wpf_ctx* c);
 void free_learned(wpf_ctx* c);
 template <typename R>
 int net_correction(wpf_ctx* c, const double* r64, double rnorm, R* out);
+// the same correction from the FP32 residual of the mixed solver
+int net_correction32(wpf_ctx* c, const float* r32, const float* invD32, double rnorm, float* out);
 // network input fields ([3][n] FP32) of the current solver state
 int solver_input(wpf_ctx* c, const double* r64, double rnorm, float* fields);
 

@@ -146,6 +146,25 @@ def test_pressure_solve(gpu, solver, flags):
     g.close()
 
 
+@pytest.mark.parametrize("grid", [(30, 14, 10), (31, 14, 10)])
+@pytest.mark.parametrize("solver,flags", [("mgcg", CHEB), ("mixed", CHEB | OP16)])
+def test_pressure_solve_row_lengths(gpu, grid, solver, flags):
+    """x rows that do not split into whole 16-byte vectors (FP32: nx % 4,
+    FP64: nx % 2) take the one-cell-per-thread kernels."""
+    case, st = _state(grid=grid, steps=3)
+    g = _gpu_from(case, st)
+    p0 = st.pressure.copy()
+    g.upload("pressure", p0 * 0.5)
+    g.calcPermeability()
+    info = g.solvePressure(solver_opts(solver, rtol=TIGHT, maxit=20000, flags=flags))
+    assert info["converged"] == 1
+    st.calc_permeability()
+    st.pressure[...] = p0 * 0.5
+    st.solve_pressure(rtol=TIGHT, maxit=20000)
+    assert np.abs(g.pressure - st.pressure).max() <= 1e-10 * np.abs(st.pressure).max()
+    g.close()
+
+
 def test_fused_step_matches_procedures(gpu):
     case, st = _state(steps=3)
     a = _gpu_from(case, st)

@@ -143,6 +143,73 @@ def calibrate_copy(reps=20):
           flush=True)
 
 
+def bench_ensemble(a):
+    """The ensemble workload (48 uneven grids, 40,894,464 cells): the same
+    steps once member by member with wpf_step (per-instance launches, one
+    host round trip per CG iteration) and once as one ensemble (batched
+    launches over all members, captured graphs).  Prints one JSON line."""
+    import numpy as np
+    from mizuho_edit_amd import io
+    from mizuho_edit_amd._lib import FLAG_CHEBYSHEV, FLAG_EXTRAPOLATE, FLAG_OP16
+    from mizuho_edit_amd.ensemble import Ensemble
+    from mizuho_edit_amd.wpfaw3d import WPFAW3D, solver_opts
+    log(f"building the ensemble workload (shrink {a.ens_shrink})")
+    cases = io.ensemble_cases(shrink=a.ens_shrink)
+    cells = sum(c.params.nx * c.params.ny * c.params.nz for c in cases)
+    opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4,
+                       flags=FLAG_EXTRAPOLATE | FLAG_CHEBYSHEV | FLAG_OP16)
+    res = {}
+    for mode in ("per_instance", "ensemble"):
+        grids = [WPFAW3D(c, device=0, opts=opts) for c in cases]
+        for g in grids:
+            g.run(a.warmup)
+        sync()
+        if mode == "per_instance":
+            t0 = time.perf_counter()
+            for s in range(a.steps):
+                for g in grids:
+                    g.step()
+                log(f"  per-instance step {s + 1}/{a.steps}")
+            sync()
+            wall = time.perf_counter() - t0
+            info = {}
+        else:
+            ens = Ensemble(grids, opts)
+            tg, tz = ens.tune()
+            ens.step(1)  # captures the graphs
+            log(f"  ensemble tuned: {tg} workgroups, {tz} planes per tile")
+            st = ens.step(a.steps)
+            wall = st["seconds"]
+            ens.sync()
+            info = {"grid_blocks": tg, "tile_depth": tz, "fallbacks": st["fallbacks"],
+                    "fp32_iterations_per_member_step": st["iterations"] / (a.steps * len(grids))}
+            ens.close()
+        res[mode] = (wall, info)
+        for g in grids:
+            g.close()
+    wall_i, _ = res["per_instance"]
+    wall_e, info = res["ensemble"]
+    out = {
+        "metric": "aggregate cell-updates/s, ensemble of 48 uneven grids",
+        "value": cells * a.steps / wall_e,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "higher_is_better": True,
+        "per_instance_value": cells * a.steps / wall_i,
+        "speedup_vs_per_instance": wall_i / wall_e,
+        "ms_per_step": 1e3 * wall_e / a.steps,
+        "dtype": "f64 stencils + f32 solve (FP16-stored faces) in f64 refinement",
+        "data": "synthetic: 48 seeded GRF rocks, correlation length 2-8 cells, mean porosity 0.12-0.25",
+        "config": {"workload": "ensemble: 16x64^3 + 16x96x64x64 + 8x128x96x96 + 8x160x128x128"
+                               + (f" (each dimension / {a.ens_shrink})" if a.ens_shrink > 1 else ""),
+                   "cells": cells, "solver": "FP32 MG-PCG (Chebyshev) in FP64 refinement, rtol 1e-10",
+                   **info},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,9 +226,18 @@ def main():
                     help="fast path with the plain path's smoother and FP32 faces")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 1 GiB copy kernel and exit")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="the 48-grid ensemble workload instead of the 256^3 grid")
+    ap.add_argument("--ens-shrink", type=int, default=1, help="divide ensemble grid sizes by this")
     a = ap.parse_args()
     if a.calibrate_copy:
         calibrate_copy()
+        return
+    if a.ensemble:
+        if a.gpus != 1:
+            print("bench.py: one GPU only", file=sys.stderr)
+            sys.exit(2)
+        bench_ensemble(a)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the hot path on one GPU only; multi-GPU runs are "

@@ -266,6 +266,52 @@ int wpf_capture_get(wpf_ctx* ctx, float* fields, float* target);
 #define WPF_KERNEL_CG_UPDATE   7  /* fine-level x/r update                  */
 int wpf_profile_start(wpf_ctx* ctx, int32_t kernel);
 int wpf_profile_stop(wpf_ctx* ctx, double* total_ms, int32_t* count);
+/* Planes per tile of the context's vector solver kernels (normally chosen
+ * by timing on the first solve); a value > 0 fixes it. */
+int wpf_set_tile_depth(wpf_ctx* ctx, int32_t planes);
+
+/* ---- ensembles -------------------------------------------------------------
+ * No counterpart in the reference (which runs one realisation per job): many
+ * independent grids of different sizes on one GPU, stepped together.  Every
+ * operation of a time step is one launch over the blocks of all members
+ * (a fixed, tuned grid of workgroups walking a table of member blocks); the
+ * pressure solve runs a fixed schedule of FP64 refinement stages and FP32
+ * MG-PCG iterations with per-member convergence decided on the device, so a
+ * whole step is replayed from captured HIP graphs with one host
+ * synchronisation.  A member the schedule leaves unconverged is finished by
+ * the single-grid solver before its fields are advanced.  Each member
+ * follows the arithmetic of the same grid stepped alone with wpf_step and
+ * the same options (WPF_SOLVER_MIXED; WPF_FLAG_CHEBYSHEV, _OP16 and
+ * _EXTRAPOLATE honoured; FP64 stencils; no learned stage).
+ *
+ * Members are contexts created by wpf_create on one device, each nx a
+ * multiple of 4; they stay owned by the caller and must outlive the
+ * ensemble.  itime/time/dtime give each member's time-loop state (steps
+ * done, time, dt of the last step); wpf_ensemble_sync returns it and hands
+ * the members back for single-grid use. */
+typedef struct wpf_ensemble wpf_ensemble;
+typedef struct wpf_ensemble_opts {
+    int32_t stages;       /* refinement stages per solve in the schedule (3)  */
+    int32_t inner_iters;  /* FP32 iterations per stage in the schedule (16)   */
+    int32_t grid_blocks;  /* workgroups of every batched launch (0: tune)     */
+    int32_t tile_depth;   /* planes per tile of the vector kernels (0: tune)  */
+} wpf_ensemble_opts;
+typedef struct wpf_ensemble_stats {
+    int32_t steps;        /* steps of every member                            */
+    int32_t fallbacks;    /* member solves finished by the single-grid solver */
+    int64_t iterations;   /* FP32 inner iterations, all members and steps     */
+    double  seconds;      /* wall time of the steps                           */
+} wpf_ensemble_stats;
+int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opts* opts,
+                        const wpf_ensemble_opts* eopts, const int32_t* itime,
+                        const double* time, const double* dtime, wpf_ensemble** out);
+/* Autotuner: times the batched smoothing sweep over the members for each
+ * candidate (workgroups, planes per tile) and keeps the fastest. */
+int wpf_ensemble_tune(wpf_ensemble* e, int32_t* grid_blocks, int32_t* tile_depth);
+int wpf_ensemble_step(wpf_ensemble* e, int32_t nsteps, wpf_ensemble_stats* stats);
+int wpf_ensemble_sync(wpf_ensemble* e, int32_t* itime, double* time, double* dtime,
+                      double* taumin);
+int wpf_ensemble_destroy(wpf_ensemble* e);
 
 #ifdef __cplusplus
 }

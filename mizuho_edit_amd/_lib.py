@@ -57,6 +57,19 @@ class WpfSolveStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class WpfEnsembleOpts(ctypes.Structure):
+    _fields_ = [("stages", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
+                ("grid_blocks", ctypes.c_int32), ("tile_depth", ctypes.c_int32)]
+
+
+class WpfEnsembleStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int32), ("fallbacks", ctypes.c_int32),
+                ("iterations", ctypes.c_int64), ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class WpfError(RuntimeError):
     def __init__(self, status: int, where: str, msg: str):
         super().__init__(f"{where}: status {status}: {msg}")
@@ -119,6 +132,14 @@ def lib():
         "wpf_net_train": ([P, I32, I32, I32, I32, F, F, I32, ctypes.c_float, D], I32),
         "wpf_attach_net": ([P, P], I32),
         "wpf_capture_get": ([P, F, F], I32),
+        "wpf_set_tile_depth": ([P, I32], I32),
+        "wpf_ensemble_create": ([ctypes.POINTER(P), I32, ctypes.POINTER(WpfSolverOpts),
+                                 ctypes.POINTER(WpfEnsembleOpts), ctypes.POINTER(I32), D, D,
+                                 ctypes.POINTER(P)], I32),
+        "wpf_ensemble_tune": ([P, ctypes.POINTER(I32), ctypes.POINTER(I32)], I32),
+        "wpf_ensemble_step": ([P, I32, ctypes.POINTER(WpfEnsembleStats)], I32),
+        "wpf_ensemble_sync": ([P, ctypes.POINTER(I32), D, D, D], I32),
+        "wpf_ensemble_destroy": ([P], I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

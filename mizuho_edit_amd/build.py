@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwpf_hip.so")
-SOURCES = ["wpf_api.hip", "k_fields.hip", "k_solver.hip", "k_learned.hip", "wpf_io.cpp"]
+SOURCES = ["wpf_api.hip", "k_fields.hip", "k_solver.hip", "k_ensemble.hip", "k_learned.hip", "wpf_io.cpp"]
 ARCH = os.environ.get("WPF_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -16,6 +16,7 @@
 // which is exactly what updateHalo leaves in the halo on one rank.
 #include "wpf_internal.h"
 #include "phase_cell.h"
+#include "ensemble.h"
 
 #include <algorithm>
 #include <cmath>
@@ -23,26 +24,7 @@
 
 namespace wpf {
 
-static PhaseConsts make_consts(const wpf_params& p)
-{
-    PhaseConsts K;
-    K.dx = p.dx;
-    K.two_dx = 2.0 * p.dx;
-    K.dx2 = p.dx * p.dx;
-    K.four_dx2 = 4.0 * (p.dx * p.dx);
-    K.w0 = 2.0 * p.dx;
-    K.maxdphi = 2.0 / (2.0 * p.dx);
-    K.eps = p.epsilon;
-    K.lambda = p.lambda;
-    K.diffusion = p.diffusion;
-    K.ral = p.ral;
-    K.rat = p.rat;
-    K.tau_num = 5.0 / 3.0 * p.alpha * p.lambda * (p.epsilon * p.epsilon);
-    K.alpha = p.alpha;
-    K.kwh = p.permeab_hole;
-    K.inv_mu_unused = 0.0;
-    return K;
-}
+static PhaseConsts make_consts(const wpf_params& p) { return phase_consts(p); }
 
 struct Dims {
     int nx, ny, nz;
@@ -69,17 +51,16 @@ __device__ __forceinline__ double hmean(double a, double b) { return 2.0 / (1.0 
 // its cell (the -faces exactly as the neighbour computes them, same operand
 // order) so the diagonal comes out of the same pass.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_perm(Dims d, double kwh, double inv_mu_dummy, double mu,
-                                              const double* __restrict__ phi,
-                                              const double* __restrict__ kr,
-                                              const uint8_t* __restrict__ mask,
-                                              double* __restrict__ tx, double* __restrict__ ty,
-                                              double* __restrict__ tz, double* __restrict__ invD)
+// block (bx, by, bz) of 64 x 4 x 1 threads
+__device__ __forceinline__ void t_perm(const Dims& d, int bx, int by, int bz, double kwh, double mu,
+                                       const double* __restrict__ phi, const double* __restrict__ kr,
+                                       const uint8_t* __restrict__ mask, double* __restrict__ tx,
+                                       double* __restrict__ ty, double* __restrict__ tz,
+                                       double* __restrict__ invD)
 {
-    (void)inv_mu_dummy;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k = bz * blockDim.z + threadIdx.z;
     if (i >= d.nx || j >= d.ny || k >= d.nz) return;
     const int64_t c = i + (int64_t)d.nx * (j + (int64_t)d.ny * k);
     const int ip = min(i + 1, d.nx - 1), im = max(i - 1, 0);
@@ -112,13 +93,21 @@ __global__ void __launch_bounds__(256) k_perm(Dims d, double kwh, double inv_mu_
     invD[c] = (fixed || dsum == 0.0) ? 0.0 : 1.0 / dsum;
 }
 
+__global__ void __launch_bounds__(256) k_perm(Dims d, double kwh, double mu, const double* __restrict__ phi,
+                                              const double* __restrict__ kr,
+                                              const uint8_t* __restrict__ mask, double* __restrict__ tx,
+                                              double* __restrict__ ty, double* __restrict__ tz,
+                                              double* __restrict__ invD)
+{
+    t_perm(d, blockIdx.x, blockIdx.y, blockIdx.z, kwh, mu, phi, kr, mask, tx, ty, tz, invD);
+}
+
 int launch_perm(wpf_ctx* c)
 {
     const Dims d = dims_of(c);
     dim3 blk(64, 4, 1);
     prof_mark(c, WPF_KERNEL_PERM, true);
-    hipLaunchKernelGGL(k_perm, grid3(d, blk), blk, 0, c->stream, d, c->prm.permeab_hole, 0.0,
-                       c->prm.viscosity, c->phi[c->cur], c->krock, c->mask, c->tx.p, c->ty.p,
+    hipLaunchKernelGGL(k_perm, grid3(d, blk), blk, 0, c->stream, d, c->prm.permeab_hole, c->prm.viscosity, c->phi[c->cur], c->krock, c->mask, c->tx.p, c->ty.p,
                        c->tz.p, c->invD.p);
     prof_mark(c, WPF_KERNEL_PERM, false);
     WPF_HIP(hipGetLastError());
@@ -380,28 +369,25 @@ struct StepScalars {
 // FP32 (face velocities from the FP64 pressure difference, rounded once)
 // while phi, c, the Euler update and taumin stay FP64.
 template <typename T>
-__global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConstsT<T> K, double dx, StepScalars S,
-                                                     const double* __restrict__ phi,
-                                                     const double* __restrict__ conc,
-                                                     const double* __restrict__ p,
-                                                     const double* __restrict__ tx,
-                                                     const double* __restrict__ ty,
-                                                     const double* __restrict__ tz,
-                                                     const uint8_t* __restrict__ mask,
-                                                     double* __restrict__ phi_out,
-                                                     double* __restrict__ conc_out,
-                                                     unsigned long long* __restrict__ keys,
-                                                     double* __restrict__ scal,
-                                                     int32_t* __restrict__ bt_step)
+__device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int bz, const PhaseConstsT<T>& K,
+                                              double dx, const StepScalars& S,
+                                              const double* __restrict__ phi,
+                                              const double* __restrict__ conc,
+                                              const double* __restrict__ p, const double* __restrict__ tx,
+                                              const double* __restrict__ ty, const double* __restrict__ tz,
+                                              const uint8_t* __restrict__ mask, double* __restrict__ phi_out,
+                                              double* __restrict__ conc_out,
+                                              unsigned long long* __restrict__ keys,
+                                              double* __restrict__ scal, int32_t* __restrict__ bt_step)
 {
     double dt = S.dt_cfg;
     if (S.device_dt && S.itime >= 3) {
         const double taumin_prev = dkey_inv(keys[S.key_prev]);
         dt = S.dt_pref * fmin(S.eps2_D, taumin_prev);
     }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z * blockDim.z + threadIdx.z;
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k = bz * blockDim.z + threadIdx.z;
     T tau = (T)1.0e6;
     if (i < d.nx && j < d.ny && k < d.nz) {
         const Nb n = neighbours(d, i, j, k);
@@ -434,13 +420,31 @@ __global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConstsT<T> K, 
         if (i == S.bt_plane && !well && phn > 0.0) atomicMin(bt_step, S.itime);
     }
     block_min_key((double)tau, &keys[S.key_cur]);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
-        threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
+    if (bx == 0 && by == 0 && bz == 0 && threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
         // key_next was last read two steps ago: reset it for the next step
         keys[S.key_next] = dkey(1.0e6);
         scal[0] = scal[0] + dt;  // time
         scal[1] = dt;            // dtime
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConstsT<T> K, double dx, StepScalars S,
+                                                     const double* __restrict__ phi,
+                                                     const double* __restrict__ conc,
+                                                     const double* __restrict__ p,
+                                                     const double* __restrict__ tx,
+                                                     const double* __restrict__ ty,
+                                                     const double* __restrict__ tz,
+                                                     const uint8_t* __restrict__ mask,
+                                                     double* __restrict__ phi_out,
+                                                     double* __restrict__ conc_out,
+                                                     unsigned long long* __restrict__ keys,
+                                                     double* __restrict__ scal,
+                                                     int32_t* __restrict__ bt_step)
+{
+    t_step_fields<T>(d, blockIdx.x, blockIdx.y, blockIdx.z, K, dx, S, phi, conc, p, tx, ty, tz, mask, phi_out,
+                     conc_out, keys, scal, bt_step);
 }
 
 int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt, bool f32)
@@ -590,6 +594,73 @@ int check_breakthrough(wpf_ctx* c, int itime)
     WPF_HIP(hipMemcpyAsync(&v, c->bt_step, 4, hipMemcpyDeviceToHost, c->stream));
     WPF_HIP(hipStreamSynchronize(c->stream));
     c->bt_host = (v == 0x7fffffff) ? 0 : v;
+    return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble forms: one launch over the blocks of every member (see
+// ensemble.h); each tile is the block (bx, by, bz) of the member's own
+// 64 x 4 x 1 launch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ens_block3(const EnsMember& M, int b, int& bx, int& by, int& bz)
+{
+    bx = b % M.gx3;
+    by = (b / M.gx3) % M.gy3;
+    bz = b / (M.gx3 * M.gy3);
+}
+
+__global__ void __launch_bounds__(256) ke_perm(const EnsMember* __restrict__ Ms,
+                                               const EnsTile* __restrict__ tiles, int ntiles)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const EnsTile T = tiles[t];
+        if (T.b < 0) continue;
+        const EnsMember& M = Ms[T.m];
+        int bx, by, bz;
+        ens_block3(M, T.b, bx, by, bz);
+        const Dims d{M.nx, M.ny, M.nz, M.plane};
+        t_perm(d, bx, by, bz, M.kwh, M.mu, M.phi[M.st->cur], M.krock, M.mask, M.tx, M.ty, M.tz, M.invD);
+    }
+}
+
+// the fused step with the device dt rule, every member at its own step
+__global__ void __launch_bounds__(256) ke_step_fields(const EnsMember* __restrict__ Ms,
+                                                      const EnsTile* __restrict__ tiles, int ntiles)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const EnsTile T = tiles[t];
+        if (T.b < 0) continue;
+        const EnsMember& M = Ms[T.m];
+        int bx, by, bz;
+        ens_block3(M, T.b, bx, by, bz);
+        const Dims d{M.nx, M.ny, M.nz, M.plane};
+        const int itime = M.st->itime, cur = M.st->cur;
+        StepScalars S;
+        S.dt_pref = M.dt_pref;
+        S.eps2_D = M.eps2_D;
+        S.dt_cfg = M.dt_cfg;
+        S.itime = itime;
+        S.device_dt = 1;
+        S.bt_plane = M.bt_plane;
+        S.key_prev = (itime + 2) % 3;
+        S.key_cur = itime % 3;
+        S.key_next = (itime + 1) % 3;
+        t_step_fields<double>(d, bx, by, bz, M.K, M.dx, S, M.phi[cur], M.conc[cur], M.p, M.tx, M.ty, M.tz,
+                              M.mask, M.phi[1 - cur], M.conc[1 - cur], M.tau_key, M.scal, M.bt_step);
+    }
+}
+
+int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid)
+{
+    hipLaunchKernelGGL(ke_perm, dim3(grid), dim3(64, 4, 1), 0, s, Ms, tiles, ntiles);
+    WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
+int ens_step_fields(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid)
+{
+    hipLaunchKernelGGL(ke_step_fields, dim3(grid), dim3(64, 4, 1), 0, s, Ms, tiles, ntiles);
+    WPF_HIP(hipGetLastError());
     return WPF_OK;
 }
 

@@ -41,12 +41,11 @@ inline ZG make_zg(const Grid& g)
     z.gz = (g.nz + kFZ - 1) / kFZ;
     return z;
 }
-inline int zg_blocks(const ZG& z) { return z.gx * z.gy * z.gz; }
+__host__ __device__ inline int zg_blocks(const ZG& z) { return z.gx * z.gy * z.gz; }
 
-__device__ __forceinline__ bool zm_tile(const ZG& G, int& i, int& j, int& k0, int& k1)
+__device__ __forceinline__ bool zm_at(const ZG& G, int b, int& i, int& j, int& k0, int& k1)
 {
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
+    const int nb = G.gx * G.gy * G.gz;
     const int lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
     const int bx = lb % G.gx;
     const int by = (lb / G.gx) % G.gy;
@@ -56,6 +55,10 @@ __device__ __forceinline__ bool zm_tile(const ZG& G, int& i, int& j, int& k0, in
     k0 = bz * kFZ;
     k1 = min(k0 + kFZ, G.nz);
     return i < G.nx && j < G.ny;
+}
+__device__ __forceinline__ bool zm_tile(const ZG& G, int& i, int& j, int& k0, int& k1)
+{
+    return zm_at(G, blockIdx.x, i, j, k0, k1);
 }
 
 // The fine operator's face coefficients are stored as C (double, float or
@@ -186,17 +189,15 @@ __global__ void __launch_bounds__(256) k_fsweep(ZG G, const C* __restrict__ tx, 
 // (b - A x) on free cells.  z pairs are summed in registers, x pairs with a
 // lane shuffle, y pairs through LDS; the (even, even) child writes.
 template <typename R, typename C>
-__global__ void __launch_bounds__(256) k_fresid_restrict(ZG G, const C* __restrict__ tx,
-                                                         const C* __restrict__ ty,
-                                                         const C* __restrict__ tz,
-                                                         const uint8_t* __restrict__ mask,
-                                                         const R* __restrict__ b,
-                                                         const R* __restrict__ x, int cnx, int cny,
-                                                         R* __restrict__ bc)
+__device__ __forceinline__ void t_fresid_restrict(const ZG& G, int bi, const C* __restrict__ tx,
+                                                  const C* __restrict__ ty, const C* __restrict__ tz,
+                                                  const uint8_t* __restrict__ mask, const R* __restrict__ b,
+                                                  const R* __restrict__ x, int cnx, int cny,
+                                                  R* __restrict__ bc)
 {
     __shared__ R sh[kFY][kFX];
     int i, j, k0, k1;
-    const bool act = zm_tile(G, i, j, k0, k1);
+    const bool act = zm_at(G, bi, i, j, k0, k1);
     int64_t c = (int64_t)min(i, G.nx - 1) + (int64_t)G.nx * (min(j, G.ny - 1) + (int64_t)G.ny * k0);
     R xm = act ? x[c - G.plane] : (R)0, x0 = act ? x[c] : (R)0;
     R tzm = act ? (R)tz[c - G.plane] : (R)0;
@@ -230,16 +231,26 @@ __global__ void __launch_bounds__(256) k_fresid_restrict(ZG G, const C* __restri
     }
 }
 
+template <typename R, typename C>
+__global__ void __launch_bounds__(256) k_fresid_restrict(ZG G, const C* __restrict__ tx,
+                                                         const C* __restrict__ ty,
+                                                         const C* __restrict__ tz,
+                                                         const uint8_t* __restrict__ mask,
+                                                         const R* __restrict__ b,
+                                                         const R* __restrict__ x, int cnx, int cny,
+                                                         R* __restrict__ bc)
+{
+    t_fresid_restrict<R, C>(G, blockIdx.x, tx, ty, tz, mask, b, x, cnx, cny, bc);
+}
+
 // Coarsest level: nsweep damped-Jacobi sweeps from 0 in one workgroup, the
 // iterate ping-ponging between two zero-padded LDS buffers.
 template <typename R>
-__global__ void __launch_bounds__(1024) k_coarsest_lds(int nx, int ny, int nz, const R* __restrict__ tx,
-                                                       const R* __restrict__ ty,
-                                                       const R* __restrict__ tz,
-                                                       const R* __restrict__ invD, R w, int nsweep,
-                                                       const R* __restrict__ b, R* __restrict__ xout)
+__device__ __forceinline__ void t_coarsest_lds(int nx, int ny, int nz, const R* __restrict__ tx,
+                                               const R* __restrict__ ty, const R* __restrict__ tz,
+                                               const R* __restrict__ invD, R w, int nsweep,
+                                               const R* __restrict__ b, R* __restrict__ xout, char* smem_raw)
 {
-    extern __shared__ char smem_raw[];
     const int64_t plane = (int64_t)nx * ny, n = plane * nz;
     const int64_t pad = plane;
     R* xa = reinterpret_cast<R*>(smem_raw) + pad;
@@ -266,18 +277,27 @@ __global__ void __launch_bounds__(1024) k_coarsest_lds(int nx, int ny, int nz, c
     for (int64_t c = threadIdx.x; c < n; c += blockDim.x) xout[c] = xa[c];
 }
 
-// Same, for at most 4*1024 cells: each thread keeps the coefficients and
-// right-hand side of its (up to 4) cells in registers, so a sweep touches
-// only LDS.
 template <typename R>
-__global__ void __launch_bounds__(1024) k_coarsest_reg(int nx, int ny, int nz, const R* __restrict__ tx,
+__global__ void __launch_bounds__(1024) k_coarsest_lds(int nx, int ny, int nz, const R* __restrict__ tx,
                                                        const R* __restrict__ ty,
                                                        const R* __restrict__ tz,
                                                        const R* __restrict__ invD, R w, int nsweep,
                                                        const R* __restrict__ b, R* __restrict__ xout)
 {
-    constexpr int S = 4;
     extern __shared__ char smem_raw[];
+    t_coarsest_lds<R>(nx, ny, nz, tx, ty, tz, invD, w, nsweep, b, xout, smem_raw);
+}
+
+// Same, for at most 4*1024 cells: each thread keeps the coefficients and
+// right-hand side of its (up to 4) cells in registers, so a sweep touches
+// only LDS.
+template <typename R>
+__device__ __forceinline__ void t_coarsest_reg(int nx, int ny, int nz, const R* __restrict__ tx,
+                                               const R* __restrict__ ty, const R* __restrict__ tz,
+                                               const R* __restrict__ invD, R w, int nsweep,
+                                               const R* __restrict__ b, R* __restrict__ xout, char* smem_raw)
+{
+    constexpr int S = 4;
     const int plane = nx * ny, n = plane * nz;
     const int pad = plane;
     R* xa = reinterpret_cast<R*>(smem_raw) + pad;
@@ -330,25 +350,31 @@ __global__ void __launch_bounds__(1024) k_coarsest_reg(int nx, int ny, int nz, c
     }
 }
 
+template <typename R>
+__global__ void __launch_bounds__(1024) k_coarsest_reg(int nx, int ny, int nz, const R* __restrict__ tx,
+                                                       const R* __restrict__ ty,
+                                                       const R* __restrict__ tz,
+                                                       const R* __restrict__ invD, R w, int nsweep,
+                                                       const R* __restrict__ b, R* __restrict__ xout)
+{
+    extern __shared__ char smem_raw[];
+    t_coarsest_reg<R>(nx, ny, nz, tx, ty, tz, invD, w, nsweep, b, xout, smem_raw);
+}
+
 // x += alpha p; r -= alpha q (alpha = res[num]/res[den]); z = w invD r.
 // Reduces ||invD r||^2 and (r, z).
 template <typename R, typename X>
-__global__ void __launch_bounds__(256) k_update(int64_t n, const double* __restrict__ res, int num,
-                                                int den, R w, X* __restrict__ x,
-                                                const R* __restrict__ p, R* __restrict__ r,
-                                                const R* __restrict__ q, const R* __restrict__ invD,
-                                                R* __restrict__ z, double* __restrict__ partials,
-                                                unsigned* __restrict__ counter,
-                                                double* __restrict__ result)
+__device__ __forceinline__ void t_update(int64_t n, const double* __restrict__ res, int num, int den, R w,
+                                         X* __restrict__ x, const R* __restrict__ p, R* __restrict__ r,
+                                         const R* __restrict__ q, const R* __restrict__ invD,
+                                         R* __restrict__ z, double (&acc)[2], int bb, int nb)
 {
     const double alpha = res[num] / res[den];
     const R a = (R)alpha;
-    const int nb = gridDim.x;
-    const int bb = blockIdx.x;
     const int lb = (nb % 8 == 0) ? (bb % 8) * (nb / 8) + bb / 8 : bb;
     const int64_t per = ((n + nb - 1) / nb + 255) / 256 * 256;
     const int64_t beg = (int64_t)lb * per, end = min(n, beg + per);
-    double acc[2] = {0.0, 0.0};
+    acc[0] = acc[1] = 0.0;
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;
     for (int64_t c = beg + tid; c < end; c += 256) {
         x[c] = x[c] + (X)(a * p[c]);
@@ -360,12 +386,25 @@ __global__ void __launch_bounds__(256) k_update(int64_t n, const double* __restr
         acc[0] += (double)zr * (double)zr;
         acc[1] += (double)rv * (double)(w * zr);
     }
+}
+
+template <typename R, typename X>
+__global__ void __launch_bounds__(256) k_update(int64_t n, const double* __restrict__ res, int num,
+                                                int den, R w, X* __restrict__ x,
+                                                const R* __restrict__ p, R* __restrict__ r,
+                                                const R* __restrict__ q, const R* __restrict__ invD,
+                                                R* __restrict__ z, double* __restrict__ partials,
+                                                unsigned* __restrict__ counter,
+                                                double* __restrict__ result)
+{
+    double acc[2];
+    t_update<R, X>(n, res, num, den, w, x, p, r, q, invD, z, acc, blockIdx.x, gridDim.x);
     block_reduce_store<2>(acc, partials, counter, result, gridDim.x, blockIdx.x);
 }
 
 // Initial guess from the two previous solutions: save the current pressure,
 // then p = p + ratio (p - pprev) on free cells when a previous one exists.
-__global__ void __launch_bounds__(256) k_extrapolate(int64_t n, int valid, double ratio,
+static __global__ void __launch_bounds__(256) k_extrapolate(int64_t n, int valid, double ratio,
                                                      const uint8_t* __restrict__ mask,
                                                      double* __restrict__ p,
                                                      const double* __restrict__ pprev,

@@ -494,4 +494,38 @@ static __global__ void k_set_inj(int n, const int32_t* __restrict__ idx, const d
 }
 
 
+// Smoother of every level: damped Jacobi (w1 = w2 = omega, a2 = 0) or the
+// degree-2 Chebyshev polynomial in D^-1 A on [0.5, 2] (the spectrum of
+// D^-1 A of these M-matrices lies in [0, 2]): step 1 x1 = x0 + w1 D^-1 r0,
+// step 2 x2 = x1 + a2 (x1 - x0) + w2 D^-1 r1.  The same polynomial before
+// and after the coarse correction keeps the V-cycle symmetric.
+struct MGOpts {
+    int sweeps;
+    double cscale;
+    int coarse_sweeps;
+    double omega;      // coarsest-level Jacobi weight
+    double w1, a2, w2;
+};
+
+static MGOpts mg_opts(const wpf_solver_opts& o)
+{
+    MGOpts m{o.mg_sweeps, o.mg_coarse_scale, 48, o.mg_omega, o.mg_omega, 0.0, o.mg_omega};
+    if (o.flags & WPF_FLAG_CHEBYSHEV) {
+        const double hi = 2.0, lo = 0.5;
+        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+        const double rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+        m.w1 = 1.0 / theta;
+        m.a2 = rho1 * rho0;
+        m.w2 = 2.0 * rho1 / delta;
+    }
+    return m;
+}
+
+// (a, w) of smoothing step s (0-based) of a pre- or post-smoothing sequence
+static inline void step_coef(const MGOpts& o, int s, double* a, double* w)
+{
+    *a = (s == 1) ? o.a2 : 0.0;
+    *w = (s == 0) ? o.w1 : (s == 1 ? o.w2 : o.omega);
+}
+
 }  // namespace wpf

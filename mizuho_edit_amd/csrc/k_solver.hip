@@ -146,40 +146,6 @@ static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv)
     return WPF_OK;
 }
 
-// Smoother of every level: damped Jacobi (w1 = w2 = omega, a2 = 0) or the
-// degree-2 Chebyshev polynomial in D^-1 A on [0.5, 2] (the spectrum of
-// D^-1 A of these M-matrices lies in [0, 2]): step 1 x1 = x0 + w1 D^-1 r0,
-// step 2 x2 = x1 + a2 (x1 - x0) + w2 D^-1 r1.  The same polynomial before
-// and after the coarse correction keeps the V-cycle symmetric.
-struct MGOpts {
-    int sweeps;
-    double cscale;
-    int coarse_sweeps;
-    double omega;      // coarsest-level Jacobi weight
-    double w1, a2, w2;
-};
-
-static MGOpts mg_opts(const wpf_solver_opts& o)
-{
-    MGOpts m{o.mg_sweeps, o.mg_coarse_scale, 48, o.mg_omega, o.mg_omega, 0.0, o.mg_omega};
-    if (o.flags & WPF_FLAG_CHEBYSHEV) {
-        const double hi = 2.0, lo = 0.5;
-        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-        const double rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
-        m.w1 = 1.0 / theta;
-        m.a2 = rho1 * rho0;
-        m.w2 = 2.0 * rho1 / delta;
-    }
-    return m;
-}
-
-// (a, w) of smoothing step s (0-based) of a pre- or post-smoothing sequence
-static inline void step_coef(const MGOpts& o, int s, double* a, double* w)
-{
-    *a = (s == 1) ? o.a2 : 0.0;
-    *w = (s == 0) ? o.w1 : (s == 1 ? o.w2 : o.omega);
-}
-
 template <typename R>
 static int coarsest(wpf_ctx* c, MGLevel<R>& L, const R* b, R* x, R* tmp, const MGOpts& o)
 {
@@ -802,7 +768,7 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
 }
 
 // FP32 MG-PCG inside FP64 iterative refinement.
-static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
+static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool cont = false)
 {
     const int64_t n = c->g.n;
     int rc = ensure_ws(c, true);
@@ -909,7 +875,7 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st)
     st->bnorm = bnorm;
     st->converged = rn <= tol;
     st->learned = learned;
-    finish(c, o);
+    if (!cont) finish(c, o);
     return st->converged ? WPF_OK : WPF_ERR_NOCONV;
 }
 
@@ -938,6 +904,29 @@ int solve_pressure(wpf_ctx* c, const wpf_solver_opts* in, wpf_solve_stats* st)
     WPF_HIP(hipStreamSynchronize(c->stream));
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     c->have_pressure = true;
+    return rc;
+}
+
+void solver_defaults(const wpf_solver_opts* in, wpf_solver_opts* out) { fill_defaults(in, *out); }
+
+int solver_workspace(wpf_ctx* c, const wpf_solver_opts* o)
+{
+    int rc = ensure_ws(c, true);
+    if (rc) return rc;
+    return ensure_hierarchy(c->mg32, o->mg_levels);
+}
+
+int solve_pressure_continue(wpf_ctx* c, const wpf_solver_opts* in, wpf_solve_stats* st)
+{
+    wpf_solver_opts o;
+    fill_defaults(in, o);
+    o.flags &= ~(WPF_FLAG_EXTRAPOLATE | WPF_FLAG_CAPTURE | WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG);
+    wpf_solve_stats local{};
+    if (!st) st = &local;
+    *st = wpf_solve_stats{};
+    int rc = mixed(c, o, st, true);
+    if (rc != WPF_OK && rc != WPF_ERR_NOCONV) return rc;
+    WPF_HIP(hipStreamSynchronize(c->stream));
     return rc;
 }
 

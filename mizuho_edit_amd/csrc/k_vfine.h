@@ -43,7 +43,7 @@ inline VZ make_vz(const Grid& g, int zc)
     z.nzt = (g.nz + zc - 1) / zc;
     return z;
 }
-inline int vz_blocks(const VZ& z) { return z.nchunk * z.nzt; }
+__host__ __device__ inline int vz_blocks(const VZ& z) { return z.nchunk * z.nzt; }
 
 template <typename T, int V>
 struct VecT {
@@ -81,10 +81,9 @@ __device__ __forceinline__ void vst(T* p, const T (&o)[V])
 // Threads past the end of the plane work on the last vector of the plane
 // (every lane must take part in the shuffles) and store nothing.
 template <int V>
-__device__ __forceinline__ bool vz_tile(const VZ& Z, int64_t& q, int& k0, int& k1)
+__device__ __forceinline__ bool vz_at(const VZ& Z, int b, int64_t& q, int& k0, int& k1)
 {
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
+    const int nb = vz_blocks(Z);
     const int lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
     const int ch = lb % Z.nchunk, kt = lb / Z.nchunk;
     q = ((int64_t)ch * 256 + threadIdx.x) * V;
@@ -143,19 +142,18 @@ __device__ __forceinline__ R xone(const R* __restrict__ x, const uint8_t* __rest
 
 // y = x + a (x - xo) + w invD (b - A x); see k_fsweep for the meaning of
 // a, xo, PROX, PROO and DOT.
+// Block bi of the grid's vz_blocks(Z) tiles; returns this thread's (b, y)
+// when DOT.
 template <typename R, typename C, int V, bool DOT, bool PROX, bool PROO>
-__global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, const C* __restrict__ ty,
-                                                const C* __restrict__ tz, const R* __restrict__ invD,
-                                                const uint8_t* __restrict__ mask, R a, R w,
-                                                const R* __restrict__ b, const R* __restrict__ x,
-                                                const R* xo, Pro<R> P, R* y,
-                                                double* __restrict__ partials,
-                                                unsigned* __restrict__ counter,
-                                                double* __restrict__ result)
+__device__ __forceinline__ double t_vsweep(const VZ& Z, int bi, const C* __restrict__ tx,
+                                           const C* __restrict__ ty, const C* __restrict__ tz,
+                                           const R* __restrict__ invD, const uint8_t* __restrict__ mask,
+                                           R a, R w, const R* __restrict__ b, const R* __restrict__ x,
+                                           const R* xo, const Pro<R>& P, R* y)
 {
     int64_t q;
     int k0, k1;
-    const bool act = vz_tile<V>(Z, q, k0, k1);
+    const bool act = vz_at<V>(Z, bi, q, k0, k1);
     const int lane = threadIdx.x & 63;
     const int i = (int)(q % Z.nx), j = (int)(q / Z.nx);
     const int64_t nx = Z.nx, pl = Z.plane;
@@ -216,25 +214,37 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
             tzm[e] = tzv[e];
         }
     }
-    if (!act) acc[0] = 0.0;
+    return act ? acc[0] : 0.0;
+}
+
+template <typename R, typename C, int V, bool DOT, bool PROX, bool PROO>
+__global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, const C* __restrict__ ty,
+                                                const C* __restrict__ tz, const R* __restrict__ invD,
+                                                const uint8_t* __restrict__ mask, R a, R w,
+                                                const R* __restrict__ b, const R* __restrict__ x,
+                                                const R* xo, Pro<R> P, R* y,
+                                                double* __restrict__ partials,
+                                                unsigned* __restrict__ counter,
+                                                double* __restrict__ result)
+{
+    double acc[1] = {t_vsweep<R, C, V, DOT, PROX, PROO>(Z, blockIdx.x, tx, ty, tz, invD, mask, a, w, b, x,
+                                                        xo, P, y)};
     if (DOT) block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
 }
 
 // p_new = z (+ beta p_old), q = A p_new on free cells, reduces (p_new, q).
 template <typename R, typename C, int V>
-__global__ void __launch_bounds__(256) k_vdir_mv(VZ Z, const C* __restrict__ tx, const C* __restrict__ ty,
-                                                 const C* __restrict__ tz, const uint8_t* __restrict__ mask,
-                                                 const R* __restrict__ z, const R* __restrict__ pold,
-                                                 R* __restrict__ pnew, R* __restrict__ qv,
-                                                 const double* __restrict__ res, int num, int den,
-                                                 int first, double* __restrict__ partials,
-                                                 unsigned* __restrict__ counter,
-                                                 double* __restrict__ result)
+__device__ __forceinline__ double t_vdir_mv(const VZ& Z, int bi, const C* __restrict__ tx,
+                                            const C* __restrict__ ty, const C* __restrict__ tz,
+                                            const uint8_t* __restrict__ mask, const R* __restrict__ z,
+                                            const R* __restrict__ pold, R* __restrict__ pnew,
+                                            R* __restrict__ qv, const double* __restrict__ res, int num,
+                                            int den, int first)
 {
     const R beta = first ? (R)0 : (R)(res[num] / res[den]);
     int64_t q;
     int k0, k1;
-    const bool act = vz_tile<V>(Z, q, k0, k1);
+    const bool act = vz_at<V>(Z, bi, q, k0, k1);
     const int lane = threadIdx.x & 63;
     const int64_t nx = Z.nx, pl = Z.plane;
     auto prow = [&](int64_t m, R(&o)[V]) {
@@ -295,7 +305,21 @@ __global__ void __launch_bounds__(256) k_vdir_mv(VZ Z, const C* __restrict__ tx,
             tzm[e] = tzv[e];
         }
     }
-    if (!act) acc[0] = 0.0;
+    return act ? acc[0] : 0.0;
+}
+
+template <typename R, typename C, int V>
+__global__ void __launch_bounds__(256) k_vdir_mv(VZ Z, const C* __restrict__ tx, const C* __restrict__ ty,
+                                                 const C* __restrict__ tz, const uint8_t* __restrict__ mask,
+                                                 const R* __restrict__ z, const R* __restrict__ pold,
+                                                 R* __restrict__ pnew, R* __restrict__ qv,
+                                                 const double* __restrict__ res, int num, int den,
+                                                 int first, double* __restrict__ partials,
+                                                 unsigned* __restrict__ counter,
+                                                 double* __restrict__ result)
+{
+    double acc[1] = {t_vdir_mv<R, C, V>(Z, blockIdx.x, tx, ty, tz, mask, z, pold, pnew, qv, res, num, den,
+                                        first)};
     block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
 }
 

@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/wpf.h"
+
 namespace wpf {
 
 template <typename T>
@@ -26,6 +28,29 @@ struct PhaseConstsT {
     T kwh, inv_mu_unused;
 };
 using PhaseConsts = PhaseConstsT<double>;
+
+// constants of calcPhaseDeriv / calcConcentDeriv from the parameters
+// (wPFAW3D.f90:1221-1230, 1298, 1566-1570)
+inline PhaseConsts phase_consts(const wpf_params& p)
+{
+    PhaseConsts K;
+    K.dx = p.dx;
+    K.two_dx = 2.0 * p.dx;
+    K.dx2 = p.dx * p.dx;
+    K.four_dx2 = 4.0 * (p.dx * p.dx);
+    K.w0 = 2.0 * p.dx;
+    K.maxdphi = 2.0 / (2.0 * p.dx);
+    K.eps = p.epsilon;
+    K.lambda = p.lambda;
+    K.diffusion = p.diffusion;
+    K.ral = p.ral;
+    K.rat = p.rat;
+    K.tau_num = 5.0 / 3.0 * p.alpha * p.lambda * (p.epsilon * p.epsilon);
+    K.alpha = p.alpha;
+    K.kwh = p.permeab_hole;
+    K.inv_mu_unused = 0.0;
+    return K;
+}
 
 // the same constants rounded to another precision
 template <typename T>

@@ -202,9 +202,21 @@ int launch_import(wpf_ctx* c, int field, const double* host);
 int launch_diag(wpf_ctx* c, wpf_diag* d);
 int check_breakthrough(wpf_ctx* c, int itime);
 
+// ensemble forms of the cell kernels (k_fields.hip); see ensemble.h
+struct EnsMember;
+struct EnsTile;
+int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid);
+int ens_step_fields(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid);
+
 // solver (k_solver.hip)
 int solve_pressure(wpf_ctx* c, const wpf_solver_opts* o, wpf_solve_stats* st);
 void free_solver(wpf_ctx* c);
+// option defaults of the solvers; FP32 workspace and hierarchy of a context
+void solver_defaults(const wpf_solver_opts* in, wpf_solver_opts* out);
+int solver_workspace(wpf_ctx* c, const wpf_solver_opts* o);
+// the mixed solver from the current pressure: no initial guess, no
+// extrapolation bookkeeping (used to finish a solve a batch left open)
+int solve_pressure_continue(wpf_ctx* c, const wpf_solver_opts* o, wpf_solve_stats* st);
 
 // learned part (k_learned.hip)
 void free_learned(wpf_ctx* c);
@@ -306,8 +318,8 @@ __device__ void block_reduce_store(double (&v)[NV], double* partials, unsigned* 
 }
 
 // Sum nv rows of nblocks partials into result[0..nv); one block of 1024.
-static __global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks,
-                                                   int nv, double* __restrict__ result)
+__device__ __forceinline__ void t_finalize(const double* __restrict__ partials, int nblocks, int nv,
+                                           double* __restrict__ result)
 {
     __shared__ double sh[16];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -324,6 +336,11 @@ static __global__ void __launch_bounds__(1024) k_finalize(const double* __restri
         }
         __syncthreads();
     }
+}
+static __global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks,
+                                                   int nv, double* __restrict__ result)
+{
+    t_finalize(partials, nblocks, nv, result);
 }
 
 }  // namespace wpf

@@ -387,12 +387,38 @@ def workload(name: str, seed: int = 1, grid=None, **overrides) -> Case:
              "mixed-256": (256, 256, 256), "learned-192": (192, 192, 192)}
     nx, ny, nz = grid if grid is not None else sizes[name]
     corr = overrides.pop("corr_len", 4.0)
+    mean = overrides.pop("phi_mean", 0.2)
     if name == "mixed-256":
-        kr = rock_permeability((nx, ny, nz), corr, seed, 0.2, 0.05, (0.02, 0.45), variance=2.0)
+        kr = rock_permeability((nx, ny, nz), corr, seed, mean, 0.05, (0.02, 0.45), variance=2.0)
     else:
-        kr = rock_permeability((nx, ny, nz), corr, seed)
+        kr = rock_permeability((nx, ny, nz), corr, seed, mean)
     c_in = overrides.pop("c_in", -1.0)
     p_in = overrides.pop("p_in", 1.0e5)
     p = physics_params(nx, ny, nz, **overrides)
     wells = inlet_outlet_wells(p, p_in=p_in, c_in=c_in)
     return build_case(p, wells, kr)
+
+
+# the ensemble workload: (count, grid) groups of uneven realisations
+ENSEMBLE_GROUPS = [(16, (64, 64, 64)), (16, (96, 64, 64)), (8, (128, 96, 96)), (8, (160, 128, 128))]
+
+
+def ensemble_cases(seed0: int = 1000, shrink: int = 1, groups=None) -> List[Case]:
+    """Seeded realisations of the plain workload's physics, one per member of
+    the ensemble groups (48 grids, 40,894,464 cells at full size), each with
+    its own correlation length in [2, 8] cells and mean porosity in
+    [0.12, 0.25] (a low-discrepancy spread over the members).  shrink divides
+    every grid dimension (tests)."""
+    groups = groups or ENSEMBLE_GROUPS
+    total = sum(cnt for cnt, _ in groups)
+    cases, m = [], 0
+    for cnt, (nx, ny, nz) in groups:
+        for _ in range(cnt):
+            u = (m * 0.6180339887498949) % 1.0
+            v = (m * 0.7548776662466927) % 1.0
+            grid = (nx // shrink, ny // shrink, nz // shrink)
+            cases.append(workload("plain-small", seed=seed0 + m, grid=grid, corr_len=2.0 + 6.0 * u,
+                                  phi_mean=0.12 + 0.13 * v))
+            m += 1
+    assert m == total
+    return cases

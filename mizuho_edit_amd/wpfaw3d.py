@@ -208,6 +208,10 @@ class WPFAW3D:
               "attach_net")
         self._net = net
 
+    def set_tile_depth(self, planes: int):
+        """Fix the planes per tile of the vector solver kernels (else tuned)."""
+        check(_lib.lib().wpf_set_tile_depth(self._ctx, planes), "set_tile_depth")
+
     # -- kernel timing ------------------------------------------------------
     def profile_start(self, kernel: str):
         check(_lib.lib().wpf_profile_start(self._ctx, _lib.KERNEL[kernel]), "profile_start")

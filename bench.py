@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="fast path without the network")
     ap.add_argument("--jacobi", action="store_true",
                     help="fast path with the plain path's smoother and FP32 faces")
+    ap.add_argument("--f64-stencils", action="store_true",
+                    help="fast path with the FP64 phase/concentration stencils")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 1 GiB copy kernel and exit")
     ap.add_argument("--ensemble", action="store_true",
@@ -259,7 +261,8 @@ def main():
     g64.close()
 
     from mizuho_edit_amd import learned
-    from mizuho_edit_amd._lib import FLAG_CHEBYSHEV, FLAG_EXTRAPOLATE, FLAG_LEARNED_WARM, FLAG_OP16
+    from mizuho_edit_amd._lib import (FLAG_CHEBYSHEV, FLAG_EXTRAPOLATE, FLAG_F32_STENCIL,
+                                      FLAG_LEARNED_WARM, FLAG_OP16)
     weights = a.weights or learned.WEIGHTS
     net = None
     if not a.no_learned:
@@ -274,6 +277,8 @@ def main():
     flags = FLAG_EXTRAPOLATE | (FLAG_LEARNED_WARM if net else 0)
     if not a.jacobi:
         flags |= FLAG_CHEBYSHEV | FLAG_OP16
+    if not a.f64_stencils:
+        flags |= FLAG_F32_STENCIL
     fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, flags=flags)
     gf, wallf, itsf, state0, prof = run_path(case, fast_opts, a.warmup, a.steps, a.kernel, net)
     pf = gf.download("pressure")
@@ -309,7 +314,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 stencils + f32 solve in f64 refinement",
+        "dtype": ("f64" if a.f64_stencils else "f32") + " stencils (f64 fields and update) + "
+                 "f32 solve in f64 refinement",
         "data": "synthetic: seeded Gaussian random field rock (corr 4 cells), Kozeny-Carman k",
         "config": {"workload": f"wormholing {n}^3, steps {a.warmup + 1}..{a.warmup + a.steps} "
                                f"of the 500-step run", "grid": [n, n, n],

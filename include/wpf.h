@@ -262,7 +262,7 @@ int wpf_capture_get(wpf_ctx* ctx, float* fields, float* target);
 #define WPF_KERNEL_MATVEC      3  /* fine-level matvec + dot                */
 #define WPF_KERNEL_TRUE_RESID  4  /* FP64 true residual                     */
 #define WPF_KERNEL_RESTRICT    5  /* fine-level residual + restriction      */
-#define WPF_KERNEL_PROLONG     6  /* fine-level prolongation                */
+#define WPF_KERNEL_PROLONG     6  /* (fused into the smoothing: no launches) */
 #define WPF_KERNEL_CG_UPDATE   7  /* fine-level x/r update                  */
 int wpf_profile_start(wpf_ctx* ctx, int32_t kernel);
 int wpf_profile_stop(wpf_ctx* ctx, double* total_ms, int32_t* count);

@@ -377,6 +377,26 @@ __global__ void __launch_bounds__(256) ke_lsmooth0(ENS_ARGS, int l, int rx, floa
     ENS_END
 }
 
+__global__ void __launch_bounds__(256) ke_lsmooth01(ENS_ARGS, int l, int ry, float w1, float a, float w)
+{
+    ENS_TILES(G_INNER)
+    const EnsLevel& V = M.lv[l];
+    t_smooth01<float>(ens_lv32(M, l), w1, a, w, V.a[EL_B], V.a[ry], T.b, V.nb);
+    ENS_END
+}
+
+template <bool PROX, bool PROO>
+__global__ void __launch_bounds__(256) ke_lsmooth_pro(ENS_ARGS, int l, int rx, int rxo, int ry, float a, float w,
+                                                      float cs)
+{
+    ENS_TILES(G_INNER)
+    const EnsLevel& V = M.lv[l];
+    const EnsLevel& C = M.lv[l + 1];
+    t_smooth_pro<float, PROX, PROO>(ens_lv32(M, l), C.nx, C.ny, cs, C.a[EL_X], a, w, V.a[EL_B], V.a[rx],
+                                    rxo >= 0 ? V.a[rxo] : nullptr, V.a[ry], T.b, V.nb);
+    ENS_END
+}
+
 __global__ void __launch_bounds__(256) ke_lsmooth(ENS_ARGS, int l, int rx, int rxo, int ry, float a, float w)
 {
     ENS_TILES(G_INNER)
@@ -393,15 +413,6 @@ __global__ void __launch_bounds__(256) ke_lresid_restrict(ENS_ARGS, int l, int r
     const EnsLevel& V = M.lv[l];
     const EnsLevel& C = M.lv[l + 1];
     t_resid_restrict<float, false>(ens_lv32(M, l), C.nx, C.ny, C.nz, V.a[EL_B], V.a[rx], C.a[EL_B], T.b, C.nb);
-    ENS_END
-}
-
-__global__ void __launch_bounds__(256) ke_lprolong(ENS_ARGS, int l, int rx, float cs)
-{
-    ENS_TILES(G_INNER)
-    const EnsLevel& V = M.lv[l];
-    const EnsLevel& C = M.lv[l + 1];
-    t_prolong<float, false>(ens_lv32(M, l), C.nx, C.ny, cs, C.a[EL_X], V.a[rx], T.b, V.nb);
     ENS_END
 }
 
@@ -556,18 +567,32 @@ int sched_coarse(const Sched& S, int l)
     }
     int cur = EL_R, oth = EL_X;
     double a, w;
-    ELAUNCH(ke_lsmooth0, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, cur, (float)o.w1);
-    for (int s = 1; s < o.sweeps; s++) {
+    if (o.sweeps >= 2) {
+        step_coef(o, 1, &a, &w);
+        ELAUNCH(ke_lsmooth01, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, oth, (float)o.w1, (float)a,
+                (float)w);
+        std::swap(cur, oth);
+    } else {
+        ELAUNCH(ke_lsmooth0, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, cur, (float)o.w1);
+    }
+    for (int s = 2; s < o.sweeps; s++) {
         step_coef(o, s, &a, &w);
         ELAUNCH(ke_lsmooth, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, cur, -1, oth, (float)a, (float)w);
         std::swap(cur, oth);
     }
     ELAUNCH(ke_lresid_restrict, e->grid, 256, 0, e->d_mem, e->tl[l + 1], e->nl[l + 1], l, cur);
     ECHK(sched_coarse(S, l + 1));
-    ELAUNCH(ke_lprolong, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, cur, (float)o.cscale);
+    const float cs = (float)o.cscale;
     for (int s = 0; s < o.sweeps; s++) {
         step_coef(o, s, &a, &w);
-        ELAUNCH(ke_lsmooth, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, cur, oth, oth, (float)a, (float)w);
+        if (s == 0)
+            ELAUNCH((ke_lsmooth_pro<true, false>), e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, cur, oth, oth,
+                    (float)a, (float)w, cs);
+        else if (s == 1 && a != 0.0)
+            ELAUNCH((ke_lsmooth_pro<false, true>), e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, cur, oth, oth,
+                    (float)a, (float)w, cs);
+        else
+            ELAUNCH(ke_lsmooth, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, cur, oth, oth, (float)a, (float)w);
         std::swap(cur, oth);
     }
     return WPF_OK;

@@ -284,26 +284,76 @@ __global__ void __launch_bounds__(kThreads) k_resid_restrict(LV<R> L, int cnx, i
     t_resid_restrict<R, FINE>(L, cnx, cny, cnz, b, x, bc, (int)blockIdx.x, (int)gridDim.x);
 }
 
-// x += scale * P e  (piecewise constant; fixed fine cells untouched)
-template <typename R, bool FINE>
-__device__ __forceinline__ void t_prolong(LV<R> L, int cnx, int cny, R scale,
-                                                      const R* __restrict__ e, R* __restrict__ x, int b_, int nb_)
+// Coarse levels: the first two pre-smoothing steps from x = 0 in one pass.
+// x0 = w1 invD b is formed on the fly for the cell and its neighbours, then
+// y = x0 + a x0 + w invD (b - A x0) (step 2 with x_{-1} = 0).  The same
+// arithmetic as k_smooth0 followed by k_smooth; pads of b and invD are 0.
+template <typename R>
+__device__ __forceinline__ void t_smooth01(LV<R> L, R w1, R a, R w, const R* __restrict__ b,
+                                           R* __restrict__ y, int b_, int nb_)
 {
     const Chunk ch = my_chunk_at(L.n, b_, nb_);
+    auto X0 = [&](int64_t m) { return w1 * L.invD[m] * b[m]; };
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        const R txp = L.tx[c], txm = L.tx[c - 1], typ = L.ty[c], tym = L.ty[c - L.nx];
+        const R tzp = L.tz[c], tzm = L.tz[c - L.plane];
+        const R s = txp * X0(c + 1) + txm * X0(c - 1) + typ * X0(c + L.nx) + tym * X0(c - L.nx) +
+                    tzp * X0(c + L.plane) + tzm * X0(c - L.plane);
+        const R xv = X0(c);
+        R yv = xv + w * L.invD[c] * (b[c] - (L.D[c] * xv - s));
+        if (a != (R)0) yv += a * (xv - (R)0);
+        y[c] = yv;
+    }
+}
+template <typename R>
+__global__ void __launch_bounds__(kThreads) k_smooth01(LV<R> L, R w1, R a, R w, const R* __restrict__ b,
+                                                       R* __restrict__ y)
+{
+    t_smooth01<R>(L, w1, a, w, b, y, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// Coarse levels: a post-smoothing step with the coarse correction of the
+// next level added on the fly, x(m) + cs * xc[parent(m)] (every coarse cell
+// is free); PROX for x, PROO for xo (the Chebyshev step's x0).  The same
+// arithmetic as adding the prolonged correction to x and then a k_smooth
+// step.
+template <typename R, bool PROX, bool PROO>
+__device__ __forceinline__ void t_smooth_pro(LV<R> L, int cnx, int cny, R cs, const R* __restrict__ xc, R a,
+                                             R w, const R* __restrict__ b, const R* __restrict__ x,
+                                             const R* xo, R* y, int b_, int nb_)
+{
+    const Chunk ch = my_chunk_at(L.n, b_, nb_);
+    const int64_t cpl = (int64_t)cnx * cny;
+    auto par = [&](int i, int j, int k) { return (i >> 1) + (int64_t)cnx * (j >> 1) + cpl * (k >> 1); };
     for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
         const int i = (int)(c % L.nx);
         const int j = (int)((c / L.nx) % L.ny);
         const int k = (int)(c / L.plane);
-        const int64_t C = (i >> 1) + (int64_t)cnx * ((j >> 1) + (int64_t)cny * (k >> 1));
-        if (is_fixed(L, c, FINE)) continue;
-        x[c] = x[c] + scale * e[C];
+        auto X = [&](int64_t m, int ii, int jj, int kk) {
+            return PROX ? x[m] + cs * xc[par(ii, jj, kk)] : x[m];
+        };
+        const R txp = L.tx[c], txm = L.tx[c - 1], typ = L.ty[c], tym = L.ty[c - L.nx];
+        const R tzp = L.tz[c], tzm = L.tz[c - L.plane];
+        const R s = txp * X(c + 1, i + 1, j, k) + txm * X(c - 1, i - 1, j, k) + typ * X(c + L.nx, i, j + 1, k) +
+                    tym * X(c - L.nx, i, j - 1, k) + tzp * X(c + L.plane, i, j, k + 1) +
+                    tzm * X(c - L.plane, i, j, k - 1);
+        const R xv = X(c, i, j, k);
+        R yv = xv + w * L.invD[c] * (b[c] - (L.D[c] * xv - s));
+        if (a != (R)0) {
+            R ov = xo ? xo[c] : (R)0;
+            if (PROO) ov = ov + cs * xc[par(i, j, k)];
+            yv += a * (xv - ov);
+        }
+        y[c] = yv;
     }
 }
-template <typename R, bool FINE>
-__global__ void __launch_bounds__(kThreads) k_prolong(LV<R> L, int cnx, int cny, R scale,
-                                                      const R* __restrict__ e, R* __restrict__ x)
+template <typename R, bool PROX, bool PROO>
+__global__ void __launch_bounds__(kThreads) k_smooth_pro(LV<R> L, int cnx, int cny, R cs,
+                                                         const R* __restrict__ xc, R a, R w,
+                                                         const R* __restrict__ b, const R* __restrict__ x,
+                                                         const R* xo, R* y)
 {
-    t_prolong<R, FINE>(L, cnx, cny, scale, e, x, (int)blockIdx.x, (int)gridDim.x);
+    t_smooth_pro<R, PROX, PROO>(L, cnx, cny, cs, xc, a, w, b, x, xo, y, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Galerkin aggregation A_c = P^T A P on 2x2x2 aggregates.

@@ -179,6 +179,9 @@ static int coarsest(wpf_ctx* c, MGLevel<R>& L, const R* b, R* x, R* tmp, const M
 }
 
 // V-cycle on a coarse level l >= 1: x = M_l b, using the level's own arrays.
+// The first two pre-smoothing steps run as one pass (k_smooth01) and the
+// prolongation of the next level's correction is folded into the first
+// post-smoothing step (k_smooth_pro).
 template <typename R>
 static int vcycle_coarse(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, const MGOpts& o)
 {
@@ -186,14 +189,19 @@ static int vcycle_coarse(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, cons
     if (l + 1 == lv.size()) return coarsest(c, L, (const R*)L.b.p, L.x.p, L.r.p, o);
     LV<R> V = view(L, nullptr, false);
     const R* b = L.b.p;
-    // ping-pong r <-> x; 2*sweeps-1 flips after the first sweep land in x
+    // ping-pong r <-> x; the result lands in x
     R* cur = L.r.p;
     R* oth = L.x.p;
     double a, w;
-    LAUNCH1D(k_smooth0<R>, L.g.n, L.g.n, (R)o.w1, b, L.invD.p, cur);
-    for (int s = 1; s < o.sweeps; s++) {
+    if (o.sweeps >= 2) {
+        step_coef(o, 1, &a, &w);
+        LAUNCH1D(k_smooth01<R>, L.g.n, V, (R)o.w1, (R)a, (R)w, b, oth);
+        std::swap(cur, oth);
+    } else {
+        LAUNCH1D(k_smooth0<R>, L.g.n, L.g.n, (R)o.w1, b, L.invD.p, cur);
+    }
+    for (int s = 2; s < o.sweeps; s++) {
         step_coef(o, s, &a, &w);
-        // the iterate before step 1 is x0 = 0
         LAUNCH1D((k_smooth<R, false>), L.g.n, V, (R)a, (R)w, b, (const R*)cur, (const R*)nullptr, oth);
         std::swap(cur, oth);
     }
@@ -201,11 +209,19 @@ static int vcycle_coarse(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, cons
     LAUNCH1D((k_resid_restrict<R, false>), C.g.n, V, C.g.nx, C.g.ny, C.g.nz, b, (const R*)cur, C.b.p);
     int rc = vcycle_coarse(c, lv, l + 1, o);
     if (rc) return rc;
-    LAUNCH1D((k_prolong<R, false>), L.g.n, V, C.g.nx, C.g.ny, (R)o.cscale, (const R*)C.x.p, cur);
+    const R cs = (R)o.cscale;
     for (int s = 0; s < o.sweeps; s++) {
         step_coef(o, s, &a, &w);
-        // step 1 reads its x0 from oth and overwrites it cell by cell
-        LAUNCH1D((k_smooth<R, false>), L.g.n, V, (R)a, (R)w, b, (const R*)cur, (const R*)oth, oth);
+        // step s reads x from cur; step 1 reads its x0 from oth and
+        // overwrites it cell by cell
+        if (s == 0)
+            LAUNCH1D((k_smooth_pro<R, true, false>), L.g.n, V, C.g.nx, C.g.ny, cs, (const R*)C.x.p, (R)a, (R)w,
+                     b, (const R*)cur, (const R*)oth, oth);
+        else if (s == 1 && a != 0.0)
+            LAUNCH1D((k_smooth_pro<R, false, true>), L.g.n, V, C.g.nx, C.g.ny, cs, (const R*)C.x.p, (R)a,
+                     (R)w, b, (const R*)cur, (const R*)oth, oth);
+        else
+            LAUNCH1D((k_smooth<R, false>), L.g.n, V, (R)a, (R)w, b, (const R*)cur, (const R*)oth, oth);
         std::swap(cur, oth);
     }
     return WPF_OK;
@@ -627,7 +643,7 @@ static int start_stage(wpf_ctx* c, const wpf_solver_opts& o, int* learned, bool 
 // with and without it; afterwards it is used while its mean count stays 3%
 // below the other's, and every kGatePeriod solves the two are probed again
 // (kGateProbe solves, fresh means), so a change of regime is followed.
-static constexpr int kGateProbe = 8, kGatePeriod = 40;
+static constexpr int kGateProbe = 4, kGatePeriod = 100;
 
 static bool gate_use(wpf_ctx* c)
 {

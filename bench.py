@@ -224,6 +224,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="fast path without the network")
     ap.add_argument("--jacobi", action="store_true",
                     help="fast path with the plain path's smoother and FP32 faces")
+    ap.add_argument("--no-project", action="store_true",
+                    help="fast path without the projected initial guess")
+    ap.add_argument("--fast-levels", type=int, default=6,
+                    help="multigrid levels of the fast path (the plain path keeps 5)")
     ap.add_argument("--f64-stencils", action="store_true",
                     help="fast path with the FP64 phase/concentration stencils")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -262,7 +266,7 @@ def main():
 
     from mizuho_edit_amd import learned
     from mizuho_edit_amd._lib import (FLAG_CHEBYSHEV, FLAG_EXTRAPOLATE, FLAG_F32_STENCIL,
-                                      FLAG_LEARNED_WARM, FLAG_OP16)
+                                      FLAG_LEARNED_WARM, FLAG_OP16, FLAG_PROJECT)
     weights = a.weights or learned.WEIGHTS
     net = None
     if not a.no_learned:
@@ -277,9 +281,11 @@ def main():
     flags = FLAG_EXTRAPOLATE | (FLAG_LEARNED_WARM if net else 0)
     if not a.jacobi:
         flags |= FLAG_CHEBYSHEV | FLAG_OP16
+    if not a.no_project:
+        flags |= FLAG_PROJECT
     if not a.f64_stencils:
         flags |= FLAG_F32_STENCIL
-    fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, flags=flags)
+    fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, mg_levels=a.fast_levels, flags=flags)
     gf, wallf, itsf, state0, prof = run_path(case, fast_opts, a.warmup, a.steps, a.kernel, net)
     pf = gf.download("pressure")
     accepted = gf.learned_accepted
@@ -319,9 +325,10 @@ def main():
         "data": "synthetic: seeded Gaussian random field rock (corr 4 cells), Kozeny-Carman k",
         "config": {"workload": f"wormholing {n}^3, steps {a.warmup + 1}..{a.warmup + a.steps} "
                                f"of the 500-step run", "grid": [n, n, n],
-                   "solver": ("learned first stage + " if net else "") +
-                             ("FP32 MG-PCG (5 levels, 2 Jacobi sweeps)" if a.jacobi else
-                              "FP32 MG-PCG (5 levels, degree-2 Chebyshev, FP16-stored fine faces)") +
+                   "solver": ("" if a.no_project else "projected initial guess (last 6 solution changes) + ") +
+                             ("learned first stage + " if net else "") +
+                             (f"FP32 MG-PCG ({a.fast_levels} levels, 2 Jacobi sweeps)" if a.jacobi else
+                              f"FP32 MG-PCG ({a.fast_levels} levels, degree-2 Chebyshev, FP16-stored fine faces)") +
                              " in FP64 refinement, rtol 1e-10",
                    "plain_solver": "FP64 MG-PCG (5 levels, 2 Jacobi sweeps), rtol 1e-10",
                    "parallelism": "1 GPU"},

@@ -36,6 +36,7 @@ FLAG_CAPTURE = 8
 FLAG_CHEBYSHEV = 16
 FLAG_OP16 = 32
 FLAG_F32_STENCIL = 64
+FLAG_PROJECT = 128
 
 
 class WpfSolverOpts(ctypes.Structure):

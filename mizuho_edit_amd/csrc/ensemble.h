@@ -26,7 +26,7 @@ namespace wpf {
 constexpr int kEnsMaxLevels = 8;
 
 // FP32 vectors of a member's inner solve
-enum EnsVec { EV_R = 0, EV_Z, EV_T, EV_PA, EV_PB, EV_Q, EV_E, EV_COUNT };
+enum EnsVec { EV_R = 0, EV_Z, EV_T, EV_PA, EV_PB, EV_Q, EV_E, EV_R2, EV_COUNT };
 // arrays of a coarse level
 enum EnsLvArr { EL_X = 0, EL_B, EL_R };
 

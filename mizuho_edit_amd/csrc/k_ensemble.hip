@@ -284,13 +284,13 @@ __global__ void __launch_bounds__(256) ke_true_resid(ENS_ARGS)
     ENS_END
 }
 
-__global__ void __launch_bounds__(256) ke_zero_smooth0(ENS_ARGS, float w1)
+__global__ void __launch_bounds__(256) ke_zero_smooth0(ENS_ARGS, int rr, float w1)
 {
     ENS_TILES(G_INNER)
     const Chunk ch = my_chunk_at(M.n, T.b, M.nb1);
     float* e = M.v[EV_E];
     float* z = M.v[EV_Z];
-    const float* r = M.v[EV_R];
+    const float* r = M.v[rr];
     for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
         e[c] = 0.0f;
         z[c] = w1 * M.invD32[c] * r[c];
@@ -298,13 +298,28 @@ __global__ void __launch_bounds__(256) ke_zero_smooth0(ENS_ARGS, float w1)
     ENS_END
 }
 
-__global__ void __launch_bounds__(256) ke_update(ENS_ARGS, int rp, int num, float w1)
+__global__ void __launch_bounds__(256) ke_update(ENS_ARGS, int rp, int rr, int num, float w1)
 {
     ENS_TILES(G_INNER)
     double acc[2];
-    t_update<float, float>(M.n, M.result, num, S_PQ, w1, M.v[EV_E], M.v[rp], M.v[EV_R], M.v[EV_Q], M.invD32,
+    t_update<float, float>(M.n, M.result, num, S_PQ, w1, M.v[EV_E], M.v[rp], M.v[rr], M.v[EV_Q], M.invD32,
                            M.v[EV_Z], acc, T.b, M.nb1);
     block_reduce_store<2>(acc, M.partials, nullptr, nullptr, M.nb1, T.b);
+    ENS_END
+}
+
+// the fused update + pre-smoothing step 1 (k_vupsweep); y into EV_T
+template <typename C>
+__global__ void __launch_bounds__(256) ke_upsweep(ENS_ARGS, int rp, int rr, int rr2, int num, float w1, float a,
+                                                  float w)
+{
+    ENS_TILES(G_INNER)
+    const C *tx, *ty, *tz;
+    ens_faces<C>(M, tx, ty, tz);
+    double acc[2];
+    t_vupsweep<float, float, C, 4>(M.Z, T.b, M.result, num, S_PQ, w1, a, w, M.v[EV_E], M.v[rp], M.v[rr],
+                                   M.v[EV_Q], M.v[rr2], tx, ty, tz, M.invD32, M.v[EV_T], acc);
+    block_reduce_store<2>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);
     ENS_END
 }
 
@@ -317,7 +332,8 @@ __global__ void __launch_bounds__(256) ke_add(ENS_ARGS)
 
 // ---- fine level, vector and restriction tiles --------------------------------
 template <typename C, bool DOT, bool PROX, bool PROO>
-__global__ void __launch_bounds__(256) ke_vsweep(ENS_ARGS, int rx, int rxo, int ry, float a, float w, float cs)
+__global__ void __launch_bounds__(256) ke_vsweep(ENS_ARGS, int rb, int rx, int rxo, int ry, float a, float w,
+                                                 float cs)
 {
     ENS_TILES(G_INNER)
     const C *tx, *ty, *tz;
@@ -325,7 +341,7 @@ __global__ void __launch_bounds__(256) ke_vsweep(ENS_ARGS, int rx, int rxo, int 
     const Pro<float> P{M.nlev > 1 ? M.lv[1].a[EL_X] : nullptr, M.nlev > 1 ? M.lv[1].nx : 0,
                        M.nlev > 1 ? M.lv[1].plane : 0, cs};
     double acc[1] = {t_vsweep<float, C, 4, DOT, PROX, PROO>(M.Z, T.b, tx, ty, tz, M.invD32, M.mask, a, w,
-                                                           M.v[EV_R], M.v[rx], rxo >= 0 ? M.v[rxo] : nullptr,
+                                                           M.v[rb], M.v[rx], rxo >= 0 ? M.v[rxo] : nullptr,
                                                            P, M.v[ry])};
     if (DOT) block_reduce_store<1>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);
     ENS_END
@@ -345,12 +361,12 @@ __global__ void __launch_bounds__(256) ke_dir_mv(ENS_ARGS, int rz, int rpold, in
 }
 
 template <typename C>
-__global__ void __launch_bounds__(256) ke_fresid_restrict(ENS_ARGS, int rx)
+__global__ void __launch_bounds__(256) ke_fresid_restrict(ENS_ARGS, int rb, int rx)
 {
     ENS_TILES(G_INNER)
     const C *tx, *ty, *tz;
     ens_faces<C>(M, tx, ty, tz);
-    t_fresid_restrict<float, C>(M.G, T.b, tx, ty, tz, M.mask, M.v[EV_R], M.v[rx], M.lv[1].nx, M.lv[1].ny,
+    t_fresid_restrict<float, C>(M.G, T.b, tx, ty, tz, M.mask, M.v[rb], M.v[rx], M.lv[1].nx, M.lv[1].ny,
                                 M.lv[1].a[EL_B]);
     ENS_END
 }
@@ -496,7 +512,7 @@ int build_tables(wpf_ensemble* e)
         X.tx32 = F.tx.p; X.ty32 = F.ty.p; X.tz32 = F.tz.p; X.invD32 = F.invD.p;
         X.htx = c->h_tx.p; X.hty = c->h_ty.p; X.htz = c->h_tz.p;
         X.v[EV_R] = c->r32.p; X.v[EV_Z] = c->z32.p; X.v[EV_T] = c->t32.p; X.v[EV_PA] = c->p32.p;
-        X.v[EV_PB] = c->p32b.p; X.v[EV_Q] = c->q32.p; X.v[EV_E] = c->x32.p;
+        X.v[EV_PB] = c->p32b.p; X.v[EV_Q] = c->q32.p; X.v[EV_E] = c->x32.p; X.v[EV_R2] = c->r32b.p;
         X.nlev = e->nlev;
         for (int l = 1; l < e->nlev; l++) {
             MGLevel<float>& L = c->mg32[l];
@@ -601,22 +617,22 @@ int sched_coarse(const Sched& S, int l)
 // fine V-cycle (vcycle_fine): z = M r with z0 already in EV_Z; the result
 // role through *zres, (r, z) into dot_slot
 template <typename C>
-int sched_vcycle(const Sched& S, int dot_slot, int* zres)
+int sched_vcycle(const Sched& S, int dot_slot, int* zres, int rb, bool pre_done)
 {
     wpf_ensemble* e = S.e;
     const MGOpts& o = S.mo;
-    int cur = EV_Z, oth = EV_T;
+    int cur = pre_done ? EV_T : EV_Z, oth = pre_done ? EV_Z : EV_T;
     double a, w;
     const float cs = (float)o.cscale;
-    for (int s = 1; s < o.sweeps; s++) {
+    for (int s = pre_done ? 2 : 1; s < o.sweeps; s++) {
         step_coef(o, s, &a, &w);
-        ELAUNCH((ke_vsweep<C, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, -1, oth,
+        ELAUNCH((ke_vsweep<C, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, -1, oth,
                 (float)a, (float)w, cs);
         std::swap(cur, oth);
     }
     const bool coarse = e->nlev > 1;
     if (coarse) {
-        ELAUNCH((ke_fresid_restrict<C>), e->grid, dim3(kFX, kFY), 0, e->d_mem, e->tz, e->nz, cur);
+        ELAUNCH((ke_fresid_restrict<C>), e->grid, dim3(kFX, kFY), 0, e->d_mem, e->tz, e->nz, rb, cur);
         ECHK(sched_coarse(S, 1));
     }
     for (int s = 0; s < o.sweeps; s++) {
@@ -625,23 +641,23 @@ int sched_vcycle(const Sched& S, int dot_slot, int* zres)
         const float fa = (float)a, fw = (float)w;
         if (s == 0 && last) {
             if (coarse)
-                ELAUNCH((ke_vsweep<C, true, true, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, oth, oth, fa, fw, cs);
+                ELAUNCH((ke_vsweep<C, true, true, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, oth, oth, fa, fw, cs);
             else
-                ELAUNCH((ke_vsweep<C, true, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, oth, oth, fa, fw, cs);
+                ELAUNCH((ke_vsweep<C, true, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, oth, oth, fa, fw, cs);
         } else if (s == 0) {
             if (coarse)
-                ELAUNCH((ke_vsweep<C, false, true, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, oth, oth, fa, fw, cs);
+                ELAUNCH((ke_vsweep<C, false, true, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, oth, oth, fa, fw, cs);
             else
-                ELAUNCH((ke_vsweep<C, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, oth, oth, fa, fw, cs);
+                ELAUNCH((ke_vsweep<C, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, oth, oth, fa, fw, cs);
         } else if (s == 1 && a != 0.0 && coarse) {
             if (last)
-                ELAUNCH((ke_vsweep<C, true, false, true>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, oth, oth, fa, fw, cs);
+                ELAUNCH((ke_vsweep<C, true, false, true>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, oth, oth, fa, fw, cs);
             else
-                ELAUNCH((ke_vsweep<C, false, false, true>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, oth, oth, fa, fw, cs);
+                ELAUNCH((ke_vsweep<C, false, false, true>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, oth, oth, fa, fw, cs);
         } else if (last) {
-            ELAUNCH((ke_vsweep<C, true, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, oth, oth, fa, fw, cs);
+            ELAUNCH((ke_vsweep<C, true, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, oth, oth, fa, fw, cs);
         } else {
-            ELAUNCH((ke_vsweep<C, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, cur, oth, oth, fa, fw, cs);
+            ELAUNCH((ke_vsweep<C, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, rb, cur, oth, oth, fa, fw, cs);
         }
         if (last) ELAUNCH(ke_finalize, S.nm, 1024, 0, e->d_mem, (int)F_VEC, 1, dot_slot, (int)G_INNER);
         std::swap(cur, oth);
@@ -677,20 +693,31 @@ int sched_solve(wpf_ensemble* e)
         // pcg_core
         int rz_cur = S_RZ0, rz_prev = S_RZ1;
         int zres = EV_Z;
-        ELAUNCH(ke_zero_smooth0, e->grid, 256, 0, e->d_mem, e->t1, e->n1, w1);
-        ECHK(sched_vcycle<C>(S, rz_cur, &zres));
+        int rr = EV_R, rr2 = EV_R2;  // current residual and the fused update's target
+        const bool fuse = S.mo.sweeps >= 2;
+        ELAUNCH(ke_zero_smooth0, e->grid, 256, 0, e->d_mem, e->t1, e->n1, rr, w1);
+        ECHK(sched_vcycle<C>(S, rz_cur, &zres, rr, false));
         int pc = EV_PA, pn = EV_PB;
         for (int it = 0; it < e->eo.inner_iters; it++) {
             ELAUNCH((ke_dir_mv<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, zres, pc, pn, rz_cur, rz_prev,
                     it == 0 ? 1 : 0);
             ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, 1, (int)S_PQ, (int)G_INNER);
             std::swap(pc, pn);
-            ELAUNCH(ke_update, e->grid, 256, 0, e->d_mem, e->t1, e->n1, pc, rz_cur, w1);
-            ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 2, (int)S_RN, (int)G_INNER);
+            if (fuse) {
+                double a1, w1s;
+                step_coef(S.mo, 1, &a1, &w1s);
+                ELAUNCH((ke_upsweep<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, pc, rr, rr2, rz_cur, w1,
+                        (float)a1, (float)w1s);
+                ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, 2, (int)S_RN, (int)G_INNER);
+                std::swap(rr, rr2);
+            } else {
+                ELAUNCH(ke_update, e->grid, 256, 0, e->d_mem, e->t1, e->n1, pc, rr, rz_cur, w1);
+                ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 2, (int)S_RN, (int)G_INNER);
+            }
             ELAUNCH(ke_inner, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.inner_maxit, o.maxit);
             if (it + 1 == e->eo.inner_iters) break;
             std::swap(rz_cur, rz_prev);
-            ECHK(sched_vcycle<C>(S, rz_cur, &zres));
+            ECHK(sched_vcycle<C>(S, rz_cur, &zres, rr, fuse));
         }
         ELAUNCH(ke_add, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
     }
@@ -761,6 +788,7 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
     wpf_solver_opts o;
     solver_defaults(opts, &o);
     WPF_CHECK(o.solver == WPF_SOLVER_MIXED, WPF_ERR_ARG, "ensembles run the mixed solver");
+    WPF_CHECK(!(o.flags & WPF_FLAG_PROJECT), WPF_ERR_ARG, "the projected initial guess is single-grid only");
     WPF_CHECK(!(o.flags & WPF_FLAG_CHEBYSHEV) || o.mg_sweeps == 2, WPF_ERR_ARG,
               "Chebyshev smoothing is the degree-2 polynomial: mg_sweeps must be 2");
     const int dev = members[0]->dev;
@@ -890,10 +918,10 @@ int wpf_ensemble_tune(wpf_ensemble* e, int32_t* grid_blocks, int32_t* tile_depth
             auto sweep = [&]() -> int {
                 if (e->op16)
                     ELAUNCH((ke_vsweep<_Float16, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv,
-                            (int)EV_Z, -1, (int)EV_T, 0.0f, 0.5f, 0.0f);
+                            (int)EV_R, (int)EV_Z, -1, (int)EV_T, 0.0f, 0.5f, 0.0f);
                 else
                     ELAUNCH((ke_vsweep<float, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv,
-                            (int)EV_Z, -1, (int)EV_T, 0.0f, 0.5f, 0.0f);
+                            (int)EV_R, (int)EV_Z, -1, (int)EV_T, 0.0f, 0.5f, 0.0f);
                 return WPF_OK;
             };
             if ((rc = sweep())) break;

@@ -238,6 +238,7 @@ struct Fine {
     const C *tx, *ty, *tz;
     const R* invD;
     R *r, *Z, *T, *pa, *pb, *q;
+    R* r2 = nullptr;  // second residual buffer of the fused update (nullptr: none)
 };
 
 // cells per thread of the vector kernels (one 16-byte vector)
@@ -360,14 +361,15 @@ static int dir_mv(wpf_ctx* c, const Fine<R, C>& F, const R* z, const R* pold, R*
 // The result pointer is returned through *zres; (r, z) goes to result slot.
 template <typename R, typename C>
 static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>& F, const MGOpts& o,
-                       int dot_slot, R** zres)
+                       int dot_slot, R** zres, bool pre_done = false)
 {
-    R* cur = F.Z;
-    R* oth = F.T;
+    // pre_done: the fused update already took pre-smoothing step 1 into T
+    R* cur = pre_done ? F.T : F.Z;
+    R* oth = pre_done ? F.Z : F.T;
     double a, w;
     Pro<R> none{nullptr, 0, 0, (R)0};
     int rc;
-    for (int s = 1; s < o.sweeps; s++) {
+    for (int s = pre_done ? 2 : 1; s < o.sweeps; s++) {
         step_coef(o, s, &a, &w);
         if ((rc = sweep<R, C, false, false, false>(c, F, a, w, cur, nullptr, none, oth, dot_slot))) return rc;
         std::swap(cur, oth);
@@ -415,9 +417,11 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>
 // mg == nullptr: Jacobi.  Stops when ||invD r|| <= tol or after maxit
 // iterations; returns the iterations done and the last recursive norm.
 template <typename R, typename X, typename C>
-static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F, const MGOpts& o,
+static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F0, const MGOpts& o,
                     X* x, double tol, int maxit, int* its_out, double* rn_out)
 {
+    Fine<R, C> F = F0;  // r and r2 trade places at every fused update
+    const bool fuse = mg && F.vec && o.sweeps >= 2 && F.r2 != nullptr;
     const int64_t n = F.g.n;
     const R w = mg ? (R)o.w1 : (R)1;
     int rz_cur = S_RZ0, rz_prev = S_RZ1;
@@ -438,12 +442,26 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
     while (its < maxit) {
         if (int rc = dir_mv(c, F, (const R*)zres, (const R*)pc, pn, rz_cur, rz_prev, first)) return rc;
         std::swap(pc, pn);
-        {
+        if (fuse) {
+            // update + pre-smoothing step 1 (into T) in one pass
+            double a1, w1s;
+            step_coef(o, 1, &a1, &w1s);
+            constexpr int V = vwidth<R>();
+            prof_mark(c, WPF_KERNEL_CG_UPDATE, true);
+            hipLaunchKernelGGL((k_vupsweep<R, X, C, V>), dim3(vz_blocks(F.V)), dim3(256), 0, c->stream, F.V,
+                               (const double*)c->red.result, rz_cur, (int)S_PQ, w, (R)a1, (R)w1s, x, (const R*)pc,
+                               (const R*)F.r, (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, F.T,
+                               c->red.partials);
+            prof_mark(c, WPF_KERNEL_CG_UPDATE, false);
+            WPF_HIP(hipGetLastError());
+            FINALIZE(vz_blocks(F.V), 2, S_RN);
+            std::swap(F.r, F.r2);
+        } else {
             const int kid_ = WPF_KERNEL_CG_UPDATE;
             LAUNCH1D((k_update<R, X>), n, n, (const double*)c->red.result, rz_cur, S_PQ, w, x,
                      (const R*)pc, F.r, (const R*)F.q, F.invD, F.Z, RED(S_RN));
+            FINALIZE(blocks_for(n), 2, S_RN);
         }
-        FINALIZE(blocks_for(n), 2, S_RN);
         its++;
         first = 0;
         int rc = read_results(c, S_RN, 2);
@@ -456,7 +474,7 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
         if (rn <= tol) break;
         std::swap(rz_cur, rz_prev);
         if (mg) {
-            rc = vcycle_fine(c, *mg, F, o, rz_cur, &zres);
+            rc = vcycle_fine(c, *mg, F, o, rz_cur, &zres, fuse);
             if (rc) return rc;
         } else {
             zres = F.Z;
@@ -488,7 +506,7 @@ static int ensure_ws(wpf_ctx* c, bool f32)
     int rc;
     if (!c->z2.raw) {
         if ((rc = dev_alloc(c->z2, c->g)) || (rc = dev_alloc(c->pp2, c->g)) ||
-            (rc = dev_alloc(c->psave, c->g)))
+            (rc = dev_alloc(c->psave, c->g)) || (rc = dev_alloc(c->rb, c->g)))
             return rc;
     }
     if (c->ninj > 0 && !c->fsrc.raw) {
@@ -500,7 +518,7 @@ static int ensure_ws(wpf_ctx* c, bool f32)
     if (f32 && c->mg32.empty()) {
         c->mg32.resize(1);
         if ((rc = alloc_level(c->mg32[0], c->g))) return rc;
-        if ((rc = dev_alloc(c->r32, c->g)) || (rc = dev_alloc(c->z32, c->g)) ||
+        if ((rc = dev_alloc(c->r32, c->g)) || (rc = dev_alloc(c->r32b, c->g)) || (rc = dev_alloc(c->z32, c->g)) ||
             (rc = dev_alloc(c->p32, c->g)) || (rc = dev_alloc(c->q32, c->g)) ||
             (rc = dev_alloc(c->x32, c->g)) || (rc = dev_alloc(c->t32, c->g)) ||
             (rc = dev_alloc(c->p32b, c->g)))
@@ -517,9 +535,10 @@ static int prepare(wpf_ctx* c, const wpf_solver_opts& o, double* bnorm)
                            c->ndir, c->dir_idx, c->dir_val, c->p.p);
         WPF_HIP(hipGetLastError());
     }
-    if (o.flags & WPF_FLAG_EXTRAPOLATE) {
+    if (o.flags & (WPF_FLAG_EXTRAPOLATE | WPF_FLAG_PROJECT)) {
+        // the projection only needs the previous solution saved (psave)
         double ratio = 1.0;
-        int valid = c->have_prev ? 1 : 0;
+        int valid = (c->have_prev && (o.flags & WPF_FLAG_EXTRAPOLATE)) ? 1 : 0;
         if (c->dt_hist[0] > 0 && c->dt_hist[1] > 0) {
             ratio = c->dt_hist[0] / c->dt_hist[1];
             if (ratio > 4.0 || ratio < 0.25) valid = 0;
@@ -564,6 +583,7 @@ static Fine<double, double> fine64(wpf_ctx* c)
     set_shape(F, c->tuned_zc[1] > 0 ? c->tuned_zc[1] : 8);
     F.tx = c->tx.p; F.ty = c->ty.p; F.tz = c->tz.p; F.invD = c->invD.p;
     F.r = c->r.p; F.Z = c->z.p; F.T = c->z2.p; F.pa = c->pp.p; F.pb = c->pp2.p; F.q = c->q.p;
+    F.r2 = c->rb.p;
     return F;
 }
 
@@ -708,6 +728,54 @@ static int learned_start(wpf_ctx* c, const Fine<float, C>& F, double rn, float* 
     return WPF_OK;
 }
 
+// Projected initial guess (k_vgram / k_proj_solve / k_vproj_update): x and
+// the FP32 residual in F.r move to the A-norm-optimal point of
+// x + span(last solution changes); *rn becomes the new ||invD r||.
+template <typename C, int M>
+static int project_m(wpf_ctx* c, const Fine<float, C>& F, double* x, double* rn)
+{
+    ProjVecs<float> V{};
+    for (int j = 0; j < M; j++) {
+        const int slot = (c->proj_next - 1 - j + 2 * wpf_ctx::kProjDepth) % wpf_ctx::kProjDepth;
+        V.v[j] = c->proj[slot].p;
+    }
+    constexpr int NV = M * (M + 1) / 2 + M;
+    const int nb = vz_blocks(F.V);
+    hipLaunchKernelGGL((k_vgram<float, C, 4, M>), dim3(nb), dim3(256), 0, c->stream, F.V, V, (const float*)F.r,
+                       F.tx, F.ty, F.tz, (const uint8_t*)c->mask, c->red.partials);
+    WPF_HIP(hipGetLastError());
+    FINALIZE(nb, NV, S_GRAM);
+    hipLaunchKernelGGL((k_proj_solve<M>), dim3(1), dim3(64), 0, c->stream, (const double*)c->red.result + S_GRAM,
+                       c->red.result + S_PROJY);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_vproj_update<float, C, 4, M>), dim3(nb), dim3(256), 0, c->stream, F.V, V,
+                       (const double*)c->red.result + S_PROJY, x, F.r, F.tx, F.ty, F.tz, F.invD,
+                       (const uint8_t*)c->mask, c->red.partials);
+    WPF_HIP(hipGetLastError());
+    FINALIZE(nb, 1, S_PN);
+    int rc = read_results(c, S_PN, 1);
+    if (rc) return rc;
+    rc = read_results(c, S_PROJY + M, 1);
+    if (rc) return rc;
+    c->proj_used = c->red.host[S_PROJY + M] != 0.0;
+    *rn = std::sqrt(c->red.host[S_PN]);
+    return WPF_OK;
+}
+
+template <typename C>
+static int project(wpf_ctx* c, const Fine<float, C>& F, double* x, double* rn)
+{
+    switch (c->proj_count) {
+    case 1: return project_m<C, 1>(c, F, x, rn);
+    case 2: return project_m<C, 2>(c, F, x, rn);
+    case 3: return project_m<C, 3>(c, F, x, rn);
+    case 4: return project_m<C, 4>(c, F, x, rn);
+    case 5: return project_m<C, 5>(c, F, x, rn);
+    case 6: return project_m<C, 6>(c, F, x, rn);
+    default: return WPF_OK;
+    }
+}
+
 static void finish(wpf_ctx* c, const wpf_solver_opts& o)
 {
     if ((o.flags & WPF_FLAG_CAPTURE) && c->cap_target) {
@@ -715,7 +783,16 @@ static void finish(wpf_ctx* c, const wpf_solver_opts& o)
                            c->g.n, (const double*)c->p.p, (const double*)c->cap_guess, c->cap_target);
         c->have_capture = true;
     }
-    if (o.flags & WPF_FLAG_EXTRAPOLATE) {
+    if ((o.flags & WPF_FLAG_PROJECT) && c->proj[c->proj_next].raw) {
+        // this solve's change of the solution joins the projection space
+        const int64_t n = c->g.n;
+        hipLaunchKernelGGL(k_proj_push, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                           (const double*)c->p.p, (const double*)c->psave.p, (const uint8_t*)c->mask,
+                           c->proj[c->proj_next].p);
+        c->proj_next = (c->proj_next + 1) % wpf_ctx::kProjDepth;
+        c->proj_count = std::min(c->proj_count + 1, (int)wpf_ctx::kProjDepth);
+    }
+    if (o.flags & (WPF_FLAG_EXTRAPOLATE | WPF_FLAG_PROJECT)) {
         std::swap(c->xprev, c->psave);  // xprev now holds the previous solution
         c->have_prev = true;
     } else {
@@ -793,6 +870,15 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
     if ((rc = prepare(c, o, &bnorm))) return rc;
     int learned = 0;
     if ((rc = start_stage(c, o, &learned, false))) return rc;
+    // projected initial guess: needs earlier solution changes and the vector
+    // kernels (nx % 4 == 0)
+    if ((o.flags & WPF_FLAG_PROJECT) && !c->proj[0].raw) {
+        for (int j = 0; j < wpf_ctx::kProjDepth; j++)
+            if ((rc = dev_alloc(c->proj[j], c->g))) return rc;
+        c->proj_count = c->proj_next = 0;
+    }
+    c->proj_used = false;
+    const bool project_now = (o.flags & WPF_FLAG_PROJECT) && c->proj_count > 0 && c->g.nx % 4 == 0;
     const bool want_net = (o.flags & (WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG)) && c->net;
     // without an extrapolated guess the network is the only warm start and
     // is always used; otherwise the probe decides
@@ -837,11 +923,12 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
     F.invD = c->mg32[0].invD.p;
     F.r = c->r32.p; F.Z = c->z32.p; F.T = c->t32.p; F.pa = c->p32.p; F.pb = c->p32b.p;
     F.q = c->q32.p;
+    F.r2 = c->r32b.p;
     Fine<float, _Float16> H;
     H.G = F.G; H.V = F.V; H.vec = F.vec; H.g = F.g;
     H.tx = c->h_tx.p; H.ty = c->h_ty.p; H.tz = c->h_tz.p;
     H.invD = F.invD;
-    H.r = F.r; H.Z = F.Z; H.T = F.T; H.pa = F.pa; H.pb = F.pb; H.q = F.q;
+    H.r = F.r; H.Z = F.Z; H.T = F.T; H.pa = F.pa; H.pb = F.pb; H.q = F.q; H.r2 = F.r2;
     if ((rc = op16 ? tune_shape(c, H, &c->tuned_zc[0]) : tune_shape(c, F, &c->tuned_zc[0]))) return rc;
     set_shape(F, c->tuned_zc[0]);
     set_shape(H, c->tuned_zc[0]);
@@ -866,6 +953,10 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         if (rn <= tol || its >= o.maxit || outer >= 50) break;
         outer++;
         LAUNCH1D(k_zero<float>, n, n, e);
+        if (outer == 1 && project_now) {
+            rc = op16 ? project(c, H, x, &rn) : project(c, F, x, &rn);
+            if (rc) return rc;
+        }
         if (learned_pending) {
             learned_pending = false;
             rc = op16 ? learned_start(c, H, rn, e) : learned_start(c, F, rn, e);
@@ -952,9 +1043,11 @@ void free_solver(wpf_ctx* c)
     c->mg64.clear();
     for (size_t l = 0; l < c->mg32.size(); l++) free_level(c->mg32[l]);
     c->mg32.clear();
-    dev_free(c->r32); dev_free(c->z32); dev_free(c->p32); dev_free(c->q32); dev_free(c->x32);
+    dev_free(c->r32); dev_free(c->r32b); dev_free(c->rb); dev_free(c->z32); dev_free(c->p32); dev_free(c->q32); dev_free(c->x32);
     dev_free(c->t32); dev_free(c->p32b);
     dev_free(c->h_tx); dev_free(c->h_ty); dev_free(c->h_tz);
+    for (int j = 0; j < wpf_ctx::kProjDepth; j++) dev_free(c->proj[j]);
+    c->proj_count = c->proj_next = 0;
     dev_free(c->z2); dev_free(c->pp2); dev_free(c->psave); dev_free(c->fsrc);
 }
 

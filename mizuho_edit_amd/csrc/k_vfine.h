@@ -232,6 +232,389 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
     if (DOT) block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
 }
 
+// The CG update fused with the first V-cycle step that follows it:
+//     e += alpha p,  r' = r - alpha q  (written to r2, not over r, which the
+//     neighbouring threads still read),  z0 = w1 (r' invD)  (never stored),
+//     y = z0 + a z0 + w invD (r' - A z0)  (the second pre-smoothing step,
+//     from x_{-1} = 0, with z0 formed on the fly for the neighbours too).
+// Reduces ||invD r'||^2 and (r', z0) like k_update.  The values are those
+// of k_update followed by k_vsweep; only the reduction's partition differs.
+template <typename R, typename X, typename C, int V>
+__device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __restrict__ res, int num,
+                                           int den, R w1, R a, R w, X* __restrict__ e,
+                                           const R* __restrict__ p, const R* __restrict__ r,
+                                           const R* __restrict__ q, R* __restrict__ r2,
+                                           const C* __restrict__ tx, const C* __restrict__ ty,
+                                           const C* __restrict__ tz, const R* __restrict__ invD, R* y,
+                                           double (&acc)[2])
+{
+    const R al = (R)(res[num] / res[den]);
+    int64_t qi;
+    int k0, k1;
+    const bool act = vz_at<V>(Z, bi, qi, k0, k1);
+    const int lane = threadIdx.x & 63;
+    const int64_t nx = Z.nx, pl = Z.plane;
+    acc[0] = acc[1] = 0.0;
+    // r' and z0 of the V cells at m
+    auto rz = [&](int64_t m, R(&rn)[V], R(&z0)[V]) {
+        R rv[V], qv[V], dv[V];
+        vld<V>(r + m, rv);
+        vld<V>(q + m, qv);
+        vld<V>(invD + m, dv);
+#pragma unroll
+        for (int h = 0; h < V; h++) {
+            rn[h] = rv[h] - al * qv[h];
+            z0[h] = w1 * (rn[h] * dv[h]);
+        }
+    };
+    auto z1 = [&](int64_t m) { return w1 * ((r[m] - al * q[m]) * invD[m]); };
+    R rm[V], r0[V], rp[V], zm[V], z0[V], zp[V], tzm[V];
+    int64_t c = qi + pl * k0;
+    rz(c - pl, rm, zm);
+    rz(c, r0, z0);
+    vldc<V>(tz + c - pl, tzm);
+    for (int k = k0; k < k1; k++, c += pl) {
+        rz(c + pl, rp, zp);
+        R ryp[V], zyp[V], rym[V], zym[V], txv[V], tyv[V], tym[V], tzv[V], di[V], pv[V];
+        rz(c + nx, ryp, zyp);
+        rz(c - nx, rym, zym);
+        vldc<V>(tx + c, txv);
+        vldc<V>(ty + c, tyv);
+        vldc<V>(ty + c - nx, tym);
+        vldc<V>(tz + c, tzv);
+        vld<V>(invD + c, di);
+        vld<V>(p + c, pv);
+        R zl = from_left(z0[V - 1]), zr = from_right(z0[0]), txl = from_left(txv[V - 1]);
+        if (lane == 0) {
+            zl = z1(c - 1);
+            txl = (R)tx[c - 1];
+        }
+        if (lane == 63) zr = z1(c + V);
+        X ev[V];
+        vld<V>(e + c, ev);
+        R yv[V];
+#pragma unroll
+        for (int h = 0; h < V; h++) {
+            ev[h] = ev[h] + (X)(al * pv[h]);
+            const R zrh = r0[h] * di[h];
+            acc[0] += (double)zrh * (double)zrh;
+            acc[1] += (double)r0[h] * (double)(w1 * zrh);
+            const R txm = h ? txv[h - 1] : txl;
+            const R zle = h ? z0[h - 1] : zl;
+            const R zre = (h + 1 < V) ? z0[h + 1] : zr;
+            const R d = txv[h] + txm + tyv[h] + tym[h] + tzv[h] + tzm[h];
+            const R s = txv[h] * zre + txm * zle + tyv[h] * zyp[h] + tym[h] * zym[h] + tzv[h] * zp[h] +
+                        tzm[h] * zm[h];
+            R v = z0[h] + w * di[h] * (r0[h] - (d * z0[h] - s));
+            if (a != (R)0) v += a * (z0[h] - (R)0);
+            yv[h] = v;
+        }
+        if (act) {
+            vst<V>(e + c, ev);
+            vst<V>(r2 + c, r0);
+            vst<V>(y + c, yv);
+        }
+#pragma unroll
+        for (int h = 0; h < V; h++) {
+            rm[h] = r0[h];
+            r0[h] = rp[h];
+            zm[h] = z0[h];
+            z0[h] = zp[h];
+            tzm[h] = tzv[h];
+        }
+    }
+    if (!act) acc[0] = acc[1] = 0.0;
+}
+
+template <typename R, typename X, typename C, int V>
+__global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict__ res, int num, int den, R w1,
+                                                  R a, R w, X* __restrict__ e, const R* __restrict__ p,
+                                                  const R* __restrict__ r, const R* __restrict__ q,
+                                                  R* __restrict__ r2, const C* __restrict__ tx,
+                                                  const C* __restrict__ ty, const C* __restrict__ tz,
+                                                  const R* __restrict__ invD, R* y,
+                                                  double* __restrict__ partials)
+{
+    double acc[2];
+    t_vupsweep<R, X, C, V>(Z, blockIdx.x, res, num, den, w1, a, w, e, p, r, q, r2, tx, ty, tz, invD, y, acc);
+    block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Projection of the initial guess onto the span of earlier solution changes
+// (P. F. Fischer, "Projection techniques for iterative solution of Ax = b
+// with successive right-hand sides", Comput. Methods Appl. Mech. Engrg. 163
+// (1998)): with V = [v_1 .. v_M] the last M changes of the solution and r
+// the residual of the starting guess, the A-norm-optimal correction in
+// span(V) is V y with (V^T A V) y = V^T r.
+// ---------------------------------------------------------------------------
+constexpr int kProjMax = 6;
+
+// the M vectors, passed by value
+template <typename R>
+struct ProjVecs {
+    const R* v[kProjMax];
+};
+
+// packed upper triangle index of (i, j), i <= j, of an M x M matrix
+__host__ __device__ constexpr int tri(int i, int j, int M) { return i * M - i * (i - 1) / 2 + (j - i); }
+
+// A v of the V cells at c (free rows; fixed rows 0) from the three planes
+// and the two in-plane rows of v held by the caller
+template <typename R, int V>
+__device__ __forceinline__ void stencil_rows(const R (&txv)[V], R txl, const R (&tyv)[V], const R (&tym)[V],
+                                             const R (&tzv)[V], const R (&tzm)[V], const R (&vm)[V],
+                                             const R (&v0)[V], const R (&vp)[V], const R (&vyp)[V],
+                                             const R (&vym)[V], R vl, R vr, const uint8_t (&m)[V],
+                                             R (&out)[V])
+{
+#pragma unroll
+    for (int h = 0; h < V; h++) {
+        const R txm = h ? txv[h - 1] : txl;
+        const R le = h ? v0[h - 1] : vl;
+        const R re = (h + 1 < V) ? v0[h + 1] : vr;
+        const R d = txv[h] + txm + tyv[h] + tym[h] + tzv[h] + tzm[h];
+        const R s = txv[h] * re + txm * le + tyv[h] * vyp[h] + tym[h] * vym[h] + tzv[h] * vp[h] + tzm[h] * vm[h];
+        out[h] = (m[h] & WPF_MASK_FIXED) ? (R)0 : d * v0[h] - s;
+    }
+}
+
+// Gram pass: G = V^T A V (packed upper triangle) and c = V^T r, one pass
+// over the M vectors, r and the faces.  acc[tri(i, j)] and acc[NG + i].
+template <typename R, typename C, int V, int M>
+__device__ __forceinline__ void t_vgram(const VZ& Z, int bi, const ProjVecs<R>& vs,
+                                        const R* __restrict__ r, const C* __restrict__ tx,
+                                        const C* __restrict__ ty, const C* __restrict__ tz,
+                                        const uint8_t* __restrict__ mask, double (&acc)[M * (M + 1) / 2 + M])
+{
+    constexpr int NG = M * (M + 1) / 2;
+    int64_t q;
+    int k0, k1;
+    const bool act = vz_at<V>(Z, bi, q, k0, k1);
+    const int lane = threadIdx.x & 63;
+    const int64_t nx = Z.nx, pl = Z.plane;
+#pragma unroll
+    for (int t = 0; t < NG + M; t++) acc[t] = 0.0;
+    R vm[M][V], v0[M][V], vp[M][V], tzm[V];
+    int64_t c = q + pl * k0;
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        vld<V>(vs.v[j] + c - pl, vm[j]);
+        vld<V>(vs.v[j] + c, v0[j]);
+    }
+    vldc<V>(tz + c - pl, tzm);
+    for (int k = k0; k < k1; k++, c += pl) {
+        R txv[V], tyv[V], tym[V], tzv[V], rv[V];
+        uint8_t mk[V];
+        vldc<V>(tx + c, txv);
+        vldc<V>(ty + c, tyv);
+        vldc<V>(ty + c - nx, tym);
+        vldc<V>(tz + c, tzv);
+        vld<V>(r + c, rv);
+        vld<V>(mask + c, mk);
+        R txl = from_left(txv[V - 1]);
+        if (lane == 0) txl = (R)tx[c - 1];
+        R av[M][V];
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            R vyp[V], vym[V];
+            vld<V>(vs.v[j] + c + pl, vp[j]);
+            vld<V>(vs.v[j] + c + nx, vyp);
+            vld<V>(vs.v[j] + c - nx, vym);
+            R vl = from_left(v0[j][V - 1]), vr = from_right(v0[j][0]);
+            if (lane == 0) vl = vs.v[j][c - 1];
+            if (lane == 63) vr = vs.v[j][c + V];
+            stencil_rows<R, V>(txv, txl, tyv, tym, tzv, tzm, vm[j], v0[j], vp[j], vyp, vym, vl, vr, mk, av[j]);
+        }
+        if (act) {
+#pragma unroll
+            for (int h = 0; h < V; h++) {
+#pragma unroll
+                for (int i = 0; i < M; i++) {
+                    acc[NG + i] += (double)v0[i][h] * (double)rv[h];
+#pragma unroll
+                    for (int j = i; j < M; j++) acc[tri(i, j, M)] += (double)v0[i][h] * (double)av[j][h];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; j++)
+#pragma unroll
+            for (int h = 0; h < V; h++) {
+                vm[j][h] = v0[j][h];
+                v0[j][h] = vp[j][h];
+            }
+#pragma unroll
+        for (int h = 0; h < V; h++) tzm[h] = tzv[h];
+    }
+}
+
+template <typename R, typename C, int V, int M>
+__global__ void __launch_bounds__(256) k_vgram(VZ Z, ProjVecs<R> vs, const R* __restrict__ r,
+                                               const C* __restrict__ tx, const C* __restrict__ ty,
+                                               const C* __restrict__ tz, const uint8_t* __restrict__ mask,
+                                               double* __restrict__ partials)
+{
+    double acc[M * (M + 1) / 2 + M];
+    t_vgram<R, C, V, M>(Z, blockIdx.x, vs, r, tx, ty, tz, mask, acc);
+    block_reduce_store<M * (M + 1) / 2 + M>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
+}
+
+// y = G^+ c for the packed Gram matrix in res[0 ..) and c after it (one
+// thread): Cholesky with a pivot threshold; a direction whose pivot falls
+// below 1e-12 of its diagonal is dependent on the earlier ones and gets
+// y = 0.  y[M] = 1 when any direction was used.
+template <int M>
+__device__ __forceinline__ void proj_solve(const double* __restrict__ res, double* __restrict__ y)
+{
+    constexpr int NG = M * (M + 1) / 2;
+    double L[M][M], b[M];
+    bool use[M];
+    for (int i = 0; i < M; i++) {
+        b[i] = res[NG + i];
+        for (int j = 0; j < M; j++) L[i][j] = 0.0;
+    }
+    for (int j = 0; j < M; j++) {
+        const double gjj = res[tri(j, j, M)];
+        double d = gjj;
+        for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k];
+        use[j] = gjj > 0.0 && d > 1e-12 * gjj;
+        if (!use[j]) continue;
+        L[j][j] = sqrt(d);
+        for (int i = j + 1; i < M; i++) {
+            double s = res[tri(j, i, M)];
+            for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
+            L[i][j] = s / L[j][j];
+        }
+    }
+    double z[M];
+    for (int i = 0; i < M; i++) {
+        double s = b[i];
+        for (int k = 0; k < i; k++) s -= L[i][k] * z[k];
+        z[i] = use[i] ? s / L[i][i] : 0.0;
+    }
+    int any = 0;
+    for (int i = M - 1; i >= 0; i--) {
+        double s = z[i];
+        for (int k = i + 1; k < M; k++) s -= L[k][i] * y[k];
+        y[i] = use[i] ? s / L[i][i] : 0.0;
+        any |= use[i] && y[i] != 0.0;
+        if (!isfinite(y[i])) y[i] = 0.0;
+    }
+    y[M] = any ? 1.0 : 0.0;
+}
+
+template <int M>
+__global__ void k_proj_solve(const double* __restrict__ res, double* __restrict__ y)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) proj_solve<M>(res, y);
+}
+
+// x += V y (FP64 field), r -= A V y (FP32 residual), reduces ||invD r||^2.
+template <typename R, typename C, int V, int M>
+__device__ __forceinline__ double t_vproj_update(const VZ& Z, int bi, const ProjVecs<R>& vs,
+                                                 const double* __restrict__ y, double* __restrict__ x,
+                                                 R* __restrict__ r, const C* __restrict__ tx,
+                                                 const C* __restrict__ ty, const C* __restrict__ tz,
+                                                 const R* __restrict__ invD, const uint8_t* __restrict__ mask)
+{
+    int64_t q;
+    int k0, k1;
+    const bool act = vz_at<V>(Z, bi, q, k0, k1);
+    const int lane = threadIdx.x & 63;
+    const int64_t nx = Z.nx, pl = Z.plane;
+    R yc[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) yc[j] = (R)y[j];
+    // u = V y of the V cells at m
+    auto urow = [&](int64_t m, R(&u)[V]) {
+#pragma unroll
+        for (int h = 0; h < V; h++) u[h] = (R)0;
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            R t[V];
+            vld<V>(vs.v[j] + m, t);
+#pragma unroll
+            for (int h = 0; h < V; h++) u[h] += yc[j] * t[h];
+        }
+    };
+    auto uone = [&](int64_t m) {
+        R u = (R)0;
+#pragma unroll
+        for (int j = 0; j < M; j++) u += yc[j] * vs.v[j][m];
+        return u;
+    };
+    double acc = 0.0;
+    R um[V], u0[V], up[V], tzm[V];
+    int64_t c = q + pl * k0;
+    urow(c - pl, um);
+    urow(c, u0);
+    vldc<V>(tz + c - pl, tzm);
+    for (int k = k0; k < k1; k++, c += pl) {
+        urow(c + pl, up);
+        R uyp[V], uym[V], txv[V], tyv[V], tym[V], tzv[V], rv[V], di[V], au[V];
+        uint8_t mk[V];
+        urow(c + nx, uyp);
+        urow(c - nx, uym);
+        vldc<V>(tx + c, txv);
+        vldc<V>(ty + c, tyv);
+        vldc<V>(ty + c - nx, tym);
+        vldc<V>(tz + c, tzv);
+        vld<V>(r + c, rv);
+        vld<V>(invD + c, di);
+        vld<V>(mask + c, mk);
+        R txl = from_left(txv[V - 1]), ul = from_left(u0[V - 1]), ur = from_right(u0[0]);
+        if (lane == 0) {
+            txl = (R)tx[c - 1];
+            ul = uone(c - 1);
+        }
+        if (lane == 63) ur = uone(c + V);
+        stencil_rows<R, V>(txv, txl, tyv, tym, tzv, tzm, um, u0, up, uyp, uym, ul, ur, mk, au);
+        double xv[V];
+        vld<V>(x + c, xv);
+#pragma unroll
+        for (int h = 0; h < V; h++) {
+            if (!(mk[h] & WPF_MASK_FIXED)) xv[h] += (double)u0[h];
+            rv[h] = rv[h] - au[h];
+            const R zr = rv[h] * di[h];
+            acc += (double)zr * (double)zr;
+        }
+        if (act) {
+            vst<V>(x + c, xv);
+            vst<V>(r + c, rv);
+        }
+#pragma unroll
+        for (int h = 0; h < V; h++) {
+            um[h] = u0[h];
+            u0[h] = up[h];
+            tzm[h] = tzv[h];
+        }
+    }
+    return act ? acc : 0.0;
+}
+
+template <typename R, typename C, int V, int M>
+__global__ void __launch_bounds__(256) k_vproj_update(VZ Z, ProjVecs<R> vs,
+                                                      const double* __restrict__ y, double* __restrict__ x,
+                                                      R* __restrict__ r, const C* __restrict__ tx,
+                                                      const C* __restrict__ ty, const C* __restrict__ tz,
+                                                      const R* __restrict__ invD, const uint8_t* __restrict__ mask,
+                                                      double* __restrict__ partials)
+{
+    double acc[1] = {t_vproj_update<R, C, V, M>(Z, blockIdx.x, vs, y, x, r, tx, ty, tz, invD, mask)};
+    block_reduce_store<1>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
+}
+
+// v = (float)(p - p_prev) on free cells (0 on fixed cells)
+static __global__ void __launch_bounds__(256) k_proj_push(int64_t n, const double* __restrict__ p,
+                                                          const double* __restrict__ pprev,
+                                                          const uint8_t* __restrict__ mask, float* __restrict__ v)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    v[c] = (mask[c] & WPF_MASK_FIXED) ? 0.0f : (float)(p[c] - pprev[c]);
+}
+
 // p_new = z (+ beta p_old), q = A p_new on free cells, reduces (p_new, q).
 template <typename R, typename C, int V>
 __device__ __forceinline__ double t_vdir_mv(const VZ& Z, int bi, const C* __restrict__ tx,

@@ -224,7 +224,7 @@ int wpf_create(const wpf_params* p, const int32_t* lw_haloed, const wpf_well* we
         WPF_HIP(hipMemcpy(c->bt_step, &v, 4, hipMemcpyHostToDevice));
     }
     c->bt_plane = g.nx >= 2 ? g.nx - 2 : -1;
-    if ((rc = reducer_init(c->red, 32, 65536))) {
+    if ((rc = reducer_init(c->red, 64, 65536))) {
         wpf_destroy(c);
         return rc;
     }
@@ -287,6 +287,7 @@ int wpf_upload(wpf_ctx* c, int32_t field, const double* h)
         }
     WPF_HIP(hipMemcpyAsync(d, tmp.data(), sizeof(double) * g.n, hipMemcpyHostToDevice, c->stream));
     WPF_HIP(hipStreamSynchronize(c->stream));
+    if (field == WPF_FIELD_PRESSURE) c->proj_count = 0;  // earlier solution changes no longer apply
     if (field == WPF_FIELD_PERMEAB_ROCK) {
         double m = 0.0;
         for (double v : tmp) m = std::max(m, v);

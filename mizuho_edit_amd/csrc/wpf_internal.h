@@ -129,6 +129,14 @@ struct wpf_ctx {
     std::vector<wpf::MGLevel<double>> mg64;   // levels 1.. (coarse)
     std::vector<wpf::MGLevel<float>> mg32;    // levels 0.. (FP32 copy incl. fine)
     wpf::Dev<float> r32, z32, p32, q32, x32, t32, p32b;
+    wpf::Dev<float> r32b;                     // second FP32 residual (fused update)
+    // the last solution changes p_n - p_(n-1) (FP32 ring) for the projected
+    // initial guess (WPF_FLAG_PROJECT)
+    static constexpr int kProjDepth = 6;
+    wpf::Dev<float> proj[kProjDepth];
+    int proj_count = 0, proj_next = 0;
+    bool proj_used = false;
+    wpf::Dev<double> rb;                      // second FP64 residual (fused update)
     wpf::Dev<_Float16> h_tx, h_ty, h_tz;      // FP16 fine faces of the inner solve
     double krock_max = 0.0;                   // largest uploaded rock permeability
     // probe of the learned stage: solves with and without it, compared by

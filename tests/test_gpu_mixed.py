@@ -4,8 +4,8 @@ Physics of the mixed workload (rock GRF variance 2, porosity clipped to
 [0.02, 0.45]) at 64x32x32: FP32 phase-field and concentration stencils with
 FP64 fields, Euler update and taumin; the pressure by FP32 MG-PCG (degree-2
 Chebyshev smoothing, FP16-stored fine faces) inside FP64 iterative
-refinement to rtol 1e-10.  The oracle runs the same steps in FP64 with
-CG + Jacobi to rtol 1e-10.  Reported: the relative error of each field after
+refinement to rtol 1e-10, from the projected initial guess.  The oracle
+runs the same steps in FP64 with CG + Jacobi to rtol 1e-10.  Reported: the relative error of each field after
 the run, the largest relative dt difference, and the breakthrough step at
 the plane next to the inlet.
 
@@ -22,7 +22,8 @@ from oracle.oracle import OracleState
 
 pytestmark = pytest.mark.gpu
 
-FAST = _lib.FLAG_EXTRAPOLATE | _lib.FLAG_CHEBYSHEV | _lib.FLAG_OP16 | _lib.FLAG_F32_STENCIL
+FAST = (_lib.FLAG_EXTRAPOLATE | _lib.FLAG_CHEBYSHEV | _lib.FLAG_OP16 | _lib.FLAG_F32_STENCIL |
+        _lib.FLAG_PROJECT)
 
 
 def _interior(a):

@@ -181,7 +181,11 @@ def test_fused_step_matches_procedures(gpu):
     b.close()
 
 
-@pytest.mark.parametrize("solver,flags", [("mgcg", 0), ("mixed", 0), ("mixed", CHEB | OP16)])
+PROJ = _lib.FLAG_PROJECT | _lib.FLAG_EXTRAPOLATE
+
+
+@pytest.mark.parametrize("solver,flags", [("mgcg", 0), ("mixed", 0), ("mixed", CHEB | OP16),
+                                          ("mixed", CHEB | OP16 | PROJ)])
 def test_run_parity_small_workload(gpu, solver, flags):
     """64x32x32 workload, 60 steps from the loaded state, both sides solving
     tightly: every field within 1e-10 relative, identical dt history and the

@@ -214,7 +214,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--kernel", default="smooth", help="kernel reported in roofline")
     ap.add_argument("--no-cpu", action="store_true")

@@ -731,7 +731,7 @@ int sched_solve(wpf_ensemble* e)
 int sched_fields(wpf_ensemble* e)
 {
     const int nm = (int)e->mem.size();
-    ECHK(ens_step_fields(e->stream, e->d_mem, e->t3, e->n3, e->grid));
+    ECHK(ens_step_fields(e->stream, e->d_mem, nm, e->t3, e->n3, e->grid));
     ELAUNCH(ke_end_step, (nm + 63) / 64, 64, 0, e->d_mem, nm, e->extrap ? 1 : 0);
     return WPF_OK;
 }

@@ -214,7 +214,9 @@ __device__ __forceinline__ void gather19(const Dims& d, const double* __restrict
 }
 
 // block-level min of tau, one atomicMin on the ordered key per block
-__device__ __forceinline__ void block_min_key(double v, unsigned long long* key)
+// The block's minimum of v into mins[bid]: one plain store per block, so
+// the grid's minimum needs no contended atomics (k_min_key finishes it).
+__device__ __forceinline__ void block_min_store(double v, double* mins, int bid)
 {
     __shared__ double sh[16];
     const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
@@ -225,8 +227,30 @@ __device__ __forceinline__ void block_min_key(double v, unsigned long long* key)
     if (tid == 0) {
         double m = sh[0];
         for (int w = 1; w < (nthr >> 6); w++) m = fmin(m, sh[w]);
-        atomicMin(key, dkey(m));
+        mins[bid] = m;
     }
+    __syncthreads();
+}
+
+// key = dkey(min(1e6, mins[0..nb))): one block of 1024
+__device__ __forceinline__ void t_min_key(const double* __restrict__ mins, int nb, unsigned long long* key)
+{
+    __shared__ double sh[16];
+    double m = 1.0e6;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) m = fmin(m, mins[b]);
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) t = fmin(t, sh[w]);
+        *key = dkey(t);
+    }
+}
+__global__ void __launch_bounds__(1024) k_min_key(const double* __restrict__ mins, int nb,
+                                                  unsigned long long* __restrict__ keys, int key)
+{
+    t_min_key(mins, nb, keys + key);
 }
 
 // ---------------------------------------------------------------------------
@@ -240,7 +264,7 @@ __global__ void __launch_bounds__(256) k_phase_deriv(Dims d, PhaseConsts K,
                                                      const double* __restrict__ vz,
                                                      const uint8_t* __restrict__ mask,
                                                      double* __restrict__ dphidt,
-                                                     unsigned long long* tau_key)
+                                                     double* __restrict__ mins)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -257,7 +281,7 @@ __global__ void __launch_bounds__(256) k_phase_deriv(Dims d, PhaseConsts K,
         const bool well = (mask[n.c] & WPF_MASK_WELL) != 0;
         dphidt[n.c] = phase_cell(K, s, conc[n.c], vx[n.c], vxm, vy[n.c], vym, vz[n.c], vzm, well, &tau);
     }
-    block_min_key(tau, tau_key);
+    block_min_store(tau, mins, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
 int launch_phase_deriv(wpf_ctx* c, double* taumin)
@@ -265,18 +289,17 @@ int launch_phase_deriv(wpf_ctx* c, double* taumin)
     WPF_CHECK(c->have_velocity, WPF_ERR_STATE, "calcPhaseDeriv before calcVelocity");
     const Dims d = dims_of(c);
     const PhaseConsts K = make_consts(c->prm);
-    // taumin starts at 1e6 (1:1223)
-    double init = 1.0e6;
-    unsigned long long key;
-    {
-        unsigned long long u;
-        memcpy(&u, &init, 8);
-        key = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
-    }
-    WPF_HIP(hipMemcpyAsync(c->tau_key + 3, &key, 8, hipMemcpyHostToDevice, c->stream));
+    // taumin starts at 1e6 (1:1223): k_min_key folds that in
     dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(k_phase_deriv, grid3(d, blk), blk, 0, c->stream, d, K, c->phi[c->cur],
-                       c->conc[c->cur], c->vx, c->vy, c->vz, c->mask, c->dphidt, c->tau_key + 3);
+    const dim3 gr = grid3(d, blk);
+    const int nb = (int)(gr.x * gr.y * gr.z);
+    WPF_CHECK((int64_t)nb <= (int64_t)c->red.max_blocks * c->red.slots, WPF_ERR_ARG,
+              "grid too large for the reduction scratch");
+    hipLaunchKernelGGL(k_phase_deriv, gr, blk, 0, c->stream, d, K, c->phi[c->cur], c->conc[c->cur], c->vx,
+                       c->vy, c->vz, c->mask, c->dphidt, c->red.partials);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_min_key, dim3(1), dim3(1024), 0, c->stream, (const double*)c->red.partials, nb,
+                       c->tau_key, 3);
     WPF_HIP(hipGetLastError());
     unsigned long long out;
     WPF_HIP(hipMemcpyAsync(&out, c->tau_key + 3, 8, hipMemcpyDeviceToHost, c->stream));
@@ -378,7 +401,8 @@ __device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int
                                               const uint8_t* __restrict__ mask, double* __restrict__ phi_out,
                                               double* __restrict__ conc_out,
                                               unsigned long long* __restrict__ keys,
-                                              double* __restrict__ scal, int32_t* __restrict__ bt_step)
+                                              double* __restrict__ scal, int32_t* __restrict__ bt_step,
+                                              double* __restrict__ mins, int bid)
 {
     double dt = S.dt_cfg;
     if (S.device_dt && S.itime >= 3) {
@@ -419,7 +443,7 @@ __device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int
         conc_out[n.c] = c0 + (double)dcd * dt;
         if (i == S.bt_plane && !well && phn > 0.0) atomicMin(bt_step, S.itime);
     }
-    block_min_key((double)tau, &keys[S.key_cur]);
+    block_min_store((double)tau, mins, bid);
     if (bx == 0 && by == 0 && bz == 0 && threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
         // key_next was last read two steps ago: reset it for the next step
         keys[S.key_next] = dkey(1.0e6);
@@ -441,10 +465,12 @@ __global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConstsT<T> K, 
                                                      double* __restrict__ conc_out,
                                                      unsigned long long* __restrict__ keys,
                                                      double* __restrict__ scal,
-                                                     int32_t* __restrict__ bt_step)
+                                                     int32_t* __restrict__ bt_step,
+                                                     double* __restrict__ mins)
 {
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     t_step_fields<T>(d, blockIdx.x, blockIdx.y, blockIdx.z, K, dx, S, phi, conc, p, tx, ty, tz, mask, phi_out,
-                     conc_out, keys, scal, bt_step);
+                     conc_out, keys, scal, bt_step, mins, bid);
 }
 
 int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt, bool f32)
@@ -464,15 +490,23 @@ int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt,
     S.key_next = (itime + 1) % 3;
     dim3 blk(64, 4, 1);
     prof_mark(c, WPF_KERNEL_STEP_FIELDS, true);
+    const dim3 gr = grid3(d, blk);
+    const int nb = (int)(gr.x * gr.y * gr.z);
+    // the block minima use the reduction scratch as one flat array
+    WPF_CHECK((int64_t)nb <= (int64_t)c->red.max_blocks * c->red.slots, WPF_ERR_ARG,
+              "grid too large for the reduction scratch");
     if (f32)
-        hipLaunchKernelGGL(k_step_fields<float>, grid3(d, blk), blk, 0, c->stream, d, consts_as<float>(K),
-                           K.dx, S, c->phi[c->cur], c->conc[c->cur], c->p.p, c->tx.p, c->ty.p, c->tz.p,
-                           c->mask, c->phi[1 - c->cur], c->conc[1 - c->cur], c->tau_key, c->scal,
-                           c->bt_step);
-    else
-        hipLaunchKernelGGL(k_step_fields<double>, grid3(d, blk), blk, 0, c->stream, d, K, K.dx, S,
+        hipLaunchKernelGGL(k_step_fields<float>, gr, blk, 0, c->stream, d, consts_as<float>(K), K.dx, S,
                            c->phi[c->cur], c->conc[c->cur], c->p.p, c->tx.p, c->ty.p, c->tz.p, c->mask,
-                           c->phi[1 - c->cur], c->conc[1 - c->cur], c->tau_key, c->scal, c->bt_step);
+                           c->phi[1 - c->cur], c->conc[1 - c->cur], c->tau_key, c->scal, c->bt_step,
+                           c->red.partials);
+    else
+        hipLaunchKernelGGL(k_step_fields<double>, gr, blk, 0, c->stream, d, K, K.dx, S, c->phi[c->cur],
+                           c->conc[c->cur], c->p.p, c->tx.p, c->ty.p, c->tz.p, c->mask, c->phi[1 - c->cur],
+                           c->conc[1 - c->cur], c->tau_key, c->scal, c->bt_step, c->red.partials);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_min_key, dim3(1), dim3(1024), 0, c->stream, (const double*)c->red.partials, nb,
+                       c->tau_key, S.key_cur);
     prof_mark(c, WPF_KERNEL_STEP_FIELDS, false);
     WPF_HIP(hipGetLastError());
     c->cur = 1 - c->cur;
@@ -646,8 +680,16 @@ __global__ void __launch_bounds__(256) ke_step_fields(const EnsMember* __restric
         S.key_cur = itime % 3;
         S.key_next = (itime + 1) % 3;
         t_step_fields<double>(d, bx, by, bz, M.K, M.dx, S, M.phi[cur], M.conc[cur], M.p, M.tx, M.ty, M.tz,
-                              M.mask, M.phi[1 - cur], M.conc[1 - cur], M.tau_key, M.scal, M.bt_step);
+                              M.mask, M.phi[1 - cur], M.conc[1 - cur], M.tau_key, M.scal, M.bt_step,
+                              M.partials, T.b);
     }
+}
+
+// each member's taumin key from its blocks' minima (one workgroup each)
+__global__ void __launch_bounds__(1024) ke_min_key(const EnsMember* __restrict__ Ms)
+{
+    const EnsMember& M = Ms[blockIdx.x];
+    t_min_key(M.partials, M.nb3, M.tau_key + M.st->itime % 3);
 }
 
 int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid)
@@ -657,9 +699,12 @@ int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntile
     return WPF_OK;
 }
 
-int ens_step_fields(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid)
+int ens_step_fields(hipStream_t s, const EnsMember* Ms, int nmembers, const EnsTile* tiles, int ntiles,
+                    int grid)
 {
     hipLaunchKernelGGL(ke_step_fields, dim3(grid), dim3(64, 4, 1), 0, s, Ms, tiles, ntiles);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ke_min_key, dim3(nmembers), dim3(1024), 0, s, Ms);
     WPF_HIP(hipGetLastError());
     return WPF_OK;
 }

@@ -214,7 +214,8 @@ int check_breakthrough(wpf_ctx* c, int itime);
 struct EnsMember;
 struct EnsTile;
 int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid);
-int ens_step_fields(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid);
+int ens_step_fields(hipStream_t s, const EnsMember* Ms, int nmembers, const EnsTile* tiles, int ntiles,
+                    int grid);
 
 // solver (k_solver.hip)
 int solve_pressure(wpf_ctx* c, const wpf_solver_opts* o, wpf_solve_stats* st);
@@ -323,6 +324,9 @@ __device__ void block_reduce_store(double (&v)[NV], double* partials, unsigned* 
         for (int w = 0; w < nw; w++) s += sh[tid][w];
         partials[(size_t)tid * nblocks_total + bid] = s;
     }
+    // a block that goes on to another tile must not overwrite sh before
+    // these reads are done
+    __syncthreads();
 }
 
 // Sum nv rows of nblocks partials into result[0..nv); one block of 1024.

@@ -268,8 +268,11 @@ int wpf_capture_get(wpf_ctx* ctx, float* fields, float* target);
 #define WPF_KERNEL_CG_UPDATE   7  /* fine-level x/r update                  */
 int wpf_profile_start(wpf_ctx* ctx, int32_t kernel);
 int wpf_profile_stop(wpf_ctx* ctx, double* total_ms, int32_t* count);
-/* Planes per tile of the context's vector solver kernels (normally chosen
- * by timing on the first solve); a value > 0 fixes it. */
+/* Planes per tile of the context's vector solver kernels: > 0 fixes it,
+ * 0 restores the default (8; 4 below 64 planes), -1 asks for the autotuner
+ * (the fastest of 2..32 planes, timed on the next solve).  Reductions are
+ * partitioned by tile, so the depth changes results in the last bits; the
+ * default keeps them independent of timing. */
 int wpf_set_tile_depth(wpf_ctx* ctx, int32_t planes);
 
 /* ---- ensembles -------------------------------------------------------------

@@ -774,7 +774,7 @@ extern "C" {
 
 int wpf_set_tile_depth(wpf_ctx* c, int32_t planes)
 {
-    WPF_CHECK(c && planes > 0 && planes <= 256, WPF_ERR_ARG, "bad tile depth");
+    WPF_CHECK(c && planes >= -1 && planes <= 256, WPF_ERR_ARG, "bad tile depth");
     c->tuned_zc[0] = c->tuned_zc[1] = planes;
     return WPF_OK;
 }

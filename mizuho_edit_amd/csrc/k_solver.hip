@@ -265,12 +265,19 @@ template <typename R, typename C, bool DOT, bool PROX, bool PROO>
 static int sweep(wpf_ctx* c, const Fine<R, C>& F, double a, double w, const R* x, const R* xo,
                  const Pro<R>& P, R* y, int slot);
 
-// Autotuner of the vector kernels' tile depth: times one smoothing sweep
-// over the grid for each candidate number of planes per tile and keeps the
-// fastest (once per context and precision; *zc caches the choice).
+// Tile depth of the vector kernels.  *zc > 0: fixed; 0: the default, 8
+// planes (4 below 64 planes), which keeps results independent of timing;
+// < 0: autotune, i.e. time one smoothing sweep over the grid for each
+// candidate number of planes per tile and keep the fastest.  The choice is
+// cached per context and precision in *zc.  The partition of the
+// reductions follows the tile shape, so a different depth changes results
+// in the last bits.
+static int default_zc(const Grid& g) { return g.nz >= 64 ? 8 : 4; }
+
 template <typename R, typename C>
 static int tune_shape(wpf_ctx* c, Fine<R, C>& F, int* zc)
 {
+    if (*zc == 0) *zc = default_zc(F.g);
     if (*zc > 0) {
         set_shape(F, *zc);
         return WPF_OK;
@@ -580,7 +587,7 @@ static Fine<double, double> fine64(wpf_ctx* c)
 {
     Fine<double, double> F;
     F.g = c->g;
-    set_shape(F, c->tuned_zc[1] > 0 ? c->tuned_zc[1] : 8);
+    set_shape(F, c->tuned_zc[1] > 0 ? c->tuned_zc[1] : default_zc(F.g));
     F.tx = c->tx.p; F.ty = c->ty.p; F.tz = c->tz.p; F.invD = c->invD.p;
     F.r = c->r.p; F.Z = c->z.p; F.T = c->z2.p; F.pa = c->pp.p; F.pb = c->pp2.p; F.q = c->q.p;
     F.r2 = c->rb.p;

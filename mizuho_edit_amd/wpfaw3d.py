@@ -209,7 +209,8 @@ class WPFAW3D:
         self._net = net
 
     def set_tile_depth(self, planes: int):
-        """Fix the planes per tile of the vector solver kernels (else tuned)."""
+        """Planes per tile of the vector solver kernels: > 0 fixed, 0 the
+        default, -1 autotuned on the next solve."""
         check(_lib.lib().wpf_set_tile_depth(self._ctx, planes), "set_tile_depth")
 
     # -- kernel timing ------------------------------------------------------

@@ -15,9 +15,11 @@ Two paths run from the same initial state:
 and the fast path's relative pressure error against the plain path after
 the timed steps are reported beside it.
 
-roofline: the dominant kernel of the fast path, timed with HIP events on the
-library's stream over the timed region; achieved = algorithmic bytes per
-launch / average launch time.
+roofline: the dominant kernel of the fast path (by default the CG update
+fused with the first pre-smoothing step, k_vupsweep: the largest share of a
+step's kernel time), timed with HIP events on the library's stream over the
+timed region; achieved = algorithmic bytes per launch / average launch
+time; traffic = HBM bytes per launch measured with rocprofv3 counters.
 cpu_baseline: the FP64 oracle (C restatement, OpenMP) timed on this host for
 one step of the same workload from the state the GPU timed region starts
 from, CG + Jacobi with rtol 1e-10 and maxit 1000 (the reference's cap).
@@ -41,27 +43,52 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per cell (each array element the math must read or
 # write once per launch; mask = 1 byte); r = vector precision, h = stored
 # face precision of the fine operator
-def smoother_bytes(r: int, h: int, cheb: bool = True) -> float:
-    """Mean over the three sweeps of one V-cycle on the fine level (degree-2
-    Chebyshev, coarse correction fused into the post-smoothing):
-      pre-smoothing step 2  x, b, invD, 3 faces -> y          4r + 3h
-      post step 1           + mask + coarse correction        4r + 3h + 1 + r/8
-      post step 2           + mask + coarse correction + x0   5r + 3h + 1 + r/8"""
-    base = 4 * r + 3 * h
-    last = base + r + 1 + r / 8 if cheb else base  # Jacobi's step 2 needs no x0
-    return (base + (base + 1 + r / 8) + last) / 3
+def smoother_bytes(r: int, h: int, cheb: bool, launches: int, its: int) -> float:
+    """Mean over the fine-level smoothing launches of the timed steps.  With
+    two sweeps and the first pre-smoothing step fused into the CG update,
+    each V-cycle launches
+      post step 1   x, b, invD, 3 faces, mask, coarse correction -> y   4r + 3h + 1 + r/8
+      post step 2   + x0 (Chebyshev; Jacobi needs none)                 5r + 3h + 1 + r/8
+    and the first V-cycle of every inner solve adds a pre-smoothing step
+    x, b, invD, 3 faces -> y (4r + 3h).  Inner solves = launches - 2 its."""
+    post1 = 4 * r + 3 * h + 1 + r / 8
+    post2 = post1 + (r if cheb else 0)
+    pre = max(0, launches - 2 * its)
+    return (its * (post1 + post2) + pre * (4 * r + 3 * h)) / max(1, launches)
 
 
-def bytes_per_cell(kernel: str, r: int, h: int, cheb: bool = True) -> float:
+def bytes_per_cell(kernel: str, r: int, h: int, cheb: bool = True, launches: int = 1,
+                   its: int = 1) -> float:
     return {
-        "smooth": smoother_bytes(r, h, cheb),
+        "smooth": smoother_bytes(r, h, cheb, launches, its),
         "matvec": 4 * r + 3 * h + 1,  # z, p_old, 3 faces, mask -> p_new, q
-        "cg_update": 8 * r,           # x, p, r, q, invD -> x, r, z
+        # CG update fused with the first pre-smoothing step:
+        # e, p, r, q, invD, 3 faces -> e, r', y
+        "cg_update": 8 * r + 3 * h,
         "restrict": 2 * r + 3 * h + 1 + r / 8,
-        "true_resid": 5 * 8 + 1 + r,  # x, t x3, invD (FP64), mask -> r (precision r)
+        "true_resid": 6 * 8 + 1 + r,  # x, f, t x3, invD (FP64), mask -> r (precision r)
         "step_fields": 6 * 8 + 1 + 2 * 8,  # phi, c, p, t x3, mask -> phi', c'
         "perm": 2 * 8 + 1 + 4 * 8,  # phi, krock, mask -> tx, ty, tz, invD
     }[kernel]
+
+
+# HBM traffic per launch of the roofline kernel, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of this benchmark, each scaled by what the
+# counter reads for a 1 GiB copy (the command and the per-kernel table are
+# in the same folder)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256.json")
+
+
+def measured_traffic(kernel: str, grid: int):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if t.get("grid") != grid or not k:
+        return None
+    return k["bytes_per_launch"]
 
 
 def log(msg):
@@ -144,6 +171,20 @@ def calibrate_copy(reps=20):
           flush=True)
 
 
+def calibrate_streams(n=1 << 24, reps=20):
+    """The library's streaming probe (wpf_stream_probe): one JSON line per
+    (arrays read, arrays written) pair, n floats per array."""
+    import ctypes
+    from mizuho_edit_amd import _lib
+    L = _lib.lib()
+    for nr, nw in ((1, 1), (2, 1), (4, 1), (5, 2), (6, 3), (7, 1), (8, 1), (8, 3), (10, 1), (0, 3)):
+        us = ctypes.c_double()
+        _lib.check(L.wpf_stream_probe(0, nr, nw, n, reps, ctypes.byref(us)), "wpf_stream_probe")
+        nbytes = 4.0 * n * (nr + nw)
+        print(json.dumps({"read_arrays": nr, "written_arrays": nw, "mib_per_array": 4 * n >> 20,
+                          "us": us.value, "GBps": nbytes / (us.value * 1e-6) / 1e9}), flush=True)
+
+
 def bench_ensemble(a):
     """The ensemble workload (48 uneven grids, 40,894,464 cells): the same
     steps once member by member with wpf_step (per-instance launches, one
@@ -217,7 +258,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--kernel", default="smooth", help="kernel reported in roofline")
+    ap.add_argument("--kernel", default="cg_update", help="kernel reported in roofline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--weights", default=None,
@@ -233,12 +274,17 @@ def main():
                     help="fast path with FP32 phase/concentration stencils (FP64 fields and "
                          "update); slower at 256^3 since the less accurate fields cost "
                          "solver iterations")
+    ap.add_argument("--calibrate-streams", action="store_true",
+                    help="streaming bandwidth of kernels that read k and write m arrays of 64 MiB")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 1 GiB copy kernel and exit")
     ap.add_argument("--ensemble", action="store_true",
                     help="the 48-grid ensemble workload instead of the 256^3 grid")
     ap.add_argument("--ens-shrink", type=int, default=1, help="divide ensemble grid sizes by this")
     a = ap.parse_args()
+    if a.calibrate_streams:
+        calibrate_streams()
+        return
     if a.calibrate_copy:
         calibrate_copy()
         return
@@ -299,7 +345,7 @@ def main():
     sps64 = a.steps / wall64
     total_ms, count = prof
     avg_s = total_ms / 1e3 / max(count, 1)
-    bpc = bytes_per_cell(a.kernel, 4, 4 if a.jacobi else 2, not a.jacobi)
+    bpc = bytes_per_cell(a.kernel, 4, 4 if a.jacobi else 2, not a.jacobi, count, itsf)
     achieved = bpc * ncell / avg_s / 1e9 if count else 0.0
 
     cpu = None
@@ -343,7 +389,9 @@ def main():
         "learned_stage": ({"weights": os.path.relpath(weights, ROOT), "accepted_steps": accepted}
                           if net else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(a.kernel, n) if a.kernel == "cg_update" and not a.jacobi
+                     else None,
                      "kernel": a.kernel, "launches": count, "avg_us": avg_s * 1e6,
                      "bytes_per_cell": bpc},
         "cpu_baseline": cpu,

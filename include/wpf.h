@@ -260,14 +260,23 @@ int wpf_capture_get(wpf_ctx* ctx, float* fields, float* target);
  * kernel time and *count the number of launches timed. */
 #define WPF_KERNEL_PERM        0  /* k_perm: permeability + operator        */
 #define WPF_KERNEL_STEP_FIELDS 1  /* k_step_fields: fused stencil update    */
-#define WPF_KERNEL_SMOOTH      2  /* fine-level damped-Jacobi sweep         */
+#define WPF_KERNEL_SMOOTH      2  /* fine-level smoothing steps (post, and
+                                     the first pre-step of each solve)      */
 #define WPF_KERNEL_MATVEC      3  /* fine-level matvec + dot                */
 #define WPF_KERNEL_TRUE_RESID  4  /* FP64 true residual                     */
 #define WPF_KERNEL_RESTRICT    5  /* fine-level residual + restriction      */
 #define WPF_KERNEL_PROLONG     6  /* (fused into the smoothing: no launches) */
-#define WPF_KERNEL_CG_UPDATE   7  /* fine-level x/r update                  */
+#define WPF_KERNEL_CG_UPDATE   7  /* fine-level CG update (fused with the
+                                     first pre-smoothing step)              */
 int wpf_profile_start(wpf_ctx* ctx, int32_t kernel);
 int wpf_profile_stop(wpf_ctx* ctx, double* total_ms, int32_t* count);
+/* Streaming bandwidth of the device: a kernel that reads `nread` (0..10)
+ * and writes `nwrite` (1..3) arrays of n floats (n a multiple of 4), one
+ * 16-byte element per thread, timed over `reps` launches after 3 untimed
+ * ones.  *us is the average launch time.  A measurement utility with no
+ * counterpart in the reference: it gives the rate a multi-array kernel can
+ * reach, beside the 8 TB/s data-sheet peak. */
+int wpf_stream_probe(int32_t device, int32_t nread, int32_t nwrite, int64_t n, int32_t reps, double* us);
 /* Planes per tile of the context's vector solver kernels: > 0 fixes it,
  * 0 restores the default (8; 4 below 64 planes), -1 asks for the autotuner
  * (the fastest of 2..32 planes, timed on the next solve).  Reductions are

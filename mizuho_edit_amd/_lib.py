@@ -123,6 +123,7 @@ def lib():
         "wpf_set_breakthrough_plane": ([P, I32], I32),
         "wpf_profile_start": ([P, I32], I32),
         "wpf_profile_stop": ([P, D, ctypes.POINTER(I32)], I32),
+        "wpf_stream_probe": ([I32, I32, I32, ctypes.c_int64, I32, D], I32),
         "wpf_net_num_params": ([], ctypes.c_int64),
         "wpf_net_create": ([I32, ctypes.POINTER(P)], I32),
         "wpf_net_destroy": ([P], I32),

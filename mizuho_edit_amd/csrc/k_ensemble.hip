@@ -789,6 +789,8 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
     solver_defaults(opts, &o);
     WPF_CHECK(o.solver == WPF_SOLVER_MIXED, WPF_ERR_ARG, "ensembles run the mixed solver");
     WPF_CHECK(!(o.flags & WPF_FLAG_PROJECT), WPF_ERR_ARG, "the projected initial guess is single-grid only");
+    WPF_CHECK(!(o.flags & (WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG | WPF_FLAG_CAPTURE | WPF_FLAG_F32_STENCIL)),
+              WPF_ERR_ARG, "ensembles step without the learned stage, sample capture or FP32 stencils");
     WPF_CHECK(!(o.flags & WPF_FLAG_CHEBYSHEV) || o.mg_sweeps == 2, WPF_ERR_ARG,
               "Chebyshev smoothing is the degree-2 polynomial: mg_sweeps must be 2");
     const int dev = members[0]->dev;

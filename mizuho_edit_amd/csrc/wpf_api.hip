@@ -58,6 +58,55 @@ static int dkey_host(double v, unsigned long long* k)
     return 0;
 }
 
+// streaming-bandwidth probe (wpf_stream_probe)
+struct StreamPtrs {
+    const float4* in[10];
+    float4* out[3];
+};
+template <int NR, int NW>
+__global__ void __launch_bounds__(256) k_stream_probe(StreamPtrs P, int64_t n4)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const float4 v = P.in[r][i];
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+#pragma unroll
+    for (int w = 0; w < NW; w++) P.out[w][i] = make_float4(s.x + (float)w, s.y, s.z, s.w);
+}
+template <int NR>
+static const void* probe_nw(int nw)
+{
+    switch (nw) {
+    case 1: return (const void*)k_stream_probe<NR, 1>;
+    case 2: return (const void*)k_stream_probe<NR, 2>;
+    case 3: return (const void*)k_stream_probe<NR, 3>;
+    }
+    return nullptr;
+}
+static const void* probe_kernel(int nr, int nw)
+{
+    switch (nr) {
+    case 0: return probe_nw<0>(nw);
+    case 1: return probe_nw<1>(nw);
+    case 2: return probe_nw<2>(nw);
+    case 3: return probe_nw<3>(nw);
+    case 4: return probe_nw<4>(nw);
+    case 5: return probe_nw<5>(nw);
+    case 6: return probe_nw<6>(nw);
+    case 7: return probe_nw<7>(nw);
+    case 8: return probe_nw<8>(nw);
+    case 9: return probe_nw<9>(nw);
+    case 10: return probe_nw<10>(nw);
+    }
+    return nullptr;
+}
 }  // namespace wpf
 
 using namespace wpf;
@@ -518,6 +567,66 @@ int wpf_profile_stop(wpf_ctx* c, double* total_ms, int32_t* count)
     c->prof_target = -1;
     c->prof_used = 0;
     return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// streaming-bandwidth probe
+// ---------------------------------------------------------------------------
+
+int wpf_stream_probe(int32_t device, int32_t nread, int32_t nwrite, int64_t n, int32_t reps, double* us)
+{
+    WPF_CHECK(us && nread >= 0 && nread <= 10 && nwrite >= 1 && nwrite <= 3 && n > 0 && n % 4 == 0 &&
+                  reps > 0,
+              WPF_ERR_ARG, "bad probe arguments");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device");
+        return WPF_ERR_NODEV;
+    }
+    WPF_CHECK(device >= 0 && device < ndev, WPF_ERR_ARG, "no device %d", device);
+    WPF_HIP(hipSetDevice(device));
+    StreamPtrs P{};
+    std::vector<void*> bufs;
+    int rc = WPF_OK;
+    for (int a = 0; a < nread + nwrite && rc == WPF_OK; a++) {
+        void* b = nullptr;
+        if (hipMalloc(&b, sizeof(float) * (size_t)n) != hipSuccess ||
+            hipMemset(b, 0, sizeof(float) * (size_t)n) != hipSuccess) {
+            if (b) (void)hipFree(b);
+            set_error("probe allocation failed");
+            rc = WPF_ERR_HIP;
+            break;
+        }
+        bufs.push_back(b);
+        if (a < nread)
+            P.in[a] = (const float4*)b;
+        else
+            P.out[a - nread] = (float4*)b;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == WPF_OK) {
+        int64_t n4 = n / 4;
+        const void* fn = probe_kernel(nread, nwrite);
+        void* args[] = {&P, &n4};
+        const dim3 grid((unsigned)((n4 + 255) / 256)), block(256);
+        float ms = 0.0f;
+        bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+        for (int r = 0; ok && r < 3; r++) ok = hipLaunchKernel(fn, grid, block, args, 0, nullptr) == hipSuccess;
+        ok = ok && hipEventRecord(e0, nullptr) == hipSuccess;
+        for (int r = 0; ok && r < reps; r++) ok = hipLaunchKernel(fn, grid, block, args, 0, nullptr) == hipSuccess;
+        ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (ok)
+            *us = 1e3 * ms / reps;
+        else {
+            set_error("probe launch failed");
+            rc = WPF_ERR_HIP;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (void* b : bufs) (void)hipFree(b);
+    return rc;
 }
 
 int wpf_capture_get(wpf_ctx* c, float* fields, float* target)

@@ -39,8 +39,8 @@ struct wpf_ensemble {
     wpf::EnsState* d_state = nullptr;        // device states
     wpf::EnsState* h_state = nullptr;        // pinned host copy
     // tile tables: 1-D fine, vector, restriction, cell kernels, coarse levels
-    wpf::EnsTile *t1 = nullptr, *tv = nullptr, *tz = nullptr, *t3 = nullptr;
-    int n1 = 0, nv = 0, nz = 0, n3 = 0;
+    wpf::EnsTile *t1 = nullptr, *tv = nullptr, *tz = nullptr, *t3 = nullptr, *ts = nullptr;
+    int n1 = 0, nv = 0, nz = 0, n3 = 0, ns = 0;
     wpf::EnsTile* tl[wpf::kEnsMaxLevels] = {};
     int nl[wpf::kEnsMaxLevels] = {};
     size_t coarsest_lds = 0;
@@ -485,7 +485,7 @@ int build_tables(wpf_ensemble* e)
 {
     const int nm = (int)e->mem.size();
     std::vector<EnsMember> M(nm);
-    std::vector<int> c1(nm), cv(nm), cz(nm), c3(nm);
+    std::vector<int> c1(nm), cv(nm), cz(nm), c3(nm), cs(nm);
     std::vector<std::vector<int>> cl(e->nlev, std::vector<int>(nm));
     for (int m = 0; m < nm; m++) {
         wpf_ctx* c = e->mem[m];
@@ -498,6 +498,10 @@ int build_tables(wpf_ensemble* e)
         X.nbv = vz_blocks(X.Z);
         X.nbz = zg_blocks(X.G);
         X.nb3 = member_blocks3(c, &X.gx3, &X.gy3);
+        {
+            int gz;
+            X.nbs = step_tiles(c->g.nx, c->g.ny, c->g.nz, &X.gxs, &X.gys, &gz);
+        }
         for (int b = 0; b < 2; b++) {
             X.phi[b] = c->phi[b];
             X.conc[b] = c->conc[b];
@@ -532,13 +536,14 @@ int build_tables(wpf_ensemble* e)
         X.dt_pref = p.tfac * 0.5 * (p.dx * p.dx) / (p.epsilon * p.epsilon);
         X.eps2_D = (p.epsilon * p.epsilon) / p.diffusion;
         X.K = phase_consts(p);
-        c1[m] = X.nb1; cv[m] = X.nbv; cz[m] = X.nbz; c3[m] = X.nb3;
+        c1[m] = X.nb1; cv[m] = X.nbv; cz[m] = X.nbz; c3[m] = X.nb3; cs[m] = X.nbs;
     }
     if (!e->d_mem) WPF_HIP(hipMalloc((void**)&e->d_mem, sizeof(EnsMember) * nm));
     WPF_HIP(hipMemcpy(e->d_mem, M.data(), sizeof(EnsMember) * nm, hipMemcpyHostToDevice));
     int rc;
     if ((rc = build_tiles(c1, &e->t1, &e->n1)) || (rc = build_tiles(cv, &e->tv, &e->nv)) ||
-        (rc = build_tiles(cz, &e->tz, &e->nz)) || (rc = build_tiles(c3, &e->t3, &e->n3)))
+        (rc = build_tiles(cz, &e->tz, &e->nz)) || (rc = build_tiles(c3, &e->t3, &e->n3)) ||
+        (rc = build_tiles(cs, &e->ts, &e->ns)))
         return rc;
     for (int l = 1; l < e->nlev; l++)
         if ((rc = build_tiles(cl[l], &e->tl[l], &e->nl[l]))) return rc;
@@ -731,7 +736,7 @@ int sched_solve(wpf_ensemble* e)
 int sched_fields(wpf_ensemble* e)
 {
     const int nm = (int)e->mem.size();
-    ECHK(ens_step_fields(e->stream, e->d_mem, nm, e->t3, e->n3, e->grid));
+    ECHK(ens_step_fields(e->stream, e->d_mem, nm, e->ts, e->ns, e->grid));
     ELAUNCH(ke_end_step, (nm + 63) / 64, 64, 0, e->d_mem, nm, e->extrap ? 1 : 0);
     return WPF_OK;
 }
@@ -1045,7 +1050,7 @@ int wpf_ensemble_destroy(wpf_ensemble* e)
     auto F = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    F(e->d_mem); F(e->d_state); F(e->t1); F(e->tv); F(e->tz); F(e->t3);
+    F(e->d_mem); F(e->d_state); F(e->t1); F(e->tv); F(e->tz); F(e->t3); F(e->ts);
     for (int l = 0; l < kEnsMaxLevels; l++) F(e->tl[l]);
     if (e->h_state) (void)hipHostFree(e->h_state);
     if (e->stream) (void)hipStreamDestroy(e->stream);

@@ -387,10 +387,27 @@ struct StepScalars {
     int key_prev, key_cur, key_next;
 };
 
+// value of the lane below (lane 0 gets its own back)
+template <typename T>
+__device__ __forceinline__ T from_left_lane(T v)
+{
+    return __shfl_up(v, 1, 64);
+}
+
 // T is the precision of the phase-field and concentration stencils.  T =
 // double is the reference arithmetic; T = float evaluates the stencils in
 // FP32 (face velocities from the FP64 pressure difference, rounded once)
 // while phi, c, the Euler update and taumin stay FP64.
+//
+// Tiling: a block of 64 x 4 threads owns a 64 x 4 column of cells and walks
+// kStepZ planes up z.  phi, c and p of each plane are staged once in LDS with
+// a one-cell xy border (66 x 6 values, clamped at the grid edge exactly as
+// updateHalo fills the ghosts), in a ring of four plane slots: the 19-point
+// phi stencil and the 7-point c and p stencils of plane k read planes k-1,
+// k, k+1 from LDS, so each value is fetched from memory once per tile
+// instead of up to 19 times.  Plane k+1 is loaded while plane k-2's slot is
+// free, which needs one barrier per plane.  The face coefficients and the
+// mask are read straight from memory (one coalesced row each).
 template <typename T>
 __device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int bz, const PhaseConstsT<T>& K,
                                               double dx, const StepScalars& S,
@@ -404,47 +421,139 @@ __device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int
                                               double* __restrict__ scal, int32_t* __restrict__ bt_step,
                                               double* __restrict__ mins, int bid)
 {
+    constexpr int LX = kStepX + 2, LY = kStepY + 2, LP = LX * LY;
+    __shared__ double sphi[4][LP], scon[4][LP], spr[4][LP];
     double dt = S.dt_cfg;
     if (S.device_dt && S.itime >= 3) {
         const double taumin_prev = dkey_inv(keys[S.key_prev]);
         dt = S.dt_pref * fmin(S.eps2_D, taumin_prev);
     }
-    const int i = bx * blockDim.x + threadIdx.x;
-    const int j = by * blockDim.y + threadIdx.y;
-    const int k = bz * blockDim.z + threadIdx.z;
-    T tau = (T)1.0e6;
-    if (i < d.nx && j < d.ny && k < d.nz) {
-        const Nb n = neighbours(d, i, j, k);
-        const double p0 = p[n.c];
-        // face velocities of the six faces (the -face of the first cell of an
-        // axis is the zero-gradient boundary face: -(t*0) = -0)
-        const T vxp = (T)face_v(tx[n.c], p[n.xp], p0, dx);
-        const T vyp = (T)face_v(ty[n.c], p[n.yp], p0, dx);
-        const T vzp = (T)face_v(tz[n.c], p[n.zp], p0, dx);
-        const T vxm = (T)((i > 0) ? face_v(tx[n.xm], p0, p[n.xm], dx) : -0.0);
-        const T vym = (T)((j > 0) ? face_v(ty[n.ym], p0, p[n.ym], dx) : -0.0);
-        const T vzm = (T)((k > 0) ? face_v(tz[n.zm], p0, p[n.zm], dx) : -0.0);
-        double s64[19];
-        gather19(d, phi, i, j, k, n, s64);
-        T s[19];
+    const int tid = threadIdx.x + kStepX * threadIdx.y;
+    const int i0 = bx * kStepX, j0 = by * kStepY, k0 = bz * kStepZ;
+    const int k1 = min(k0 + kStepZ, d.nz);
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool in = i < d.nx && j < d.ny;
+    const int64_t nx = d.nx, pl = d.plane;
+    // this thread's share of a plane's staged values (elements tid, tid + 256)
+    // and their in-plane offsets (clamped to the grid)
+    constexpr int NT = kStepX * kStepY, NE = (LP + NT - 1) / NT;
+    double rphi[NE], rcon[NE], rpr[NE];
+    int64_t goff[NE];
 #pragma unroll
-        for (int q = 0; q < 19; q++) s[q] = (T)s64[q];
-        const bool well = (mask[n.c] & WPF_MASK_WELL) != 0;
-        const double c0 = conc[n.c];
-        const T dph = phase_cell<T>(K, s, (T)c0, vxp, vxm, vyp, vym, vzp, vzm, well, &tau);
-        T dcd = (T)0;
-        if (!well) {
-            T cc[7] = {(T)c0, (T)conc[n.xp], (T)conc[n.xm], (T)conc[n.yp], (T)conc[n.ym], (T)conc[n.zp],
-                       (T)conc[n.zm]};
-            dcd = concent_cell<T>(K, cc, dph, vxp, vxm, vyp, vym, vzp, vzm);
+    for (int q = 0; q < NE; q++) {
+        const int e = tid + q * NT;
+        const int gi = min(max(i0 + e % LX - 1, 0), d.nx - 1);
+        const int gj = min(max(j0 + e / LX - 1, 0), d.ny - 1);
+        goff[q] = gi + nx * gj;
+    }
+    // plane kk (clamped to the grid) into registers
+    auto fetch = [&](int kk) {
+        const int64_t po = pl * min(max(kk, 0), d.nz - 1);
+#pragma unroll
+        for (int q = 0; q < NE; q++) {
+            if (tid + q * NT < LP) {
+                const int64_t g = goff[q] + po;
+                rphi[q] = phi[g];
+                rcon[q] = conc[g];
+                rpr[q] = p[g];
+            }
         }
-        const double phn = s64[0] + (double)dph * dt;
-        phi_out[n.c] = phn;
-        conc_out[n.c] = c0 + (double)dcd * dt;
-        if (i == S.bt_plane && !well && phn > 0.0) atomicMin(bt_step, S.itime);
+    };
+    // registers into ring slot kk & 3
+    auto stage = [&](int kk) {
+        const int slot = kk & 3;
+#pragma unroll
+        for (int q = 0; q < NE; q++) {
+            const int e = tid + q * NT;
+            if (e < LP) {
+                sphi[slot][e] = rphi[q];
+                scon[slot][e] = rcon[q];
+                spr[slot][e] = rpr[q];
+            }
+        }
+    };
+    // the cell's own face coefficients and mask of plane kk
+    struct CellIn {
+        double tx, txm, ty, tym, tz;
+        uint8_t m;
+    };
+    auto cell_in = [&](int kk, CellIn& o) {
+        const int64_t c = i + nx * j + pl * kk;
+        o.tx = tx[c];
+        o.txm = threadIdx.x == 0 ? tx[c - 1] : 0.0;  // lane 0 forms its -x face itself
+        o.ty = ty[c];
+        o.tym = ty[c - nx];
+        o.tz = tz[c];
+        o.m = mask[c];
+    };
+    fetch(k0 - 1);
+    stage(k0 - 1);
+    fetch(k0);
+    stage(k0);
+    fetch(k0 + 1);
+    CellIn cur{}, nxt{};
+    double tzm = 0.0;
+    if (in) {
+        cell_in(k0, cur);
+        if (k0 > 0) tzm = tz[i + nx * j + pl * (k0 - 1)];
+    }
+    const int e0 = (threadIdx.y + 1) * LX + threadIdx.x + 1;
+    T tau = (T)1.0e6;
+    T vz_prev = (T)0;  // +z face velocity of the plane below (= this plane's -z face)
+    for (int k = k0; k < k1; k++) {
+        stage(k + 1);
+        __syncthreads();
+        // loads for the next plane, in flight while this one is computed
+        if (k + 2 <= k1) fetch(k + 2);
+        if (in && k + 1 < k1) cell_in(k + 1, nxt);
+        const int s0 = k & 3, sp = (k + 1) & 3, sm = (k - 1) & 3;
+        const double p0 = spr[s0][e0];
+        // face velocities: each wave holds one row, so the -x face of a cell
+        // is the +x face of the lane below (the same operands, so the same
+        // value); the -z face is the +z face of the plane below.  The -face
+        // of the first cell of an axis is the zero-gradient boundary face:
+        // -(t*0) = -0.
+        const T vxp = (T)face_v(cur.tx, spr[s0][e0 + 1], p0, dx);
+        const T vyp = (T)face_v(cur.ty, spr[s0][e0 + LX], p0, dx);
+        const T vzp = (T)face_v(cur.tz, spr[sp][e0], p0, dx);
+        T vxm = from_left_lane(vxp);
+        if (threadIdx.x == 0) vxm = (T)((i > 0) ? face_v(cur.txm, p0, spr[s0][e0 - 1], dx) : -0.0);
+        const T vym = (T)((j > 0) ? face_v(cur.tym, p0, spr[s0][e0 - LX], dx) : -0.0);
+        const T vzm = (k == k0) ? (T)((k > 0) ? face_v(tzm, p0, spr[sm][e0], dx) : -0.0) : vz_prev;
+        vz_prev = vzp;
+        if (in) {
+            const int64_t c = i + nx * j + pl * k;
+            // the 19 points in the order of the per-procedure kernels' gather
+            const double s64[19] = {
+                sphi[s0][e0],
+                sphi[s0][e0 + 1], sphi[s0][e0 - 1], sphi[s0][e0 + LX], sphi[s0][e0 - LX],
+                sphi[sp][e0], sphi[sm][e0],
+                sphi[s0][e0 + LX + 1], sphi[s0][e0 - LX + 1], sphi[s0][e0 + LX - 1], sphi[s0][e0 - LX - 1],
+                sphi[sp][e0 + LX], sphi[sp][e0 - LX], sphi[sm][e0 + LX], sphi[sm][e0 - LX],
+                sphi[sp][e0 + 1], sphi[sp][e0 - 1], sphi[sm][e0 + 1], sphi[sm][e0 - 1]};
+            T st[19];
+#pragma unroll
+            for (int q = 0; q < 19; q++) st[q] = (T)s64[q];
+            const bool well = (cur.m & WPF_MASK_WELL) != 0;
+            const double c0 = scon[s0][e0];
+            T tc = (T)1.0e6;
+            const T dph = phase_cell<T>(K, st, (T)c0, vxp, vxm, vyp, vym, vzp, vzm, well, &tc);
+            tau = fmin(tau, tc);
+            T dcd = (T)0;
+            if (!well) {
+                T cc[7] = {(T)c0, (T)scon[s0][e0 + 1], (T)scon[s0][e0 - 1], (T)scon[s0][e0 + LX],
+                           (T)scon[s0][e0 - LX], (T)scon[sp][e0], (T)scon[sm][e0]};
+                dcd = concent_cell<T>(K, cc, dph, vxp, vxm, vyp, vym, vzp, vzm);
+            }
+            const double phn = s64[0] + (double)dph * dt;
+            phi_out[c] = phn;
+            conc_out[c] = c0 + (double)dcd * dt;
+            if (i == S.bt_plane && !well && phn > 0.0) atomicMin(bt_step, S.itime);
+        }
+        cur = nxt;
     }
     block_min_store((double)tau, mins, bid);
-    if (bx == 0 && by == 0 && bz == 0 && threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
+    if (bx == 0 && by == 0 && bz == 0 && tid == 0) {
         // key_next was last read two steps ago: reset it for the next step
         keys[S.key_next] = dkey(1.0e6);
         scal[0] = scal[0] + dt;  // time
@@ -452,8 +561,11 @@ __device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int
     }
 }
 
+// occupancy: the FP64 stencil needs up to 168 VGPRs for 3 waves per SIMD;
+// the FP32 one fits 4 waves (values measured at 256^3: 361 and 298 us)
 template <typename T>
-__global__ void __launch_bounds__(256) k_step_fields(Dims d, PhaseConstsT<T> K, double dx, StepScalars S,
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4)
+    k_step_fields(Dims d, PhaseConstsT<T> K, double dx, StepScalars S,
                                                      const double* __restrict__ phi,
                                                      const double* __restrict__ conc,
                                                      const double* __restrict__ p,
@@ -488,10 +600,11 @@ int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt,
     S.key_prev = (itime + 2) % 3;
     S.key_cur = itime % 3;
     S.key_next = (itime + 1) % 3;
-    dim3 blk(64, 4, 1);
+    const dim3 blk(kStepX, kStepY, 1);
+    int gx, gy, gz;
+    const int nb = step_tiles(d.nx, d.ny, d.nz, &gx, &gy, &gz);
+    const dim3 gr(gx, gy, gz);
     prof_mark(c, WPF_KERNEL_STEP_FIELDS, true);
-    const dim3 gr = grid3(d, blk);
-    const int nb = (int)(gr.x * gr.y * gr.z);
     // the block minima use the reduction scratch as one flat array
     WPF_CHECK((int64_t)nb <= (int64_t)c->red.max_blocks * c->red.slots, WPF_ERR_ARG,
               "grid too large for the reduction scratch");
@@ -665,8 +778,7 @@ __global__ void __launch_bounds__(256) ke_step_fields(const EnsMember* __restric
         const EnsTile T = tiles[t];
         if (T.b < 0) continue;
         const EnsMember& M = Ms[T.m];
-        int bx, by, bz;
-        ens_block3(M, T.b, bx, by, bz);
+        const int bx = T.b % M.gxs, by = (T.b / M.gxs) % M.gys, bz = T.b / (M.gxs * M.gys);
         const Dims d{M.nx, M.ny, M.nz, M.plane};
         const int itime = M.st->itime, cur = M.st->cur;
         StepScalars S;
@@ -689,7 +801,7 @@ __global__ void __launch_bounds__(256) ke_step_fields(const EnsMember* __restric
 __global__ void __launch_bounds__(1024) ke_min_key(const EnsMember* __restrict__ Ms)
 {
     const EnsMember& M = Ms[blockIdx.x];
-    t_min_key(M.partials, M.nb3, M.tau_key + M.st->itime % 3);
+    t_min_key(M.partials, M.nbs, M.tau_key + M.st->itime % 3);
 }
 
 int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid)
@@ -702,7 +814,7 @@ int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntile
 int ens_step_fields(hipStream_t s, const EnsMember* Ms, int nmembers, const EnsTile* tiles, int ntiles,
                     int grid)
 {
-    hipLaunchKernelGGL(ke_step_fields, dim3(grid), dim3(64, 4, 1), 0, s, Ms, tiles, ntiles);
+    hipLaunchKernelGGL(ke_step_fields, dim3(grid), dim3(kStepX, kStepY, 1), 0, s, Ms, tiles, ntiles);
     WPF_HIP(hipGetLastError());
     hipLaunchKernelGGL(ke_min_key, dim3(nmembers), dim3(1024), 0, s, Ms);
     WPF_HIP(hipGetLastError());

@@ -24,6 +24,10 @@ namespace wpf {
 template <typename T>
 struct PhaseConstsT {
     T dx, two_dx, dx2, four_dx2;
+    // reciprocals of the grid constants: the stencils multiply by them where
+    // the reference divides (within one rounding of the quotient; an FP64
+    // division costs about ten instructions)
+    T inv_dx, inv_two_dx, inv_dx2, inv_four_dx2;
     T w0, maxdphi, eps, lambda, diffusion, ral, rat, tau_num, alpha;
     T kwh, inv_mu_unused;
 };
@@ -38,6 +42,10 @@ inline PhaseConsts phase_consts(const wpf_params& p)
     K.two_dx = 2.0 * p.dx;
     K.dx2 = p.dx * p.dx;
     K.four_dx2 = 4.0 * (p.dx * p.dx);
+    K.inv_dx = 1.0 / K.dx;
+    K.inv_two_dx = 1.0 / K.two_dx;
+    K.inv_dx2 = 1.0 / K.dx2;
+    K.inv_four_dx2 = 1.0 / K.four_dx2;
     K.w0 = 2.0 * p.dx;
     K.maxdphi = 2.0 / (2.0 * p.dx);
     K.eps = p.epsilon;
@@ -58,6 +66,8 @@ inline PhaseConstsT<T> consts_as(const PhaseConsts& K)
 {
     PhaseConstsT<T> k;
     k.dx = (T)K.dx; k.two_dx = (T)K.two_dx; k.dx2 = (T)K.dx2; k.four_dx2 = (T)K.four_dx2;
+    k.inv_dx = (T)K.inv_dx; k.inv_two_dx = (T)K.inv_two_dx; k.inv_dx2 = (T)K.inv_dx2;
+    k.inv_four_dx2 = (T)K.inv_four_dx2;
     k.w0 = (T)K.w0; k.maxdphi = (T)K.maxdphi; k.eps = (T)K.eps; k.lambda = (T)K.lambda;
     k.diffusion = (T)K.diffusion; k.ral = (T)K.ral; k.rat = (T)K.rat; k.tau_num = (T)K.tau_num;
     k.alpha = (T)K.alpha; k.kwh = (T)K.kwh; k.inv_mu_unused = (T)K.inv_mu_unused;
@@ -77,9 +87,9 @@ template <typename T>
 __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T conc, T vxp, T vxm,
                                         T vyp, T vym, T vzp, T vzm, bool is_well, T* tau_out)
 {
-    const T gx = (s[1] - s[2]) / K.two_dx;
-    const T gy = (s[3] - s[4]) / K.two_dx;
-    const T gz = (s[5] - s[6]) / K.two_dx;
+    const T gx = (s[1] - s[2]) * K.inv_two_dx;
+    const T gy = (s[3] - s[4]) * K.inv_two_dx;
+    const T gz = (s[5] - s[6]) * K.inv_two_dx;
     const T dpf = sqrt(gx * gx + gy * gy + gz * gz);
     T vs = T(0.0);
     if (dpf > T(1.0e-6)) {
@@ -101,12 +111,12 @@ __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T 
     if (is_well) return T(0.0);
 
     const T phix = gx, phiy = gy, phiz = gz;
-    const T phixx = (s[1] - T(2.0) * s[0] + s[2]) / K.dx2;
-    const T phiyy = (s[3] - T(2.0) * s[0] + s[4]) / K.dx2;
-    const T phizz = (s[5] - T(2.0) * s[0] + s[6]) / K.dx2;
-    const T phixy = (s[7] - s[8] - s[9] + s[10]) / K.four_dx2;
-    const T phiyz = (s[11] - s[12] - s[13] + s[14]) / K.four_dx2;
-    const T phizx = (s[15] - s[16] - s[17] + s[18]) / K.four_dx2;
+    const T phixx = (s[1] - T(2.0) * s[0] + s[2]) * K.inv_dx2;
+    const T phiyy = (s[3] - T(2.0) * s[0] + s[4]) * K.inv_dx2;
+    const T phizz = (s[5] - T(2.0) * s[0] + s[6]) * K.inv_dx2;
+    const T phixy = (s[7] - s[8] - s[9] + s[10]) * K.inv_four_dx2;
+    const T phiyz = (s[11] - s[12] - s[13] + s[14]) * K.inv_four_dx2;
+    const T phizx = (s[15] - s[16] - s[17] + s[18]) * K.inv_four_dx2;
     if (phix == T(0.0) && phiy == T(0.0) && phiz == T(0.0) && phixx == T(0.0) && phiyy == T(0.0) &&
         phizz == T(0.0) && phixy == T(0.0) && phiyz == T(0.0) && phizx == T(0.0))
         return T(0.0);
@@ -134,7 +144,7 @@ __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T 
 
     const T ph = s[0];
     const T f1 = (T(1.0) - ph * ph) * (ph - K.lambda * conc);
-    const T lap = (+s[1] + s[2] + s[3] + s[4] + s[5] + s[6] - T(6.0) * s[0]) / K.dx2;
+    const T lap = (+s[1] + s[2] + s[3] + s[4] + s[5] + s[6] - T(6.0) * s[0]) * K.inv_dx2;
     const T f3 = w * w * lap;
 
     const T md = K.maxdphi;
@@ -205,7 +215,7 @@ template <typename T>
 __device__ __forceinline__ T concent_cell(const PhaseConstsT<T>& K, const T* cc, T dphidt, T vxp,
                                           T vxm, T vyp, T vym, T vzp, T vzm)
 {
-    const T lap = (+cc[1] + cc[2] + cc[3] + cc[4] + cc[5] + cc[6] - T(6.0) * cc[0]) / K.dx2;
+    const T lap = (+cc[1] + cc[2] + cc[3] + cc[4] + cc[5] + cc[6] - T(6.0) * cc[0]) * K.inv_dx2;
     const T c1 = K.diffusion * lap;
     const T c2 = K.alpha * dphidt;
     const T c3 = T(0.0);
@@ -215,9 +225,20 @@ __device__ __forceinline__ T concent_cell(const PhaseConstsT<T>& K, const T* cc,
     const T fym = vym * (vym > T(0.0) ? cc[4] : cc[0]);
     const T fzp = vzp * (vzp > T(0.0) ? cc[0] : cc[5]);
     const T fzm = vzm * (vzm > T(0.0) ? cc[6] : cc[0]);
-    const T div = (fxp - fxm) / K.dx + (fyp - fym) / K.dx + (fzp - fzm) / K.dx;
+    const T div = (fxp - fxm) * K.inv_dx + (fyp - fym) * K.inv_dx + (fzp - fzm) * K.inv_dx;
     const T c7 = -div;
     return c1 + c2 + c3 + c7;
+}
+
+// Tiles of the fused step kernel: 64 x 4 cells of a plane, kStepZ planes.
+constexpr int kStepX = 64, kStepY = 4, kStepZ = 16;
+
+inline int step_tiles(int nx, int ny, int nz, int* gx, int* gy, int* gz)
+{
+    *gx = (nx + kStepX - 1) / kStepX;
+    *gy = (ny + kStepY - 1) / kStepY;
+    *gz = (nz + kStepZ - 1) / kStepZ;
+    return *gx * *gy * *gz;
 }
 
 }  // namespace wpf

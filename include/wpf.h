@@ -47,6 +47,13 @@ extern "C" {
 #define WPF_FIELD_DCONCENTDT   5  /* dconcentdt                               */
 #define WPF_FIELD_PERMEAB      6  /* permeab(0:3,...), download only          */
 #define WPF_FIELD_VELOCITY     7  /* velocity(1:3,...), download only         */
+/* The two above are rebuilt on the host from the device's phase, k_rock and
+ * pressure in the reference's AoS layout (every face 0..n, halos included).
+ * The next two are the device's own arrays, (1:3, 0:nx+1, 0:ny+1, 0:nz+1)
+ * with the +x/+y/+z faces of each interior cell and 0 on the halo: */
+#define WPF_FIELD_FACE_T       8  /* t = permeab(face)/viscosity, 0 on the
+                                     global edge (the pressure operator)    */
+#define WPF_FIELD_FACE_V       9  /* face velocities of calcVelocity        */
 
 /* The subset of Param_type (wPFAW3D.f90:27-76) that the hot path reads.
  * Global and local sizes coincide: one GPU holds the whole grid. */

@@ -24,7 +24,7 @@ WPF_ERR_NODEV = 5
 WPF_ERR_STATE = 6
 
 FIELD = {"permeab_rock": 0, "pressure": 1, "phase": 2, "concent": 3, "dphasedt": 4,
-         "dconcentdt": 5, "permeab": 6, "velocity": 7}
+         "dconcentdt": 5, "permeab": 6, "velocity": 7, "face_t": 8, "face_v": 9}
 
 SOLVER = {"cg": 0, "mgcg": 1, "mixed": 2}
 KERNEL = {"perm": 0, "step_fields": 1, "smooth": 2, "matvec": 3, "true_resid": 4,

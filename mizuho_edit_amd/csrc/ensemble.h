@@ -50,8 +50,8 @@ struct EnsMember {
     int64_t plane, n;
     VZ Z;    // vector-kernel tiles (tuned depth)
     ZG G;    // restriction tiles
-    int nb1, nbv, nbz, gx3, gy3, nb3;  // blocks of the 1-D, vector, restriction and cell kernels
-    int gxs, gys, nbs;                  // tiles of the fused step kernel
+    int nb1, nbv, nbz;  // blocks of the 1-D, vector and restriction kernels
+    int gxs, gys, nbs;  // tiles of the cell kernels (permeability, fused step)
     // FP64 state and operator
     double *phi[2], *conc[2], *p, *krock, *tx, *ty, *tz, *invD, *fsrc;
     double* xb[2];  // previous solution / saved start, roles swapped by sel

@@ -2,9 +2,9 @@
 // ensemble.h for the tiling and include/wpf.h for the interface).
 //
 // One time step of all members:
-//   graph A  begin (step count, extrapolation test) -> permeability ->
-//            Dirichlet values -> extrapolated guess -> ||D^-1 b|| and the
-//            tolerance -> FP32 operator and Galerkin levels -> S refinement
+//   graph A  begin (step count, extrapolation test) -> permeability with
+//            the FP32 operator -> Dirichlet values -> extrapolated guess ->
+//            ||D^-1 b|| and the tolerance -> Galerkin levels -> S refinement
 //            stages, each: FP64 true residual, stage test, K iterations of
 //            FP32 MG-PCG (every launch skips members whose stage or inner
 //            solve is done), e added to p -> final FP64 residual test
@@ -39,8 +39,8 @@ struct wpf_ensemble {
     wpf::EnsState* d_state = nullptr;        // device states
     wpf::EnsState* h_state = nullptr;        // pinned host copy
     // tile tables: 1-D fine, vector, restriction, cell kernels, coarse levels
-    wpf::EnsTile *t1 = nullptr, *tv = nullptr, *tz = nullptr, *t3 = nullptr, *ts = nullptr;
-    int n1 = 0, nv = 0, nz = 0, n3 = 0, ns = 0;
+    wpf::EnsTile *t1 = nullptr, *tv = nullptr, *tz = nullptr, *ts = nullptr;
+    int n1 = 0, nv = 0, nz = 0, ns = 0;
     wpf::EnsTile* tl[wpf::kEnsMaxLevels] = {};
     int nl[wpf::kEnsMaxLevels] = {};
     size_t coarsest_lds = 0;
@@ -262,20 +262,6 @@ __global__ void __launch_bounds__(256) ke_bnorm(ENS_ARGS)
     ENS_END
 }
 
-__global__ void __launch_bounds__(256) ke_to_f32(ENS_ARGS)
-{
-    ENS_TILES(G_ALL)
-    t_to_f32(ens_lv64(M), M.s32, M.tx32, M.ty32, M.tz32, M.invD32, T.b, M.nb1);
-    ENS_END
-}
-
-__global__ void __launch_bounds__(256) ke_to_f16(ENS_ARGS)
-{
-    ENS_TILES(G_ALL)
-    t_to_f16(ens_lv64(M), M.s32, M.htx, M.hty, M.htz, M.tx32, M.ty32, M.tz32, M.invD32, T.b, M.nb1);
-    ENS_END
-}
-
 __global__ void __launch_bounds__(256) ke_true_resid(ENS_ARGS)
 {
     ENS_TILES(G_STAGE)
@@ -456,13 +442,6 @@ using namespace wpf;
 // ---------------------------------------------------------------------------
 namespace {
 
-int member_blocks3(const wpf_ctx* c, int* gx, int* gy)
-{
-    *gx = (c->g.nx + 63) / 64;
-    *gy = (c->g.ny + 3) / 4;
-    return *gx * *gy * c->g.nz;
-}
-
 // (member, block) table with each member's list padded to a multiple of 8
 int build_tiles(const std::vector<int>& counts, EnsTile** dev, int* n)
 {
@@ -485,7 +464,7 @@ int build_tables(wpf_ensemble* e)
 {
     const int nm = (int)e->mem.size();
     std::vector<EnsMember> M(nm);
-    std::vector<int> c1(nm), cv(nm), cz(nm), c3(nm), cs(nm);
+    std::vector<int> c1(nm), cv(nm), cz(nm), cs(nm);
     std::vector<std::vector<int>> cl(e->nlev, std::vector<int>(nm));
     for (int m = 0; m < nm; m++) {
         wpf_ctx* c = e->mem[m];
@@ -497,7 +476,6 @@ int build_tables(wpf_ensemble* e)
         X.nb1 = blocks_for(c->g.n);
         X.nbv = vz_blocks(X.Z);
         X.nbz = zg_blocks(X.G);
-        X.nb3 = member_blocks3(c, &X.gx3, &X.gy3);
         {
             int gz;
             X.nbs = step_tiles(c->g.nx, c->g.ny, c->g.nz, &X.gxs, &X.gys, &gz);
@@ -536,14 +514,13 @@ int build_tables(wpf_ensemble* e)
         X.dt_pref = p.tfac * 0.5 * (p.dx * p.dx) / (p.epsilon * p.epsilon);
         X.eps2_D = (p.epsilon * p.epsilon) / p.diffusion;
         X.K = phase_consts(p);
-        c1[m] = X.nb1; cv[m] = X.nbv; cz[m] = X.nbz; c3[m] = X.nb3; cs[m] = X.nbs;
+        c1[m] = X.nb1; cv[m] = X.nbv; cz[m] = X.nbz; cs[m] = X.nbs;
     }
     if (!e->d_mem) WPF_HIP(hipMalloc((void**)&e->d_mem, sizeof(EnsMember) * nm));
     WPF_HIP(hipMemcpy(e->d_mem, M.data(), sizeof(EnsMember) * nm, hipMemcpyHostToDevice));
     int rc;
     if ((rc = build_tiles(c1, &e->t1, &e->n1)) || (rc = build_tiles(cv, &e->tv, &e->nv)) ||
-        (rc = build_tiles(cz, &e->tz, &e->nz)) || (rc = build_tiles(c3, &e->t3, &e->n3)) ||
-        (rc = build_tiles(cs, &e->ts, &e->ns)))
+        (rc = build_tiles(cz, &e->tz, &e->nz)) || (rc = build_tiles(cs, &e->ts, &e->ns)))
         return rc;
     for (int l = 1; l < e->nlev; l++)
         if ((rc = build_tiles(cl[l], &e->tl[l], &e->nl[l]))) return rc;
@@ -679,16 +656,13 @@ int sched_solve(wpf_ensemble* e)
     Sched S{e, mg_opts(o), (int)e->mem.size()};
     const int nm = S.nm;
     ELAUNCH(ke_begin, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, e->extrap ? 1 : 0);
-    ECHK(ens_perm(e->stream, e->d_mem, e->t3, e->n3, e->grid));
+    // permeability, faces, 1/D and the scaled FP32 operator in one pass
+    ECHK(ens_perm(e->stream, e->d_mem, e->ts, e->ns, e->grid, e->op16 ? PERM_F16 : PERM_F32));
     ELAUNCH(ke_dirichlet, nm, 256, 0, e->d_mem);
     if (e->extrap) ELAUNCH(ke_extrapolate, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
     ELAUNCH(ke_bnorm, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
     ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 1, (int)S_BN, (int)G_ALL);
     ELAUNCH(ke_tol, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.rtol, o.atol);
-    if (e->op16)
-        ELAUNCH(ke_to_f16, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
-    else
-        ELAUNCH(ke_to_f32, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
     for (int l = 1; l < e->nlev; l++) ELAUNCH(ke_coarsen, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l);
     const float w1 = (float)S.mo.w1;
     for (int stage = 0; stage < e->eo.stages; stage++) {
@@ -844,12 +818,7 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
                 (rc = dev_alloc(c->h_tz, c->g)))
                 return fail(rc);
         }
-        double s = c->t_scale;
-        if (e->op16) {
-            const double tmax = std::max(c->prm.permeab_hole, c->krock_max) / c->prm.viscosity * s;
-            if (tmax > 0.0) s *= std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / tmax)));
-        }
-        e->s32[m] = s;
+        e->s32[m] = op32_scale(c, e->op16);
         const int L = (int)c->mg32.size();
         if (nlev < 0) nlev = L;
         if (L != nlev || L > kEnsMaxLevels) {
@@ -1021,7 +990,7 @@ int wpf_ensemble_sync(wpf_ensemble* e, int32_t* itime, double* time, double* dti
             std::swap(c->xprev, c->psave);
             s.sel = 0;
         }
-        c->have_perm = false;
+        perm_stale(c);
         c->have_velocity = false;
         double sc[2];
         unsigned long long k = 0;
@@ -1050,7 +1019,7 @@ int wpf_ensemble_destroy(wpf_ensemble* e)
     auto F = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    F(e->d_mem); F(e->d_state); F(e->t1); F(e->tv); F(e->tz); F(e->t3); F(e->ts);
+    F(e->d_mem); F(e->d_state); F(e->t1); F(e->tv); F(e->tz); F(e->ts);
     for (int l = 0; l < kEnsMaxLevels; l++) F(e->tl[l]);
     if (e->h_state) (void)hipHostFree(e->h_state);
     if (e->stream) (void)hipStreamDestroy(e->stream);

@@ -44,74 +44,168 @@ __device__ __forceinline__ double kcell(double kwh, double phi, double kr)
     const double f = 0.5 * (1.0 + phi);
     return kwh * f + kr * (1.0 - f);
 }
-__device__ __forceinline__ double hmean(double a, double b) { return 2.0 / (1.0 / a + 1.0 / b); }
+
+// value of the lane below (lane 0 gets its own back)
+template <typename T>
+__device__ __forceinline__ T from_left_lane(T v)
+{
+    return __shfl_up(v, 1, 64);
+}
 
 // ---------------------------------------------------------------------------
-// calcPermeability + operator set-up.  Each thread rebuilds the six faces of
-// its cell (the -faces exactly as the neighbour computes them, same operand
-// order) so the diagonal comes out of the same pass.
+// calcPermeability + operator set-up: the faces t = k_face/mu, the Jacobi
+// diagonal, and (when the mixed solver asks for it) its scaled FP32 operator
+// with FP16-stored faces, in one pass.
+//
+// Tiling as the fused step (64 x 4 cells, kStepZ planes up z).  Each plane's
+// 1/k0 is staged once in LDS with a one-cell border (the reference's
+// harmonic mean 2/(1/a + 1/b) then needs one division per face instead of
+// three); a cell forms its +x, +y, +z faces, takes its -x face from the lane
+// below (the same operands in the same order, so the same value) and its -z
+// face from the plane below, so every face is computed once or twice rather
+// than twice with three divisions each.
 // ---------------------------------------------------------------------------
-// block (bx, by, bz) of 64 x 4 x 1 threads
+template <int MODE>
 __device__ __forceinline__ void t_perm(const Dims& d, int bx, int by, int bz, double kwh, double mu,
                                        const double* __restrict__ phi, const double* __restrict__ kr,
                                        const uint8_t* __restrict__ mask, double* __restrict__ tx,
                                        double* __restrict__ ty, double* __restrict__ tz,
-                                       double* __restrict__ invD)
+                                       double* __restrict__ invD, const PermOut32& O)
 {
-    const int i = bx * blockDim.x + threadIdx.x;
-    const int j = by * blockDim.y + threadIdx.y;
-    const int k = bz * blockDim.z + threadIdx.z;
-    if (i >= d.nx || j >= d.ny || k >= d.nz) return;
-    const int64_t c = i + (int64_t)d.nx * (j + (int64_t)d.ny * k);
-    const int ip = min(i + 1, d.nx - 1), im = max(i - 1, 0);
-    const int jp = min(j + 1, d.ny - 1), jm = max(j - 1, 0);
-    const int kp = min(k + 1, d.nz - 1), km = max(k - 1, 0);
-    const int64_t row = (int64_t)d.nx * (j + (int64_t)d.ny * k);
-    auto at = [&](int ii, int jj, int kk) { return ii + (int64_t)d.nx * (jj + (int64_t)d.ny * kk); };
-    const double k0 = kcell(kwh, phi[c], kr[c]);
-    const int64_t cxp = row + ip, cxm = row + im;
-    const int64_t cyp = at(i, jp, k), cym = at(i, jm, k);
-    const int64_t czp = at(i, j, kp), czm = at(i, j, km);
-    const double kxp = kcell(kwh, phi[cxp], kr[cxp]);
-    const double kxm = kcell(kwh, phi[cxm], kr[cxm]);
-    const double kyp = kcell(kwh, phi[cyp], kr[cyp]);
-    const double kym = kcell(kwh, phi[cym], kr[cym]);
-    const double kzp = kcell(kwh, phi[czp], kr[czp]);
-    const double kzm = kcell(kwh, phi[czm], kr[czm]);
-    // t = permeab(face)/viscosity, 0 on the global edge   (1:843-848)
-    const double txp = (i < d.nx - 1) ? hmean(k0, kxp) / mu : 0.0;
-    const double txm = (i > 0) ? hmean(kxm, k0) / mu : 0.0;
-    const double typ = (j < d.ny - 1) ? hmean(k0, kyp) / mu : 0.0;
-    const double tym = (j > 0) ? hmean(kym, k0) / mu : 0.0;
-    const double tzp = (k < d.nz - 1) ? hmean(k0, kzp) / mu : 0.0;
-    const double tzm = (k > 0) ? hmean(kzm, k0) / mu : 0.0;
-    tx[c] = txp;
-    ty[c] = typ;
-    tz[c] = tzp;
-    const double dsum = txp + txm + typ + tym + tzp + tzm;  // -tc   (1:849)
-    const bool fixed = (mask[c] & WPF_MASK_FIXED) != 0;
-    invD[c] = (fixed || dsum == 0.0) ? 0.0 : 1.0 / dsum;
+    constexpr int LX = kStepX + 2, LY = kStepY + 2, LP = LX * LY;
+    __shared__ double sik[4][LP];
+    const int tid = threadIdx.x + kStepX * threadIdx.y;
+    const int i0 = bx * kStepX, j0 = by * kStepY, k0 = bz * kStepZ;
+    const int k1 = min(k0 + kStepZ, d.nz);
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool in = i < d.nx && j < d.ny;
+    const int64_t nx = d.nx, pl = d.plane;
+    constexpr int NT = kStepX * kStepY, NE = (LP + NT - 1) / NT;
+    double rphi[NE], rkr[NE];
+    int64_t goff[NE];
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+        const int e = tid + q * NT;
+        const int gi = min(max(i0 + e % LX - 1, 0), d.nx - 1);
+        const int gj = min(max(j0 + e / LX - 1, 0), d.ny - 1);
+        goff[q] = gi + nx * gj;
+    }
+    auto fetch = [&](int kk) {
+        const int64_t po = pl * min(max(kk, 0), d.nz - 1);
+#pragma unroll
+        for (int q = 0; q < NE; q++) {
+            if (tid + q * NT < LP) {
+                rphi[q] = phi[goff[q] + po];
+                rkr[q] = kr[goff[q] + po];
+            }
+        }
+    };
+    // 1/k0 with k0 = permeab(0) = kwh*f + k_rock*(1-f), f = (1+phase)/2
+    // (1:567-574): the 1.0/a of the harmonic mean
+    auto stage = [&](int kk) {
+#pragma unroll
+        for (int q = 0; q < NE; q++) {
+            const int e = tid + q * NT;
+            if (e < LP) sik[kk & 3][e] = 1.0 / kcell(kwh, rphi[q], rkr[q]);
+        }
+    };
+    // t = permeab(face)/viscosity = (2/(1/k_a + 1/k_b))/mu   (1:577-599, 843-848)
+    auto face = [&](double ia, double ib) { return 2.0 / (ia + ib) / mu; };
+    fetch(k0 - 1);
+    stage(k0 - 1);
+    fetch(k0);
+    stage(k0);
+    fetch(k0 + 1);
+    const int e0 = (threadIdx.y + 1) * LX + threadIdx.x + 1;
+    uint8_t m_cur = in ? mask[i + nx * j + pl * k0] : 0, m_nxt = 0;
+    double tz_prev = 0.0;
+    for (int k = k0; k < k1; k++) {
+        stage(k + 1);
+        __syncthreads();
+        if (k + 2 <= k1) fetch(k + 2);
+        if (in && k + 1 < k1) m_nxt = mask[i + nx * j + pl * (k + 1)];
+        const int s0 = k & 3, sp = (k + 1) & 3, sm = (k - 1) & 3;
+        const double ic = sik[s0][e0];
+        // 0 on the global edge
+        const double txp = (i < d.nx - 1) ? face(ic, sik[s0][e0 + 1]) : 0.0;
+        const double typ = (j < d.ny - 1) ? face(ic, sik[s0][e0 + LX]) : 0.0;
+        const double tzp = (k < d.nz - 1) ? face(ic, sik[sp][e0]) : 0.0;
+        double txm = from_left_lane(txp);
+        if (threadIdx.x == 0) txm = (i > 0) ? face(sik[s0][e0 - 1], ic) : 0.0;
+        const double tym = (j > 0) ? face(sik[s0][e0 - LX], ic) : 0.0;
+        const double tzm = (k == k0) ? ((k > 0) ? face(sik[sm][e0], ic) : 0.0) : tz_prev;
+        tz_prev = tzp;
+        if (in) {
+            const int64_t c = i + nx * j + pl * k;
+            tx[c] = txp;
+            ty[c] = typ;
+            tz[c] = tzp;
+            const double dsum = txp + txm + typ + tym + tzp + tzm;  // -tc   (1:849)
+            const bool fixed = (m_cur & WPF_MASK_FIXED) != 0;
+            const double id = (fixed || dsum == 0.0) ? 0.0 : 1.0 / dsum;
+            invD[c] = id;
+            if (MODE == PERM_F32) {
+                // the FP32 system: t32 = float(t * s)
+                const double s = O.s;
+                const float a = (float)(txp * s), b = (float)(typ * s), f = (float)(tzp * s);
+                O.tx[c] = a;
+                O.ty[c] = b;
+                O.tz[c] = f;
+                const float D = a + (float)(txm * s) + b + (float)(tym * s) + f + (float)(tzm * s);
+                O.invD[c] = (id == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
+            } else if (MODE == PERM_F16) {
+                // FP16-stored faces h = half(float(t * s)); the FP32 copy and
+                // 1/D hold exactly the values the FP16 faces decode to
+                const double s = O.s;
+                auto h = [&](double t) { return (_Float16)(float)(t * s); };
+                const _Float16 a = h(txp), b = h(typ), f = h(tzp);
+                O.htx[c] = a;
+                O.hty[c] = b;
+                O.htz[c] = f;
+                O.tx[c] = (float)a;
+                O.ty[c] = (float)b;
+                O.tz[c] = (float)f;
+                const float D = (float)a + (float)h(txm) + (float)b + (float)h(tym) + (float)f + (float)h(tzm);
+                O.invD[c] = (id == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
+            }
+        }
+        m_cur = m_nxt;
+    }
+    // the ring is reused by the next tile of a persistent launch
+    __syncthreads();
 }
 
+template <int MODE>
 __global__ void __launch_bounds__(256) k_perm(Dims d, double kwh, double mu, const double* __restrict__ phi,
                                               const double* __restrict__ kr,
                                               const uint8_t* __restrict__ mask, double* __restrict__ tx,
                                               double* __restrict__ ty, double* __restrict__ tz,
-                                              double* __restrict__ invD)
+                                              double* __restrict__ invD, PermOut32 O)
 {
-    t_perm(d, blockIdx.x, blockIdx.y, blockIdx.z, kwh, mu, phi, kr, mask, tx, ty, tz, invD);
+    t_perm<MODE>(d, blockIdx.x, blockIdx.y, blockIdx.z, kwh, mu, phi, kr, mask, tx, ty, tz, invD, O);
 }
 
-int launch_perm(wpf_ctx* c)
+int launch_perm(wpf_ctx* c, int mode, const PermOut32* out)
 {
     const Dims d = dims_of(c);
-    dim3 blk(64, 4, 1);
+    const dim3 blk(kStepX, kStepY, 1);
+    int gx, gy, gz;
+    (void)step_tiles(d.nx, d.ny, d.nz, &gx, &gy, &gz);
+    const dim3 gr(gx, gy, gz);
+    PermOut32 O{};
+    if (mode != PERM_F64) {
+        WPF_CHECK(out, WPF_ERR_ARG, "perm: no FP32 operator");
+        O = *out;
+    }
     prof_mark(c, WPF_KERNEL_PERM, true);
-    hipLaunchKernelGGL(k_perm, grid3(d, blk), blk, 0, c->stream, d, c->prm.permeab_hole, c->prm.viscosity, c->phi[c->cur], c->krock, c->mask, c->tx.p, c->ty.p,
-                       c->tz.p, c->invD.p);
+    auto* const kern = mode == PERM_F16 ? k_perm<PERM_F16> : mode == PERM_F32 ? k_perm<PERM_F32> : k_perm<PERM_F64>;
+    hipLaunchKernelGGL(kern, gr, blk, 0, c->stream, d, c->prm.permeab_hole, c->prm.viscosity, c->phi[c->cur],
+                       c->krock, c->mask, c->tx.p, c->ty.p, c->tz.p, c->invD.p, O);
     prof_mark(c, WPF_KERNEL_PERM, false);
     WPF_HIP(hipGetLastError());
     c->have_perm = true;
+    c->op32_mode = mode;
+    c->op32_scale = O.s;
     return WPF_OK;
 }
 
@@ -386,13 +480,6 @@ struct StepScalars {
     int bt_plane;
     int key_prev, key_cur, key_next;
 };
-
-// value of the lane below (lane 0 gets its own back)
-template <typename T>
-__device__ __forceinline__ T from_left_lane(T v)
-{
-    return __shfl_up(v, 1, 64);
-}
 
 // T is the precision of the phase-field and concentration stencils.  T =
 // double is the reference arithmetic; T = float evaluates the stencils in
@@ -747,15 +834,10 @@ int check_breakthrough(wpf_ctx* c, int itime)
 // ---------------------------------------------------------------------------
 // Ensemble forms: one launch over the blocks of every member (see
 // ensemble.h); each tile is the block (bx, by, bz) of the member's own
-// 64 x 4 x 1 launch.
+// launch of the same kernel (64 x 4 cells, kStepZ planes).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void ens_block3(const EnsMember& M, int b, int& bx, int& by, int& bz)
-{
-    bx = b % M.gx3;
-    by = (b / M.gx3) % M.gy3;
-    bz = b / (M.gx3 * M.gy3);
-}
 
+template <int MODE>
 __global__ void __launch_bounds__(256) ke_perm(const EnsMember* __restrict__ Ms,
                                                const EnsTile* __restrict__ tiles, int ntiles)
 {
@@ -763,10 +845,10 @@ __global__ void __launch_bounds__(256) ke_perm(const EnsMember* __restrict__ Ms,
         const EnsTile T = tiles[t];
         if (T.b < 0) continue;
         const EnsMember& M = Ms[T.m];
-        int bx, by, bz;
-        ens_block3(M, T.b, bx, by, bz);
+        const int bx = T.b % M.gxs, by = (T.b / M.gxs) % M.gys, bz = T.b / (M.gxs * M.gys);
         const Dims d{M.nx, M.ny, M.nz, M.plane};
-        t_perm(d, bx, by, bz, M.kwh, M.mu, M.phi[M.st->cur], M.krock, M.mask, M.tx, M.ty, M.tz, M.invD);
+        const PermOut32 O{M.s32, M.htx, M.hty, M.htz, M.tx32, M.ty32, M.tz32, M.invD32};
+        t_perm<MODE>(d, bx, by, bz, M.kwh, M.mu, M.phi[M.st->cur], M.krock, M.mask, M.tx, M.ty, M.tz, M.invD, O);
     }
 }
 
@@ -804,9 +886,10 @@ __global__ void __launch_bounds__(1024) ke_min_key(const EnsMember* __restrict__
     t_min_key(M.partials, M.nbs, M.tau_key + M.st->itime % 3);
 }
 
-int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid)
+int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid, int mode)
 {
-    hipLaunchKernelGGL(ke_perm, dim3(grid), dim3(64, 4, 1), 0, s, Ms, tiles, ntiles);
+    auto* const kern = mode == PERM_F16 ? ke_perm<PERM_F16> : mode == PERM_F32 ? ke_perm<PERM_F32> : ke_perm<PERM_F64>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kStepX, kStepY, 1), 0, s, Ms, tiles, ntiles);
     WPF_HIP(hipGetLastError());
     return WPF_OK;
 }

@@ -896,24 +896,15 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
     LV<double> L = view(F0, c->mask, true);
-    // The FP32 system is A' e = r' with A' = s A64 and r' = s r64, which has
-    // the same solution.  s = mu/kwh makes the faces O(1); with FP16 faces a
-    // further power of two puts the largest face near 4096 (sums of six stay
-    // below FP16's maximum, the smallest faces of these rocks stay normal).
-    double s = c->t_scale;
+    // the FP32 system A' = s A (op32_scale)
     const bool op16 = (o.flags & WPF_FLAG_OP16) != 0;
-    if (op16) {
-        const double tmax = std::max(c->prm.permeab_hole, c->krock_max) / c->prm.viscosity * s;
-        if (tmax > 0.0) s *= std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / tmax)));
-        if (!c->h_tx.raw) {
-            if ((rc = dev_alloc(c->h_tx, c->g)) || (rc = dev_alloc(c->h_ty, c->g)) ||
-                (rc = dev_alloc(c->h_tz, c->g)))
-                return rc;
-        }
-    }
-    // grow the hierarchy first: it appends to mg32, which may move mg32[0]
-    if ((rc = ensure_hierarchy(c->mg32, o.mg_levels))) return rc;
-    {
+    PermOut32 P;
+    int mode;
+    if ((rc = op32_setup(c, &o, &P, &mode))) return rc;
+    const double s = P.s;
+    // the permeability pass of a fused step may have written the FP32
+    // operator already (the same values: the same arithmetic per face)
+    if (c->op32_mode != mode || c->op32_scale != s) {
         MGLevel<float>& F32 = c->mg32[0];
         if (op16)
             LAUNCH1D(k_to_f16, n, L, s, c->h_tx.p, c->h_ty.p, c->h_tz.p, F32.tx.p, F32.ty.p, F32.tz.p,
@@ -1022,6 +1013,44 @@ int solve_pressure(wpf_ctx* c, const wpf_solver_opts* in, wpf_solve_stats* st)
 }
 
 void solver_defaults(const wpf_solver_opts* in, wpf_solver_opts* out) { fill_defaults(in, *out); }
+
+// The FP32 system is A' e = r' with A' = s A64 and r' = s r64, which has the
+// same solution.  s = mu/kwh makes the faces O(1); with FP16 faces a further
+// power of two puts the largest face near 4096 (sums of six stay below
+// FP16's maximum, the smallest faces of these rocks stay normal).
+double op32_scale(const wpf_ctx* c, bool op16)
+{
+    double s = c->t_scale;
+    if (op16) {
+        const double tmax = std::max(c->prm.permeab_hole, c->krock_max) / c->prm.viscosity * s;
+        if (tmax > 0.0) s *= std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / tmax)));
+    }
+    return s;
+}
+
+int op32_setup(wpf_ctx* c, const wpf_solver_opts* o, PermOut32* out, int* mode)
+{
+    const bool op16 = (o->flags & WPF_FLAG_OP16) != 0;
+    int rc = ensure_ws(c, true);
+    if (rc) return rc;
+    if (op16 && !c->h_tx.raw) {
+        if ((rc = dev_alloc(c->h_tx, c->g)) || (rc = dev_alloc(c->h_ty, c->g)) || (rc = dev_alloc(c->h_tz, c->g)))
+            return rc;
+    }
+    // grow the hierarchy first: it appends to mg32, which may move mg32[0]
+    if ((rc = ensure_hierarchy(c->mg32, o->mg_levels))) return rc;
+    const MGLevel<float>& F32 = c->mg32[0];
+    out->s = op32_scale(c, op16);
+    out->htx = op16 ? c->h_tx.p : nullptr;
+    out->hty = op16 ? c->h_ty.p : nullptr;
+    out->htz = op16 ? c->h_tz.p : nullptr;
+    out->tx = F32.tx.p;
+    out->ty = F32.ty.p;
+    out->tz = F32.tz.p;
+    out->invD = F32.invD.p;
+    *mode = op16 ? PERM_F16 : PERM_F32;
+    return WPF_OK;
+}
 
 int solver_workspace(wpf_ctx* c, const wpf_solver_opts* o)
 {

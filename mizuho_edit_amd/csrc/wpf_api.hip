@@ -342,7 +342,7 @@ int wpf_upload(wpf_ctx* c, int32_t field, const double* h)
         for (double v : tmp) m = std::max(m, v);
         c->krock_max = m;
     }
-    if (field == WPF_FIELD_PHASE || field == WPF_FIELD_PERMEAB_ROCK) c->have_perm = false;
+    if (field == WPF_FIELD_PHASE || field == WPF_FIELD_PERMEAB_ROCK) perm_stale(c);
     if (field == WPF_FIELD_PRESSURE) c->have_velocity = false;
     return WPF_OK;
 }
@@ -371,6 +371,26 @@ int wpf_download(wpf_ctx* c, int32_t field, double* h)
     const Grid& g = c->g;
     const size_t nb = sizeof(double) * (size_t)g.n;
     const int64_t nh = (int64_t)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    if (field == WPF_FIELD_FACE_T || field == WPF_FIELD_FACE_V) {
+        // the device's own +x/+y/+z face arrays of the interior cells
+        const bool t = field == WPF_FIELD_FACE_T;
+        WPF_CHECK(t ? c->have_perm : c->have_velocity, WPF_ERR_STATE,
+                  t ? "face coefficients before calcPermeability" : "face velocities before calcVelocity");
+        const double* src[3] = {t ? c->tx.p : c->vx, t ? c->ty.p : c->vy, t ? c->tz.p : c->vz};
+        std::vector<double> a(3 * (size_t)g.n);
+        for (int d = 0; d < 3; d++)
+            WPF_HIP(hipMemcpyAsync(a.data() + d * (size_t)g.n, src[d], nb, hipMemcpyDeviceToHost, c->stream));
+        WPF_HIP(hipStreamSynchronize(c->stream));
+        memset(h, 0, sizeof(double) * 3 * nh);
+        for (int k = 0; k < g.nz; k++)
+            for (int j = 0; j < g.ny; j++)
+                for (int i = 0; i < g.nx; i++) {
+                    const size_t q = (size_t)i + (size_t)g.nx * (j + (size_t)g.ny * k);
+                    const int64_t o = hidx(c, i + 1, j + 1, k + 1);
+                    for (int d = 0; d < 3; d++) h[3 * o + d] = a[d * (size_t)g.n + q];
+                }
+        return WPF_OK;
+    }
     if (field == WPF_FIELD_PERMEAB || field == WPF_FIELD_VELOCITY) {
         // permeab(0:3,...) and velocity(1:3,...) in the reference's AoS layout:
         // node value and faces 0..n, with the face below cell 1 and halo rows
@@ -434,7 +454,7 @@ int wpf_download(wpf_ctx* c, int32_t field, double* h)
 int wpf_calc_permeability(wpf_ctx* c)  // wPFAW3D.f90:561-599
 {
     WPF_CHECK(c, WPF_ERR_ARG, "null context");
-    return launch_perm(c);
+    return launch_perm(c, PERM_F64, nullptr);
 }
 
 int wpf_solve_pressure(wpf_ctx* c, const wpf_solver_opts* o, wpf_solve_stats* st)  // :603-1160
@@ -465,7 +485,7 @@ int wpf_evolve_phase(wpf_ctx* c, double dtime)  // :1628-1632
 {
     WPF_CHECK(c, WPF_ERR_ARG, "null context");
     int rc = launch_evolve(c, 0, dtime);
-    c->have_perm = false;
+    perm_stale(c);
     c->dt_hist[1] = c->dt_hist[0];
     c->dt_hist[0] = dtime;
     return rc;
@@ -517,7 +537,18 @@ int wpf_step(wpf_ctx* c, const wpf_solver_opts* o, int32_t itime, double* time, 
                  std::fmin((p.epsilon * p.epsilon) / p.diffusion, *taumin);
     }
     *time = *time + *dtime;
-    int rc = launch_perm(c);
+    // the mixed solver's FP32 operator comes out of the permeability pass
+    wpf_solver_opts od;
+    solver_defaults(o, &od);
+    int rc;
+    if (od.solver == WPF_SOLVER_MIXED) {
+        PermOut32 P;
+        int mode;
+        if ((rc = op32_setup(c, &od, &P, &mode))) return rc;
+        rc = launch_perm(c, mode, &P);
+    } else {
+        rc = launch_perm(c, PERM_F64, nullptr);
+    }
     if (rc) return rc;
     rc = solve_pressure(c, o, st);
     if (rc && rc != WPF_ERR_NOCONV) return rc;
@@ -532,7 +563,7 @@ int wpf_step(wpf_ctx* c, const wpf_solver_opts* o, int32_t itime, double* time, 
     WPF_HIP(hipStreamSynchronize(c->stream));
     unsigned long long u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
     memcpy(taumin, &u, 8);
-    c->have_perm = false;
+    perm_stale(c);
     c->have_velocity = false;
     return solve_rc;
 }

@@ -122,6 +122,10 @@ struct wpf_ctx {
     double* dcdt = nullptr;
     double* vx = nullptr; double* vy = nullptr; double* vz = nullptr;  // lazily
     bool have_perm = false, have_velocity = false, have_pressure = false;
+    // the scaled FP32 operator the last permeability pass also wrote
+    // (wpf::PermMode; -1: none, the solver converts the FP64 faces itself)
+    int op32_mode = -1;
+    double op32_scale = 0.0;
 
     // FP64 operator and solver vectors (fine level)
     wpf::Dev<double> tx, ty, tz, invD;
@@ -199,7 +203,25 @@ inline void prof_mark(wpf_ctx* c, int id, bool begin)
 namespace wpf {
 
 // kernels' host launchers (k_fields.hip)
-int launch_perm(wpf_ctx* c);
+// what the permeability pass writes besides the FP64 faces and 1/D
+enum PermMode { PERM_F64 = 0, PERM_F32 = 1, PERM_F16 = 2 };
+struct PermOut32 {
+    double s;                    // scale of the FP32 system
+    _Float16 *htx, *hty, *htz;   // PERM_F16: FP16 faces
+    float *tx, *ty, *tz, *invD;  // FP32 faces (the decoded FP16 ones) and 1/D
+};
+int launch_perm(wpf_ctx* c, int mode, const PermOut32* out);
+// the permeability (and everything derived from it) no longer matches phase
+inline void perm_stale(wpf_ctx* c)
+{
+    c->have_perm = false;
+    c->op32_mode = -1;
+}
+// scale s of the mixed solver's FP32 system A' = s A (see k_solver.hip)
+double op32_scale(const wpf_ctx* c, bool op16);
+// allocate the mixed solver's FP32 operator for options o and say where the
+// permeability pass may write it
+int op32_setup(wpf_ctx* c, const wpf_solver_opts* o, PermOut32* out, int* mode);
 int launch_velocity(wpf_ctx* c);
 int launch_phase_deriv(wpf_ctx* c, double* taumin);
 int launch_concent_deriv(wpf_ctx* c);
@@ -213,7 +235,7 @@ int check_breakthrough(wpf_ctx* c, int itime);
 // ensemble forms of the cell kernels (k_fields.hip); see ensemble.h
 struct EnsMember;
 struct EnsTile;
-int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid);
+int ens_perm(hipStream_t s, const EnsMember* Ms, const EnsTile* tiles, int ntiles, int grid, int mode);
 int ens_step_fields(hipStream_t s, const EnsMember* Ms, int nmembers, const EnsTile* tiles, int ntiles,
                     int grid);
 

@@ -99,7 +99,7 @@ class WPFAW3D:
               f"wpf_upload({name})")
 
     def download(self, name: str) -> np.ndarray:
-        comps = {"permeab": 4, "velocity": 3}.get(name, 0)
+        comps = {"permeab": 4, "velocity": 3, "face_t": 3, "face_v": 3}.get(name, 0)
         out = np.zeros(self._shape(comps), dtype=np.float64, order="F")
         check(_lib.lib().wpf_download(self._ctx, FIELD[name], out.ctypes.data_as(_lib.D)),
               f"wpf_download({name})")

@@ -45,6 +45,28 @@ def _interior(a):
 
 
 def test_permeability_bit_exact(gpu):
+    """The device's face coefficients t = permeab(face)/viscosity, bitwise
+    against the oracle's harmonic means (0 on the global edge, where the
+    reference's assembly drops the face, wPFAW3D.f90:843-848)."""
+    case, st = _state()
+    g = _gpu_from(case, st)
+    g.calcPermeability()
+    st.calc_permeability()
+    p = case.params
+    t = g.download("face_t")
+    inner = (slice(1, p.nx + 1), slice(1, p.ny + 1), slice(1, p.nz + 1))
+    for a, ref in ((0, st.kx), (1, st.ky), (2, st.kz)):
+        exp = ref[inner] / p.viscosity
+        edge = [slice(None)] * 3
+        edge[a] = -1
+        exp[tuple(edge)] = 0.0
+        assert np.array_equal(t[a][inner], exp), a
+    g.close()
+
+
+def test_permeab_download_matches_oracle(gpu):
+    """permeab(0:3,...) as the reference holds it (node value and faces
+    0..n with halos), rebuilt on download from the device's fields."""
     case, st = _state()
     g = _gpu_from(case, st)
     g.calcPermeability()
@@ -52,7 +74,6 @@ def test_permeability_bit_exact(gpu):
     perm = g.permeab
     assert np.array_equal(perm[0], st.k0)
     p = case.params
-    # faces 0..n on every axis (halo faces included, as the reference fills them)
     for a, ref in ((1, st.kx), (2, st.ky), (3, st.kz)):
         assert np.array_equal(perm[a][:p.nx + 1, :p.ny + 1, :p.nz + 1],
                               ref[:p.nx + 1, :p.ny + 1, :p.nz + 1]), a
@@ -60,17 +81,24 @@ def test_permeability_bit_exact(gpu):
 
 
 def test_velocity_bit_exact(gpu):
+    """The device's face velocities of calcVelocity, bitwise against the
+    oracle's at the +faces of every interior cell."""
     case, st = _state()
     g = _gpu_from(case, st)
     g.calcPermeability()
     g.calcVelocity()
     st.calc_permeability()
     st.calc_velocity()
-    v = g.velocity
+    v = g.download("face_v")
     p = case.params
+    inner = (slice(1, p.nx + 1), slice(1, p.ny + 1), slice(1, p.nz + 1))
+    for a, ref in ((0, st.vx), (1, st.vy), (2, st.vz)):
+        assert np.array_equal(v[a][inner], ref[inner]), a
+    # the reference layout rebuilt on download
+    vr = g.velocity
     sl = (slice(0, p.nx + 1), slice(0, p.ny + 1), slice(0, p.nz + 1))
     for a, ref in ((0, st.vx), (1, st.vy), (2, st.vz)):
-        assert np.array_equal(v[a][sl], ref[sl]), a
+        assert np.array_equal(vr[a][sl], ref[sl]), a
     g.close()
 
 

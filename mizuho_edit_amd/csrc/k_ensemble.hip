@@ -138,7 +138,7 @@ __global__ void ke_tol(const EnsMember* __restrict__ Ms, int n, double rtol, dou
     if (m >= n) return;
     const EnsMember& M = Ms[m];
     EnsState& s = *M.st;
-    s.bnorm = sqrt(M.result[S_BN]);
+    s.bnorm = sqrt(M.result[S_T0 + 1]);
     s.tol = fmax(rtol * s.bnorm, atol);
 }
 
@@ -255,18 +255,12 @@ __global__ void __launch_bounds__(256) ke_extrapolate(ENS_ARGS)
     ENS_END
 }
 
-__global__ void __launch_bounds__(256) ke_bnorm(ENS_ARGS)
-{
-    ENS_TILES(G_ALL)
-    t_bnorm(ens_lv64(M), M.p, M.fsrc, M.partials, nullptr, nullptr, T.b, M.nb1);
-    ENS_END
-}
-
-__global__ void __launch_bounds__(256) ke_true_resid(ENS_ARGS)
+// bn: the first stage also sums ||D^-1 b||^2 (into the second value)
+__global__ void __launch_bounds__(256) ke_true_resid(ENS_ARGS, int bn)
 {
     ENS_TILES(G_STAGE)
     t_true_resid(ens_lv64(M), M.p, M.fsrc, (double*)nullptr, (double*)nullptr, M.v[EV_R], M.s32, M.partials,
-                 nullptr, nullptr, T.b, M.nb1);
+                 nullptr, nullptr, T.b, M.nb1, bn != 0);
     ENS_END
 }
 
@@ -660,14 +654,13 @@ int sched_solve(wpf_ensemble* e)
     ECHK(ens_perm(e->stream, e->d_mem, e->ts, e->ns, e->grid, e->op16 ? PERM_F16 : PERM_F32));
     ELAUNCH(ke_dirichlet, nm, 256, 0, e->d_mem);
     if (e->extrap) ELAUNCH(ke_extrapolate, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
-    ELAUNCH(ke_bnorm, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
-    ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 1, (int)S_BN, (int)G_ALL);
-    ELAUNCH(ke_tol, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.rtol, o.atol);
     for (int l = 1; l < e->nlev; l++) ELAUNCH(ke_coarsen, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l);
     const float w1 = (float)S.mo.w1;
     for (int stage = 0; stage < e->eo.stages; stage++) {
-        ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
+        // the first stage's residual pass also gives ||D^-1 b|| and the tolerance
+        ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t1, e->n1, stage == 0 ? 1 : 0);
         ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 2, (int)S_T0, (int)G_STAGE);
+        if (stage == 0) ELAUNCH(ke_tol, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.rtol, o.atol);
         ELAUNCH(ke_outer, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.inner_rtol, o.inner_maxit, o.maxit);
         // pcg_core
         int rz_cur = S_RZ0, rz_prev = S_RZ1;
@@ -700,7 +693,7 @@ int sched_solve(wpf_ensemble* e)
         }
         ELAUNCH(ke_add, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
     }
-    ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
+    ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t1, e->n1, 0);
     ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 2, (int)S_T0, (int)G_STAGE);
     ELAUNCH(ke_final, S.mblocks(), S.mthreads(), 0, e->d_mem, nm);
     return WPF_OK;

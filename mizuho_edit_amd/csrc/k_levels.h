@@ -100,14 +100,34 @@ __device__ __forceinline__ bool is_fixed(const LV<R>& L, int64_t c, bool fine)
 // true residual r = f - A x in FP64 (fine level, x holds Dirichlet values),
 // reduces ||invD r||^2 into result[0]; optionally also writes z = invD r and
 // reduces (r, z) into result[1]; optionally writes a scaled FP32 copy.
+// With bn (and no z) result[1] is instead ||D^-1 b||^2 of the reference
+// system, the per-cell values and partition of t_bnorm (so the same sum),
+// which spares the solver a separate pass over the operator.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double bnorm_cell(const LV<double>& L, const double* __restrict__ x,
+                                             const double* __restrict__ f, int64_t c)
+{
+    if (L.mask[c] & WPF_MASK_FIXED) return x[c];
+    auto fx = [&](int64_t m) { return (L.mask[m] & WPF_MASK_FIXED) ? x[m] : 0.0; };
+    // masks outside the grid are never read: their coefficient is 0
+    double s = 0.0;
+    if (L.tx[c] != 0.0) s += L.tx[c] * fx(c + 1);
+    if (L.tx[c - 1] != 0.0) s += L.tx[c - 1] * fx(c - 1);
+    if (L.ty[c] != 0.0) s += L.ty[c] * fx(c + L.nx);
+    if (L.ty[c - L.nx] != 0.0) s += L.ty[c - L.nx] * fx(c - L.nx);
+    if (L.tz[c] != 0.0) s += L.tz[c] * fx(c + L.plane);
+    if (L.tz[c - L.plane] != 0.0) s += L.tz[c - L.plane] * fx(c - L.plane);
+    return ((f ? f[c] : 0.0) + s) * L.invD[c];
+}
+
 static __device__ __forceinline__ void t_true_resid(LV<double> L, const double* __restrict__ x,
                                                          const double* __restrict__ f,
                                                          double* __restrict__ r,
                                                          double* __restrict__ z, float* __restrict__ r32,
                                                          double s32, double* __restrict__ partials,
                                                          unsigned* __restrict__ counter,
-                                                         double* __restrict__ result, int b_, int nb_)
+                                                         double* __restrict__ result, int b_, int nb_,
+                                                         bool bn = false)
 {
     const Chunk ch = my_chunk_at(L.n, b_, nb_);
     double acc[2] = {0.0, 0.0};
@@ -122,6 +142,9 @@ static __device__ __forceinline__ void t_true_resid(LV<double> L, const double* 
         if (z) {
             z[c] = zv;
             acc[1] += rv * zv;
+        } else if (bn) {
+            const double v = bnorm_cell(L, x, f, c);
+            acc[1] += v * v;
         }
         if (r32) r32[c] = (float)(rv * s32);
     }
@@ -133,9 +156,9 @@ static __global__ void __launch_bounds__(kThreads) k_true_resid(LV<double> L, co
                                                          double* __restrict__ z, float* __restrict__ r32,
                                                          double s32, double* __restrict__ partials,
                                                          unsigned* __restrict__ counter,
-                                                         double* __restrict__ result)
+                                                         double* __restrict__ result, int bn)
 {
-    t_true_resid(L, x, f, r, z, r32, s32, partials, counter, result, (int)blockIdx.x, (int)gridDim.x);
+    t_true_resid(L, x, f, r, z, r32, s32, partials, counter, result, (int)blockIdx.x, (int)gridDim.x, bn != 0);
 }
 
 // ||D^-1 b|| of the reference system: Dirichlet rows carry p_D; free rows
@@ -149,21 +172,7 @@ static __device__ __forceinline__ void t_bnorm(LV<double> L, const double* __res
     const Chunk ch = my_chunk_at(L.n, b_, nb_);
     double acc[1] = {0.0};
     for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        double v;
-        if (L.mask[c] & WPF_MASK_FIXED) {
-            v = x[c];
-        } else {
-            auto fx = [&](int64_t m) { return (L.mask[m] & WPF_MASK_FIXED) ? x[m] : 0.0; };
-            // masks outside the grid are never read: their coefficient is 0
-            double s = 0.0;
-            if (L.tx[c] != 0.0) s += L.tx[c] * fx(c + 1);
-            if (L.tx[c - 1] != 0.0) s += L.tx[c - 1] * fx(c - 1);
-            if (L.ty[c] != 0.0) s += L.ty[c] * fx(c + L.nx);
-            if (L.ty[c - L.nx] != 0.0) s += L.ty[c - L.nx] * fx(c - L.nx);
-            if (L.tz[c] != 0.0) s += L.tz[c] * fx(c + L.plane);
-            if (L.tz[c - L.plane] != 0.0) s += L.tz[c - L.plane] * fx(c - L.plane);
-            v = ((f ? f[c] : 0.0) + s) * L.invD[c];
-        }
+        const double v = bnorm_cell(L, x, f, c);
         acc[0] += v * v;
     }
     block_reduce_store<1>(acc, partials, counter, result, nb_, b_);

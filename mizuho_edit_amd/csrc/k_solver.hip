@@ -535,6 +535,8 @@ static int ensure_ws(wpf_ctx* c, bool f32)
 }
 
 // Dirichlet values into x, optional extrapolated guess, ||D^-1 b||.
+// Dirichlet values, the extrapolated or saved guess, and ||D^-1 b|| unless
+// the caller takes it from its first true residual (bnorm == nullptr).
 static int prepare(wpf_ctx* c, const wpf_solver_opts& o, double* bnorm)
 {
     if (c->ndir > 0) {
@@ -558,6 +560,7 @@ static int prepare(wpf_ctx* c, const wpf_solver_opts& o, double* bnorm)
     } else {
         c->extrapolated = false;
     }
+    if (!bnorm) return WPF_OK;
     MGLevel<double> F;
     F.g = c->g; F.tx = c->tx; F.ty = c->ty; F.tz = c->tz; F.invD = c->invD;
     LV<double> L = view(F, c->mask, true);
@@ -603,7 +606,7 @@ static int true_resid64(wpf_ctx* c, double* rn)
     LV<double> L = view(F0, c->mask, true);
     const int64_t n = c->g.n;
     LAUNCH1D(k_true_resid, n, L, (const double*)c->p.p, (const double*)c->fsrc.p, c->r.p,
-             (double*)nullptr, (float*)nullptr, 0.0, RED(S_T0));
+             (double*)nullptr, (float*)nullptr, 0.0, RED(S_T0), 0);
     FINALIZE(blocks_for(n), 2, S_T0);
     int rc = read_results(c, S_T0, 1);
     if (rc) return rc;
@@ -837,7 +840,7 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
         {
             const int kid_ = WPF_KERNEL_TRUE_RESID;
             LAUNCH1D(k_true_resid, n, L, (const double*)x, (const double*)c->fsrc.p, F.r,
-                     (double*)nullptr, (float*)nullptr, 0.0, RED(S_T0));
+                     (double*)nullptr, (float*)nullptr, 0.0, RED(S_T0), 0);
         }
         FINALIZE(blocks_for(n), 2, S_T0);
         if ((rc = read_results(c, S_T0, 1))) return rc;
@@ -873,8 +876,9 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
     const int64_t n = c->g.n;
     int rc = ensure_ws(c, true);
     if (rc) return rc;
-    double bnorm;
-    if ((rc = prepare(c, o, &bnorm))) return rc;
+    // ||D^-1 b|| comes with the first true residual
+    double bnorm = -1.0, tol = 0.0;
+    if ((rc = prepare(c, o, nullptr))) return rc;
     int learned = 0;
     if ((rc = start_stage(c, o, &learned, false))) return rc;
     // projected initial guess: needs earlier solution changes and the vector
@@ -892,7 +896,6 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
     const bool gated = want_net && c->extrapolated;
     const bool use_net = want_net && (!gated || gate_use(c));
     bool learned_pending = use_net;
-    const double tol = std::max(o.rtol * bnorm, o.atol);
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
     LV<double> L = view(F0, c->mask, true);
@@ -938,11 +941,15 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         {
             const int kid_ = WPF_KERNEL_TRUE_RESID;
             LAUNCH1D(k_true_resid, n, L, (const double*)x, (const double*)c->fsrc.p, (double*)nullptr,
-                     (double*)nullptr, F.r, s, RED(S_T0));
+                     (double*)nullptr, F.r, s, RED(S_T0), bnorm < 0.0 ? 1 : 0);
         }
         FINALIZE(blocks_for(n), 2, S_T0);
-        if ((rc = read_results(c, S_T0, 1))) return rc;
+        if ((rc = read_results(c, S_T0, 2))) return rc;
         rn = std::sqrt(c->red.host[S_T0]);
+        if (bnorm < 0.0) {
+            bnorm = std::sqrt(c->red.host[S_T0 + 1]);
+            tol = std::max(o.rtol * bnorm, o.atol);
+        }
         if (!std::isfinite(rn)) {
             set_error("pressure residual is not finite");
             return WPF_ERR_NAN;

@@ -44,6 +44,7 @@ struct wpf_ensemble {
     wpf::EnsTile* tl[wpf::kEnsMaxLevels] = {};
     int nl[wpf::kEnsMaxLevels] = {};
     size_t coarsest_lds = 0;
+    size_t up_lds = 0;  // LDS of the staged fused update (0: the global-load form)
     hipGraphExec_t gA = nullptr, gB = nullptr;
 };
 
@@ -289,16 +290,18 @@ __global__ void __launch_bounds__(256) ke_update(ENS_ARGS, int rp, int rr, int n
 }
 
 // the fused update + pre-smoothing step 1 (k_vupsweep); y into EV_T
-template <typename C>
+template <typename C, bool LDSY>
 __global__ void __launch_bounds__(256) ke_upsweep(ENS_ARGS, int rp, int rr, int rr2, int num, float w1, float a,
                                                   float w)
 {
+    extern __shared__ char smem_raw[];
     ENS_TILES(G_INNER)
     const C *tx, *ty, *tz;
     ens_faces<C>(M, tx, ty, tz);
     double acc[2];
-    t_vupsweep<float, float, C, 4>(M.Z, T.b, M.result, num, S_PQ, w1, a, w, M.v[EV_E], M.v[rp], M.v[rr],
-                                   M.v[EV_Q], M.v[rr2], tx, ty, tz, M.invD32, M.v[EV_T], acc);
+    t_vupsweep<float, float, C, 4, LDSY>(M.Z, T.b, M.result, num, S_PQ, w1, a, w, M.v[EV_E], M.v[rp], M.v[rr],
+                                         M.v[EV_Q], M.v[rr2], tx, ty, tz, M.invD32, M.v[EV_T], acc,
+                                         reinterpret_cast<float*>(smem_raw));
     block_reduce_store<2>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);
     ENS_END
 }
@@ -678,8 +681,12 @@ int sched_solve(wpf_ensemble* e)
             if (fuse) {
                 double a1, w1s;
                 step_coef(S.mo, 1, &a1, &w1s);
-                ELAUNCH((ke_upsweep<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, pc, rr, rr2, rz_cur, w1,
-                        (float)a1, (float)w1s);
+                if (e->up_lds)
+                    ELAUNCH((ke_upsweep<C, true>), e->grid, 256, e->up_lds, e->d_mem, e->tv, e->nv, pc, rr, rr2,
+                            rz_cur, w1, (float)a1, (float)w1s);
+                else
+                    ELAUNCH((ke_upsweep<C, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, pc, rr, rr2, rz_cur,
+                            w1, (float)a1, (float)w1s);
                 ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, 2, (int)S_RN, (int)G_INNER);
                 std::swap(rr, rr2);
             } else {
@@ -828,6 +835,11 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
         return fail(WPF_ERR_ARG);
     }
     e->coarsest_lds = lds;
+    {
+        int nxmax = 0;
+        for (wpf_ctx* c : e->mem) nxmax = std::max(nxmax, c->g.nx);
+        e->up_lds = vupsweep_lds_fits<float, 4>(nxmax) ? vupsweep_lds<float, 4>(nxmax) : 0;
+    }
     if (hipFuncSetAttribute((const void*)ke_coarsest, hipFuncAttributeMaxDynamicSharedMemorySize,
                             150 * 1024) != hipSuccess) {
         set_error("hipFuncSetAttribute failed");

@@ -455,7 +455,11 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
             step_coef(o, 1, &a1, &w1s);
             constexpr int V = vwidth<R>();
             prof_mark(c, WPF_KERNEL_CG_UPDATE, true);
-            hipLaunchKernelGGL((k_vupsweep<R, X, C, V>), dim3(vz_blocks(F.V)), dim3(256), 0, c->stream, F.V,
+            // y-neighbour rows staged in LDS where the rows fit
+            const bool lds = vupsweep_lds_fits<R, V>(F.g.nx);
+            const size_t lds_bytes = lds ? vupsweep_lds<R, V>(F.g.nx) : 0;
+            auto* const kern = lds ? k_vupsweep<R, X, C, V, true> : k_vupsweep<R, X, C, V, false>;
+            hipLaunchKernelGGL(kern, dim3(vz_blocks(F.V)), dim3(256), lds_bytes, c->stream, F.V,
                                (const double*)c->red.result, rz_cur, (int)S_PQ, w, (R)a1, (R)w1s, x, (const R*)pc,
                                (const R*)F.r, (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, F.T,
                                c->red.partials);

@@ -94,6 +94,21 @@ __device__ __forceinline__ bool vz_at(const VZ& Z, int b, int64_t& q, int& k0, i
     return act;
 }
 
+// first plane cell of block b's chunk (as vz_at maps it)
+template <int V>
+__device__ __forceinline__ int64_t vz_chunk0(const VZ& Z, int b)
+{
+    const int nb = vz_blocks(Z);
+    const int lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    return (int64_t)(lb % Z.nchunk) * 256 * V;
+}
+// dynamic LDS bytes of the LDS-staged fused update; rows longer than the
+// default 64 KiB allows take the form with global y-neighbour loads
+template <typename R, int V>
+inline size_t vupsweep_lds(int nx) { return sizeof(R) * 2 * (size_t)(256 * V + 2 * nx); }
+template <typename R, int V>
+inline bool vupsweep_lds_fits(int nx) { return vupsweep_lds<R, V>(nx) <= 65536; }
+
 // value from the lane below / above (lanes 0 / 63 get their own back)
 template <typename R>
 __device__ __forceinline__ R from_left(R v) { return __shfl_up(v, 1, 64); }
@@ -239,14 +254,20 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
 //     from x_{-1} = 0, with z0 formed on the fly for the neighbours too).
 // Reduces ||invD r'||^2 and (r', z0) like k_update.  The values are those
 // of k_update followed by k_vsweep; only the reduction's partition differs.
-template <typename R, typename X, typename C, int V>
+//
+// LDSY: the y-1 / y+1 rows come from LDS instead of global loads.  Each
+// plane's z0 of the block's chunk and of the row below and above it is
+// staged once in zs (two buffers of 256*V + 2*nx values, one barrier per
+// plane); the neighbours' z0 are the values the owning threads computed, so
+// the result is the same.
+template <typename R, typename X, typename C, int V, bool LDSY = false>
 __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __restrict__ res, int num,
                                            int den, R w1, R a, R w, X* __restrict__ e,
                                            const R* __restrict__ p, const R* __restrict__ r,
                                            const R* __restrict__ q, R* __restrict__ r2,
                                            const C* __restrict__ tx, const C* __restrict__ ty,
                                            const C* __restrict__ tz, const R* __restrict__ invD, R* y,
-                                           double (&acc)[2])
+                                           double (&acc)[2], R* zs = nullptr)
 {
     const R al = (R)(res[num] / res[den]);
     int64_t qi;
@@ -254,10 +275,12 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
     const bool act = vz_at<V>(Z, bi, qi, k0, k1);
     const int lane = threadIdx.x & 63;
     const int64_t nx = Z.nx, pl = Z.plane;
+    const int64_t q0 = vz_chunk0<V>(Z, bi);         // first plane cell of the block's chunk
+    const int zw = 256 * V + 2 * Z.nx, nxv = Z.nx / V;  // LDS values per plane, vectors per row
     acc[0] = acc[1] = 0.0;
-    // r' and z0 of the V cells at m
-    auto rz = [&](int64_t m, R(&rn)[V], R(&z0)[V]) {
-        R rv[V], qv[V], dv[V];
+    // r' and z0 of the V cells at m (and their invD)
+    auto rz = [&](int64_t m, R(&rn)[V], R(&z0)[V], R(&dv)[V]) {
+        R rv[V], qv[V];
         vld<V>(r + m, rv);
         vld<V>(q + m, qv);
         vld<V>(invD + m, dv);
@@ -268,28 +291,51 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
         }
     };
     auto z1 = [&](int64_t m) { return w1 * ((r[m] - al * q[m]) * invD[m]); };
-    R rm[V], r0[V], rp[V], zm[V], z0[V], zp[V], tzm[V];
+    R rm[V], r0[V], rp[V], zm[V], z0[V], zp[V], tzm[V], dm[V], di[V], dp[V];
     int64_t c = qi + pl * k0;
-    rz(c - pl, rm, zm);
-    rz(c, r0, z0);
+    rz(c - pl, rm, zm, dm);
+    rz(c, r0, z0, di);
     vldc<V>(tz + c - pl, tzm);
     for (int k = k0; k < k1; k++, c += pl) {
-        rz(c + pl, rp, zp);
-        R ryp[V], zyp[V], rym[V], zym[V], txv[V], tyv[V], tym[V], tzv[V], di[V], pv[V];
-        rz(c + nx, ryp, zyp);
-        rz(c - nx, rym, zym);
+        rz(c + pl, rp, zp, dp);
+        R zyp[V], zym[V], txv[V], tyv[V], tym[V], tzv[V], pv[V];
         vldc<V>(tx + c, txv);
         vldc<V>(ty + c, tyv);
         vldc<V>(ty + c - nx, tym);
         vldc<V>(tz + c, tzv);
-        vld<V>(invD + c, di);
         vld<V>(p + c, pv);
         R zl = from_left(z0[V - 1]), zr = from_right(z0[0]), txl = from_left(txv[V - 1]);
-        if (lane == 0) {
-            zl = z1(c - 1);
-            txl = (R)tx[c - 1];
+        if (lane == 0) txl = (R)tx[c - 1];
+        if (LDSY) {
+            R* buf = zs + (k & 1) * zw;
+            const int o = (int)nx + (int)threadIdx.x * V;
+#pragma unroll
+            for (int h = 0; h < V; h++) buf[o + h] = z0[h];
+            // the rows below and above the chunk (clamped into the grid's
+            // planes and pads; clamped values only meet zero coefficients)
+            for (int hv = threadIdx.x; hv < 2 * nxv; hv += 256) {
+                const int lo = hv < nxv ? hv * V : (int)nx + 256 * V + (hv - nxv) * V;
+                const int64_t m = min(q0 - nx + lo, pl - V);
+                R rn[V], zz[V], dd[V];
+                rz(pl * k + m, rn, zz, dd);
+#pragma unroll
+                for (int h = 0; h < V; h++) buf[lo + h] = zz[h];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < V; h++) {
+                zyp[h] = buf[o + (int)nx + h];
+                zym[h] = buf[o - (int)nx + h];
+            }
+            if (lane == 0) zl = buf[o - 1];
+            if (lane == 63) zr = buf[o + V];
+        } else {
+            R ryp[V], rym[V], dy[V];
+            rz(c + nx, ryp, zyp, dy);
+            rz(c - nx, rym, zym, dy);
+            if (lane == 0) zl = z1(c - 1);
+            if (lane == 63) zr = z1(c + V);
         }
-        if (lane == 63) zr = z1(c + V);
         X ev[V];
         vld<V>(e + c, ev);
         R yv[V];
@@ -320,13 +366,16 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
             r0[h] = rp[h];
             zm[h] = z0[h];
             z0[h] = zp[h];
+            di[h] = dp[h];
             tzm[h] = tzv[h];
         }
     }
+    // the buffers are reused by the next tile of a persistent launch
+    if (LDSY) __syncthreads();
     if (!act) acc[0] = acc[1] = 0.0;
 }
 
-template <typename R, typename X, typename C, int V>
+template <typename R, typename X, typename C, int V, bool LDSY>
 __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict__ res, int num, int den, R w1,
                                                   R a, R w, X* __restrict__ e, const R* __restrict__ p,
                                                   const R* __restrict__ r, const R* __restrict__ q,
@@ -336,7 +385,9 @@ __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict
                                                   double* __restrict__ partials)
 {
     double acc[2];
-    t_vupsweep<R, X, C, V>(Z, blockIdx.x, res, num, den, w1, a, w, e, p, r, q, r2, tx, ty, tz, invD, y, acc);
+    extern __shared__ char smem_raw[];
+    t_vupsweep<R, X, C, V, LDSY>(Z, blockIdx.x, res, num, den, w1, a, w, e, p, r, q, r2, tx, ty, tz, invD, y, acc,
+                                 reinterpret_cast<R*>(smem_raw));
     block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
 }
 

@@ -267,11 +267,37 @@ static int sweep(wpf_ctx* c, const Fine<R, C>& F, double a, double w, const R* x
 
 // Tile depth of the vector kernels.  *zc > 0: fixed; 0: the default, 8
 // planes (4 below 64 planes), which keeps results independent of timing;
-// < 0: autotune, i.e. time one smoothing sweep over the grid for each
-// candidate number of planes per tile and keep the fastest.  The choice is
+// < 0: autotune, i.e. time the vector kernels of one CG iteration (two
+// smoothing steps, the fused update, the direction update with its matvec)
+// for each candidate number of planes per tile and keep the fastest.  The choice is
 // cached per context and precision in *zc.  The partition of the
 // reductions follows the tile shape, so a different depth changes results
 // in the last bits.
+template <typename R, typename C>
+static int dir_mv(wpf_ctx* c, const Fine<R, C>& F, const R* z, const R* pold, R* pnew, int num, int den,
+                  int first);
+
+// The CG update fused with the first pre-smoothing step (k_vupsweep): x +=
+// alpha p, r2 = r - alpha q, y (into F.T) from z0 = w1 invD r2; alpha =
+// result[num] / result[den].  The y-neighbour rows are staged in LDS where
+// the rows fit.
+template <typename R, typename X, typename C>
+static int upsweep(wpf_ctx* c, const Fine<R, C>& F, int num, int den, R w1, double a1, double w1s, X* x,
+                   const R* p)
+{
+    constexpr int V = vwidth<R>();
+    const bool lds = vupsweep_lds_fits<R, V>(F.g.nx);
+    const size_t lds_bytes = lds ? vupsweep_lds<R, V>(F.g.nx) : 0;
+    auto* const kern = lds ? k_vupsweep<R, X, C, V, true> : k_vupsweep<R, X, C, V, false>;
+    prof_mark(c, WPF_KERNEL_CG_UPDATE, true);
+    hipLaunchKernelGGL(kern, dim3(vz_blocks(F.V)), dim3(256), lds_bytes, c->stream, F.V,
+                       (const double*)c->red.result, num, den, w1, (R)a1, (R)w1s, x, p, (const R*)F.r,
+                       (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, F.T, c->red.partials);
+    prof_mark(c, WPF_KERNEL_CG_UPDATE, false);
+    WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
 static int default_zc(const Grid& g) { return g.nz >= 64 ? 8 : 4; }
 
 template <typename R, typename C>
@@ -295,14 +321,24 @@ static int tune_shape(wpf_ctx* c, Fine<R, C>& F, int* zc)
     int best = 8;
     float best_ms = 1e30f;
     int rc = WPF_OK;
+    // the solver's vectors are scratch before its first stage; the fused
+    // update's step length reads a slot holding 1
+    const double one = 1.0;
+    WPF_HIP(hipStreamSynchronize(c->stream));
+    WPF_HIP(hipMemcpy(c->red.result + kTuneSlot, &one, sizeof(double), hipMemcpyHostToDevice));
+    auto round = [&]() {
+        int r = sweep<R, C, false, false, false>(c, F, 0.0, 0.5, F.Z, nullptr, none, F.T, S_T0);
+        if (!r) r = sweep<R, C, false, false, false>(c, F, 0.0, 0.5, F.T, nullptr, none, F.Z, S_T0);
+        if (!r) r = upsweep<R, R, C>(c, F, kTuneSlot, kTuneSlot, (R)0.5, 0.0, 0.5, F.pa, F.pb);
+        if (!r) r = dir_mv<R, C>(c, F, F.Z, F.pa, F.pb, kTuneSlot, kTuneSlot, 1);
+        return r;
+    };
     for (int zcv : cand) {
         if (zcv > 2 * F.g.nz) break;
         set_shape(F, zcv);
-        // scratch vectors only: Z is read, T written
-        if ((rc = sweep<R, C, false, false, false>(c, F, 0.0, 0.5, F.Z, nullptr, none, F.T, S_T0))) break;
+        if ((rc = round())) break;
         (void)hipEventRecord(e0, c->stream);
-        for (int r = 0; r < 3 && !rc; r++)
-            rc = sweep<R, C, false, false, false>(c, F, 0.0, 0.5, F.Z, nullptr, none, F.T, S_T0);
+        for (int r = 0; r < 2 && !rc; r++) rc = round();
         (void)hipEventRecord(e1, c->stream);
         if (rc || hipEventSynchronize(e1) != hipSuccess) {
             if (!rc) rc = WPF_ERR_HIP;
@@ -453,18 +489,7 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
             // update + pre-smoothing step 1 (into T) in one pass
             double a1, w1s;
             step_coef(o, 1, &a1, &w1s);
-            constexpr int V = vwidth<R>();
-            prof_mark(c, WPF_KERNEL_CG_UPDATE, true);
-            // y-neighbour rows staged in LDS where the rows fit
-            const bool lds = vupsweep_lds_fits<R, V>(F.g.nx);
-            const size_t lds_bytes = lds ? vupsweep_lds<R, V>(F.g.nx) : 0;
-            auto* const kern = lds ? k_vupsweep<R, X, C, V, true> : k_vupsweep<R, X, C, V, false>;
-            hipLaunchKernelGGL(kern, dim3(vz_blocks(F.V)), dim3(256), lds_bytes, c->stream, F.V,
-                               (const double*)c->red.result, rz_cur, (int)S_PQ, w, (R)a1, (R)w1s, x, (const R*)pc,
-                               (const R*)F.r, (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, F.T,
-                               c->red.partials);
-            prof_mark(c, WPF_KERNEL_CG_UPDATE, false);
-            WPF_HIP(hipGetLastError());
+            if (int rc = upsweep<R, X, C>(c, F, rz_cur, S_PQ, w, a1, w1s, x, pc)) return rc;
             FINALIZE(vz_blocks(F.V), 2, S_RN);
             std::swap(F.r, F.r2);
         } else {

@@ -681,7 +681,8 @@ static int start_stage(wpf_ctx* c, const wpf_solver_opts& o, int* learned, bool 
     FINALIZE(blocks_for(n), 1, S_LR);
     if ((rc = read_results(c, 0, S_LR + 1))) return rc;
     const double alpha = c->red.host[S_LR] / c->red.host[S_PQ];
-    if (!std::isfinite(alpha) || !(alpha > 0.0)) return WPF_OK;
+    // either sign: the network's loss does not fix the sign of its output
+    if (!(c->red.host[S_PQ] > 0.0) || !std::isfinite(alpha) || alpha == 0.0) return WPF_OK;
     hipLaunchKernelGGL(k_axpy64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n, alpha,
                        (const double*)d, c->p.p);
     WPF_HIP(hipGetLastError());
@@ -737,8 +738,10 @@ __global__ void __launch_bounds__(kThreads) k_lin_start(int64_t n, const double*
                                                         const float* __restrict__ q, float* __restrict__ e,
                                                         float* __restrict__ r)
 {
+    // the network's loss does not fix the sign of its output: the line
+    // search takes either sign (the A-norm drops by (d, r)^2 / (d, A d))
     double al = res[num] / res[den];
-    if (!(al > 0.0) || !(al < 1e300)) al = 0.0;
+    if (!(res[den] > 0.0) || !(fabs(al) < 1e300)) al = 0.0;
     const float a = (float)al;
     const Chunk ch = my_chunk(n);
     for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {

@@ -159,9 +159,10 @@ def make_training_set(nreal: int = 8, grid=(64, 64, 64), nsteps: int = 40, every
 
 
 def compare(case, net: Optional["PCNet"], nsteps: int, skip: int = 3, solver: str = "mixed",
-            inner_rtol: float = 1e-4):
-    """Run `case` for nsteps with the extrapolated start, without and with
-    the learned stage.  Returns per-run iterations (steps after `skip`),
+            inner_rtol: float = 1e-4, extrapolate: bool = True):
+    """Run `case` for nsteps from the extrapolated start (or, with
+    extrapolate=False, from the previous pressure), without and with the
+    learned stage.  Returns per-run iterations (steps after `skip`),
     accepted learned stages, wall seconds and the final pressures."""
     import time
     from .wpfaw3d import WPFAW3D, solver_opts
@@ -170,7 +171,7 @@ def compare(case, net: Optional["PCNet"], nsteps: int, skip: int = 3, solver: st
         g = WPFAW3D(case)
         if use:
             g.attach_net(net)
-        flags = _lib.FLAG_EXTRAPOLATE | (_lib.FLAG_LEARNED_WARM if use else 0)
+        flags = (_lib.FLAG_EXTRAPOLATE if extrapolate else 0) | (_lib.FLAG_LEARNED_WARM if use else 0)
         o = solver_opts(solver, rtol=1e-10, inner_rtol=inner_rtol, flags=flags)
         its, acc, wall = [], 0, 0.0
         for s in range(nsteps):
@@ -201,6 +202,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-extrapolate", action="store_true",
+                    help="eval: start each solve from the previous pressure, not the extrapolated one")
     a = ap.parse_args(argv)
     n = a.grid
     if a.mode == "train":
@@ -215,7 +218,8 @@ def main(argv=None):
         net = PCNet()
         net.load(a.weights)
         case = io.workload("plain-small", seed=a.seed + 1000, grid=(n, n, n), corr_len=4.0)
-        r = compare(case, net, a.nsteps)
+        r = compare(case, net, a.nsteps, extrapolate=not a.no_extrapolate)
+        r["grid"], r["extrapolate"] = n, not a.no_extrapolate
         p0, p1 = r["plain"].pop("pressure"), r["net"].pop("pressure")
         r["p_rel_diff"] = float(np.abs(p1 - p0).max() / np.abs(p0).max())
         print(json.dumps(r), flush=True)

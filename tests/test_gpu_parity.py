@@ -193,6 +193,55 @@ def test_pressure_solve_row_lengths(gpu, grid, solver, flags):
     g.close()
 
 
+@pytest.mark.parametrize("solver,flags", [("mgcg", CHEB), ("mixed", CHEB | OP16)])
+def test_pressure_solve_long_rows(gpu, solver, flags):
+    """Rows too long for the LDS staging of the fused update's y-neighbours
+    (FP32: nx > 3584, FP64: nx > 1792) take its global-load form."""
+    case, st = _state(grid=(3600, 4, 4), steps=1)
+    g = _gpu_from(case, st)
+    p0 = st.pressure.copy()
+    g.upload("pressure", p0 * 0.5)
+    g.calcPermeability()
+    info = g.solvePressure(solver_opts(solver, rtol=TIGHT, maxit=20000, flags=flags))
+    assert info["converged"] == 1
+    st.calc_permeability()
+    st.pressure[...] = p0 * 0.5
+    st.solve_pressure(rtol=TIGHT, maxit=200000)
+    assert np.abs(g.pressure - st.pressure).max() <= 1e-10 * np.abs(st.pressure).max()
+    g.close()
+
+
+def test_tile_depths_solve_to_the_same_answer(gpu):
+    """The default, an autotuned and a fixed shallow tile depth partition the
+    reductions differently; every run still meets the FP64 stopping test, so
+    the pressures agree to the solver tolerance."""
+    case = io.workload("plain-small", seed=7, grid=(64, 32, 32))
+    flags = CHEB | OP16 | _lib.FLAG_PROJECT | _lib.FLAG_EXTRAPOLATE
+    runs = []
+    for depth in (0, -1, 2):
+        g = WPFAW3D(case, opts=solver_opts("mixed", rtol=1e-10, flags=flags))
+        g.set_tile_depth(depth)
+        for _ in range(6):
+            assert g.step()["converged"] == 1
+        runs.append(g.pressure)
+        g.close()
+    ref = np.abs(runs[0]).max()
+    for p in runs[1:]:
+        assert np.abs(p - runs[0]).max() <= 1e-8 * ref
+
+
+def test_stream_probe_reports_a_bandwidth(gpu):
+    import ctypes
+    L = _lib.lib()
+    us = ctypes.c_double()
+    n = 1 << 22
+    _lib.check(L.wpf_stream_probe(0, 4, 1, n, 5, ctypes.byref(us)), "wpf_stream_probe")
+    gbps = 4.0 * n * 5 / (us.value * 1e-6) / 1e9
+    assert 100.0 < gbps < 20000.0, gbps
+    assert L.wpf_stream_probe(0, 11, 1, n, 5, ctypes.byref(us)) == _lib.WPF_ERR_ARG
+    assert L.wpf_stream_probe(0, 2, 1, n + 2, 5, ctypes.byref(us)) == _lib.WPF_ERR_ARG
+
+
 def test_fused_step_matches_procedures(gpu):
     case, st = _state(steps=3)
     a = _gpu_from(case, st)

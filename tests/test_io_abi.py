@@ -131,6 +131,10 @@ def test_no_device_is_an_error_not_a_fallback():
     with pytest.raises(_lib.WpfError) as e:
         WPFAW3D(case)
     assert e.value.status == _lib.WPF_ERR_NODEV
+    # the bandwidth probe too: an error, never a host-side number
+    us = ctypes.c_double(-1.0)
+    assert _lib.lib().wpf_stream_probe(0, 2, 1, 1 << 16, 3, ctypes.byref(us)) == _lib.WPF_ERR_NODEV
+    assert us.value == -1.0
 
 
 def _write_case(tmp_path, grid=(10, 6, 5), seed=4, with_field=True):

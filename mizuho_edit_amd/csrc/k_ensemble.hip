@@ -355,15 +355,20 @@ __global__ void __launch_bounds__(256) ke_fresid_restrict(ENS_ARGS, int rb, int 
 }
 
 // ---- coarse levels, 1-D tiles of level l ---------------------------------------
-__global__ void __launch_bounds__(256) ke_coarsen(ENS_ARGS, int l)
+// h16: level 1 from the fine level's FP16-stored faces
+__global__ void __launch_bounds__(256) ke_coarsen(ENS_ARGS, int l, int h16)
 {
     ENS_TILES(G_ALL)
     const EnsLevel& C = M.lv[l];
-    if (l == 1)
-        t_coarsen<float, true>(ens_lv32(M, 0), C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D, C.invD, T.b,
+    const LV<float> F = ens_lv32(M, l - 1);
+    if (l == 1 && h16)
+        t_coarsen<float, true, _Float16>(F, M.htx, M.hty, M.htz, C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D,
+                                         C.invD, T.b, C.nb);
+    else if (l == 1)
+        t_coarsen<float, true>(F, F.tx, F.ty, F.tz, C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D, C.invD, T.b,
                                C.nb);
     else
-        t_coarsen<float, false>(ens_lv32(M, l - 1), C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D, C.invD,
+        t_coarsen<float, false>(F, F.tx, F.ty, F.tz, C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D, C.invD,
                                 T.b, C.nb);
     ENS_END
 }
@@ -657,7 +662,8 @@ int sched_solve(wpf_ensemble* e)
     ECHK(ens_perm(e->stream, e->d_mem, e->ts, e->ns, e->grid, e->op16 ? PERM_F16 : PERM_F32));
     ELAUNCH(ke_dirichlet, nm, 256, 0, e->d_mem);
     if (e->extrap) ELAUNCH(ke_extrapolate, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
-    for (int l = 1; l < e->nlev; l++) ELAUNCH(ke_coarsen, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l);
+    for (int l = 1; l < e->nlev; l++)
+        ELAUNCH(ke_coarsen, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, e->op16 ? 1 : 0);
     const float w1 = (float)S.mo.w1;
     for (int stage = 0; stage < e->eo.stages; stage++) {
         // the first stage's residual pass also gives ||D^-1 b|| and the tolerance

@@ -154,17 +154,15 @@ __device__ __forceinline__ void t_perm(const Dims& d, int bx, int by, int bz, do
                 const float D = a + (float)(txm * s) + b + (float)(tym * s) + f + (float)(tzm * s);
                 O.invD[c] = (id == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
             } else if (MODE == PERM_F16) {
-                // FP16-stored faces h = half(float(t * s)); the FP32 copy and
-                // 1/D hold exactly the values the FP16 faces decode to
+                // FP16-stored faces h = half(float(t * s)); 1/D is formed from
+                // the decoded values (every FP32 consumer of these faces reads
+                // the FP16 copy, so no FP32 copy is written)
                 const double s = O.s;
                 auto h = [&](double t) { return (_Float16)(float)(t * s); };
                 const _Float16 a = h(txp), b = h(typ), f = h(tzp);
                 O.htx[c] = a;
                 O.hty[c] = b;
                 O.htz[c] = f;
-                O.tx[c] = (float)a;
-                O.ty[c] = (float)b;
-                O.tz[c] = (float)f;
                 const float D = (float)a + (float)h(txm) + (float)b + (float)h(tym) + (float)f + (float)h(tzm);
                 O.invD[c] = (id == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
             }

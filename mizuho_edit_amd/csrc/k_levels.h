@@ -369,13 +369,19 @@ __global__ void __launch_bounds__(kThreads) k_smooth_pro(LV<R> L, int cnx, int c
     t_smooth_pro<R, PROX, PROO>(L, cnx, cny, cs, xc, a, w, b, x, xo, y, (int)blockIdx.x, (int)gridDim.x);
 }
 
-// Galerkin aggregation A_c = P^T A P on 2x2x2 aggregates.
-template <typename R, bool FINE>
-__device__ __forceinline__ void t_coarsen(LV<R> L, int cnx, int cny, int cnz,
+// Galerkin aggregation A_c = P^T A P on 2x2x2 aggregates.  The faces are
+// read from fx, fy, fz in precision CF (the fine level's FP16-stored faces,
+// or the level's own L.tx, L.ty, L.tz).
+template <typename R, bool FINE, typename CF = R>
+__device__ __forceinline__ void t_coarsen(LV<R> L, const CF* __restrict__ fx, const CF* __restrict__ fy,
+                                                      const CF* __restrict__ fz, int cnx, int cny, int cnz,
                                                       int64_t cplane, R* __restrict__ ctx,
                                                       R* __restrict__ cty, R* __restrict__ ctz,
                                                       R* __restrict__ cD, R* __restrict__ cinvD, int b_, int nb_)
 {
+    auto TX = [&](int64_t m) { return (R)fx[m]; };
+    auto TY = [&](int64_t m) { return (R)fy[m]; };
+    auto TZ = [&](int64_t m) { return (R)fz[m]; };
     const int64_t nc = (int64_t)cnx * cny * cnz;
     const Chunk ch = my_chunk_at(nc, b_, nb_);
     for (int64_t C = ch.begin + threadIdx.x; C < ch.end; C += kThreads) {
@@ -395,22 +401,21 @@ __device__ __forceinline__ void t_coarsen(LV<R> L, int cnx, int cny, int cnz,
                     const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
                     if (is_fixed(L, c, FINE)) continue;
                     if (FINE)
-                        D += L.tx[c] + L.tx[c - 1] + L.ty[c] + L.ty[c - L.nx] + L.tz[c] +
-                             L.tz[c - L.plane];
+                        D += TX(c) + TX(c - 1) + TY(c) + TY(c - L.nx) + TZ(c) + TZ(c - L.plane);
                     else
                         D += L.D[c];
                     // +x face
                     if (i + 1 < L.nx && !is_fixed(L, c + 1, FINE)) {
-                        if (di == 0) D -= (R)2 * L.tx[c];
-                        else tx += L.tx[c];
+                        if (di == 0) D -= (R)2 * TX(c);
+                        else tx += TX(c);
                     }
                     if (j + 1 < L.ny && !is_fixed(L, c + L.nx, FINE)) {
-                        if (dj == 0) D -= (R)2 * L.ty[c];
-                        else ty += L.ty[c];
+                        if (dj == 0) D -= (R)2 * TY(c);
+                        else ty += TY(c);
                     }
                     if (k + 1 < L.nz && !is_fixed(L, c + L.plane, FINE)) {
-                        if (dk == 0) D -= (R)2 * L.tz[c];
-                        else tz += L.tz[c];
+                        if (dk == 0) D -= (R)2 * TZ(c);
+                        else tz += TZ(c);
                     }
                 }
             }
@@ -426,13 +431,15 @@ __device__ __forceinline__ void t_coarsen(LV<R> L, int cnx, int cny, int cnz,
         cinvD[C] = D > (R)0 ? (R)1 / D : (R)0;
     }
 }
-template <typename R, bool FINE>
-__global__ void __launch_bounds__(kThreads) k_coarsen(LV<R> L, int cnx, int cny, int cnz,
+template <typename R, bool FINE, typename CF = R>
+__global__ void __launch_bounds__(kThreads) k_coarsen(LV<R> L, const CF* __restrict__ fx, const CF* __restrict__ fy,
+                                                      const CF* __restrict__ fz, int cnx, int cny, int cnz,
                                                       int64_t cplane, R* __restrict__ ctx,
                                                       R* __restrict__ cty, R* __restrict__ ctz,
                                                       R* __restrict__ cD, R* __restrict__ cinvD)
 {
-    t_coarsen<R, FINE>(L, cnx, cny, cnz, cplane, ctx, cty, ctz, cD, cinvD, (int)blockIdx.x, (int)gridDim.x);
+    t_coarsen<R, FINE, CF>(L, fx, fy, fz, cnx, cny, cnz, cplane, ctx, cty, ctz, cD, cinvD, (int)blockIdx.x,
+                          (int)gridDim.x);
 }
 
 // coarsest level: nsweep damped-Jacobi sweeps from 0, one workgroup,

@@ -129,19 +129,25 @@ static int ensure_hierarchy(std::vector<MGLevel<R>>& lv, int nlevels)
     return WPF_OK;
 }
 
+// Galerkin levels 1.. of lv; with h16 the fine level's faces are read from
+// its FP16-stored copy (whose decoded values the FP32 faces would hold).
 template <typename R>
-static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv)
+static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv, bool h16 = false)
 {
     for (size_t l = 1; l < lv.size(); l++) {
         MGLevel<R>& C = lv[l];
         const bool fine = (l == 1);
         LV<R> F = view(lv[l - 1], c->mask, fine);
-        if (fine)
-            LAUNCH1D((k_coarsen<R, true>), C.g.n, F, C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p,
+        if (fine && h16)
+            LAUNCH1D((k_coarsen<R, true, _Float16>), C.g.n, F, (const _Float16*)c->h_tx.p,
+                     (const _Float16*)c->h_ty.p, (const _Float16*)c->h_tz.p, C.g.nx, C.g.ny, C.g.nz, C.g.plane,
+                     C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
+        else if (fine)
+            LAUNCH1D((k_coarsen<R, true>), C.g.n, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p,
                      C.ty.p, C.tz.p, C.D.p, C.invD.p);
         else
-            LAUNCH1D((k_coarsen<R, false>), C.g.n, F, C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p,
-                     C.ty.p, C.tz.p, C.D.p, C.invD.p);
+            LAUNCH1D((k_coarsen<R, false>), C.g.n, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny, C.g.nz, C.g.plane,
+                     C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
     }
     return WPF_OK;
 }
@@ -947,7 +953,7 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         else
             LAUNCH1D(k_to_f32, n, L, s, F32.tx.p, F32.ty.p, F32.tz.p, F32.invD.p);
     }
-    if ((rc = coarsen_all(c, c->mg32))) return rc;
+    if ((rc = coarsen_all(c, c->mg32, op16))) return rc;
     const MGOpts mo = mg_opts(o);
     Fine<float, float> F;
     F.g = c->g;

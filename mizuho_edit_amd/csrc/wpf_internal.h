@@ -208,7 +208,7 @@ enum PermMode { PERM_F64 = 0, PERM_F32 = 1, PERM_F16 = 2 };
 struct PermOut32 {
     double s;                    // scale of the FP32 system
     _Float16 *htx, *hty, *htz;   // PERM_F16: FP16 faces
-    float *tx, *ty, *tz, *invD;  // FP32 faces (the decoded FP16 ones) and 1/D
+    float *tx, *ty, *tz, *invD;  // FP32 faces (PERM_F32 only) and 1/D
 };
 int launch_perm(wpf_ctx* c, int mode, const PermOut32* out);
 // the permeability (and everything derived from it) no longer matches phase

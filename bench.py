@@ -374,7 +374,7 @@ def main():
         "data": "synthetic: seeded Gaussian random field rock (corr 4 cells), Kozeny-Carman k",
         "config": {"workload": f"wormholing {n}^3, steps {a.warmup + 1}..{a.warmup + a.steps} "
                                f"of the 500-step run", "grid": [n, n, n],
-                   "solver": ("" if a.no_project else "projected initial guess (last 6 solution changes) + ") +
+                   "solver": ("" if a.no_project else "projected initial guess (last 4 solution changes) + ") +
                              ("learned first stage + " if net else "") +
                              (f"FP32 MG-PCG ({a.fast_levels} levels, 2 Jacobi sweeps)" if a.jacobi else
                               f"FP32 MG-PCG ({a.fast_levels} levels, degree-2 Chebyshev, FP16-stored fine faces)") +

@@ -131,6 +131,7 @@ static int ensure_hierarchy(std::vector<MGLevel<R>>& lv, int nlevels)
 
 // Galerkin levels 1.. of lv; with h16 the fine level's faces are read from
 // its FP16-stored copy (whose decoded values the FP32 faces would hold).
+// (no reduction: one coarse cell per thread, as many blocks as that takes)
 template <typename R>
 static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv, bool h16 = false)
 {
@@ -138,16 +139,20 @@ static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv, bool h16 = false
         MGLevel<R>& C = lv[l];
         const bool fine = (l == 1);
         LV<R> F = view(lv[l - 1], c->mask, fine);
+        int64_t nb = (C.g.n + kThreads - 1) / kThreads;
+        if (nb >= 8) nb = nb / 8 * 8;
+        const dim3 grid((unsigned)nb), block(kThreads);
         if (fine && h16)
-            LAUNCH1D((k_coarsen<R, true, _Float16>), C.g.n, F, (const _Float16*)c->h_tx.p,
-                     (const _Float16*)c->h_ty.p, (const _Float16*)c->h_tz.p, C.g.nx, C.g.ny, C.g.nz, C.g.plane,
-                     C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
+            hipLaunchKernelGGL((k_coarsen<R, true, _Float16>), grid, block, 0, c->stream, F,
+                               (const _Float16*)c->h_tx.p, (const _Float16*)c->h_ty.p, (const _Float16*)c->h_tz.p,
+                               C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
         else if (fine)
-            LAUNCH1D((k_coarsen<R, true>), C.g.n, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p,
-                     C.ty.p, C.tz.p, C.D.p, C.invD.p);
+            hipLaunchKernelGGL((k_coarsen<R, true>), grid, block, 0, c->stream, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny,
+                               C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
         else
-            LAUNCH1D((k_coarsen<R, false>), C.g.n, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny, C.g.nz, C.g.plane,
-                     C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
+            hipLaunchKernelGGL((k_coarsen<R, false>), grid, block, 0, c->stream, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny,
+                               C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
+        WPF_HIP(hipGetLastError());
     }
     return WPF_OK;
 }

@@ -513,7 +513,7 @@ __global__ void __launch_bounds__(256) k_vgram(VZ Z, ProjVecs<R> vs, const R* __
 
 // y = G^+ c for the packed Gram matrix in res[0 ..) and c after it (one
 // thread): Cholesky with a pivot threshold; a direction whose pivot falls
-// below 1e-12 of its diagonal is dependent on the earlier ones and gets
+// below 1e-6 of its diagonal is dependent on the more recent ones and gets
 // y = 0.  y[M] = 1 when any direction was used.
 template <int M>
 __device__ __forceinline__ void proj_solve(const double* __restrict__ res, double* __restrict__ y)
@@ -529,7 +529,12 @@ __device__ __forceinline__ void proj_solve(const double* __restrict__ res, doubl
         const double gjj = res[tri(j, j, M)];
         double d = gjj;
         for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k];
-        use[j] = gjj > 0.0 && d > 1e-12 * gjj;
+        // a direction whose A-energy outside the span of the more recent
+        // ones is below 1e-6 of its own is numerically dependent: the
+        // changes are stored and multiplied by A in FP32, so the Gram
+        // entries are good to about 1e-7, and keeping such a direction only
+        // buys large cancelling coefficients
+        use[j] = gjj > 0.0 && d > 1e-6 * gjj;
         if (!use[j]) continue;
         L[j][j] = sqrt(d);
         for (int i = j + 1; i < M; i++) {

@@ -135,8 +135,9 @@ struct wpf_ctx {
     wpf::Dev<float> r32, z32, p32, q32, x32, t32, p32b;
     wpf::Dev<float> r32b;                     // second FP32 residual (fused update)
     // the last solution changes p_n - p_(n-1) (FP32 ring) for the projected
-    // initial guess (WPF_FLAG_PROJECT)
-    static constexpr int kProjDepth = 6;
+    // initial guess (WPF_FLAG_PROJECT); four measured best at 256^3 (three,
+    // five and six cost more steps or more per step)
+    static constexpr int kProjDepth = 4;
     wpf::Dev<float> proj[kProjDepth];
     int proj_count = 0, proj_next = 0;
     bool proj_used = false;

@@ -49,8 +49,9 @@ struct EnsMember {
     int nx, ny, nz;
     int64_t plane, n;
     VZ Z;    // vector-kernel tiles (tuned depth)
+    VZ Z64;  // the same depth, 2 FP64 cells per thread (the FP64 residual)
     ZG G;    // restriction tiles
-    int nb1, nbv, nbz;  // blocks of the 1-D, vector and restriction kernels
+    int nb1, nbv, nbz, nbv64;  // blocks of the 1-D, vector, restriction and FP64 vector kernels
     int gxs, gys, nbs;  // tiles of the cell kernels (permeability, fused step)
     // FP64 state and operator
     double *phi[2], *conc[2], *p, *krock, *tx, *ty, *tz, *invD, *fsrc;

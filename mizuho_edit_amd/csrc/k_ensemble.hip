@@ -39,8 +39,8 @@ struct wpf_ensemble {
     wpf::EnsState* d_state = nullptr;        // device states
     wpf::EnsState* h_state = nullptr;        // pinned host copy
     // tile tables: 1-D fine, vector, restriction, cell kernels, coarse levels
-    wpf::EnsTile *t1 = nullptr, *tv = nullptr, *tz = nullptr, *ts = nullptr;
-    int n1 = 0, nv = 0, nz = 0, ns = 0;
+    wpf::EnsTile *t1 = nullptr, *tv = nullptr, *tz = nullptr, *ts = nullptr, *t64 = nullptr;
+    int n1 = 0, nv = 0, nz = 0, ns = 0, n64 = 0;
     wpf::EnsTile* tl[wpf::kEnsMaxLevels] = {};
     int nl[wpf::kEnsMaxLevels] = {};
     size_t coarsest_lds = 0;
@@ -51,7 +51,7 @@ struct wpf_ensemble {
 namespace wpf {
 
 enum EnsGate { G_ALL = 0, G_INNER = 1, G_STAGE = 2 };
-enum EnsFamily { F_1D = 0, F_VEC = 1 };
+enum EnsFamily { F_1D = 0, F_VEC = 1, F_VEC64 = 2 };
 
 __device__ __forceinline__ bool ens_on(const EnsMember& M, int gate)
 {
@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(1024) ke_finalize(const EnsMember* __restrict_
 {
     const EnsMember& M = Ms[blockIdx.x];
     if (!ens_on(M, gate)) return;
-    t_finalize(M.partials, family == F_VEC ? M.nbv : M.nb1, nvals, M.result + slot);
+    t_finalize(M.partials, family == F_VEC ? M.nbv : family == F_VEC64 ? M.nbv64 : M.nb1, nvals, M.result + slot);
 }
 
 // ---- fine level, 1-D tiles --------------------------------------------------
@@ -256,12 +256,14 @@ __global__ void __launch_bounds__(256) ke_extrapolate(ENS_ARGS)
     ENS_END
 }
 
-// bn: the first stage also sums ||D^-1 b||^2 (into the second value)
+// the FP64 residual of k_vtrue_resid over FP64 vector tiles; bn: the first
+// stage also sums ||D^-1 b||^2 (into the second value)
 __global__ void __launch_bounds__(256) ke_true_resid(ENS_ARGS, int bn)
 {
     ENS_TILES(G_STAGE)
-    t_true_resid(ens_lv64(M), M.p, M.fsrc, (double*)nullptr, (double*)nullptr, M.v[EV_R], M.s32, M.partials,
-                 nullptr, nullptr, T.b, M.nb1, bn != 0);
+    double acc[2];
+    t_vtrue_resid<2>(M.Z64, T.b, ens_lv64(M), M.p, M.fsrc, M.v[EV_R], M.s32, bn != 0, acc);
+    block_reduce_store<2>(acc, M.partials, nullptr, nullptr, M.nbv64, T.b);
     ENS_END
 }
 
@@ -466,7 +468,7 @@ int build_tables(wpf_ensemble* e)
 {
     const int nm = (int)e->mem.size();
     std::vector<EnsMember> M(nm);
-    std::vector<int> c1(nm), cv(nm), cz(nm), cs(nm);
+    std::vector<int> c1(nm), cv(nm), cz(nm), cs(nm), c64(nm);
     std::vector<std::vector<int>> cl(e->nlev, std::vector<int>(nm));
     for (int m = 0; m < nm; m++) {
         wpf_ctx* c = e->mem[m];
@@ -478,6 +480,8 @@ int build_tables(wpf_ensemble* e)
         X.nb1 = blocks_for(c->g.n);
         X.nbv = vz_blocks(X.Z);
         X.nbz = zg_blocks(X.G);
+        X.Z64 = make_vz<2>(c->g, e->zc);
+        X.nbv64 = vz_blocks(X.Z64);
         {
             int gz;
             X.nbs = step_tiles(c->g.nx, c->g.ny, c->g.nz, &X.gxs, &X.gys, &gz);
@@ -516,13 +520,14 @@ int build_tables(wpf_ensemble* e)
         X.dt_pref = p.tfac * 0.5 * (p.dx * p.dx) / (p.epsilon * p.epsilon);
         X.eps2_D = (p.epsilon * p.epsilon) / p.diffusion;
         X.K = phase_consts(p);
-        c1[m] = X.nb1; cv[m] = X.nbv; cz[m] = X.nbz; cs[m] = X.nbs;
+        c1[m] = X.nb1; cv[m] = X.nbv; cz[m] = X.nbz; cs[m] = X.nbs; c64[m] = X.nbv64;
     }
     if (!e->d_mem) WPF_HIP(hipMalloc((void**)&e->d_mem, sizeof(EnsMember) * nm));
     WPF_HIP(hipMemcpy(e->d_mem, M.data(), sizeof(EnsMember) * nm, hipMemcpyHostToDevice));
     int rc;
     if ((rc = build_tiles(c1, &e->t1, &e->n1)) || (rc = build_tiles(cv, &e->tv, &e->nv)) ||
-        (rc = build_tiles(cz, &e->tz, &e->nz)) || (rc = build_tiles(cs, &e->ts, &e->ns)))
+        (rc = build_tiles(cz, &e->tz, &e->nz)) || (rc = build_tiles(cs, &e->ts, &e->ns)) ||
+        (rc = build_tiles(c64, &e->t64, &e->n64)))
         return rc;
     for (int l = 1; l < e->nlev; l++)
         if ((rc = build_tiles(cl[l], &e->tl[l], &e->nl[l]))) return rc;
@@ -667,8 +672,8 @@ int sched_solve(wpf_ensemble* e)
     const float w1 = (float)S.mo.w1;
     for (int stage = 0; stage < e->eo.stages; stage++) {
         // the first stage's residual pass also gives ||D^-1 b|| and the tolerance
-        ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t1, e->n1, stage == 0 ? 1 : 0);
-        ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 2, (int)S_T0, (int)G_STAGE);
+        ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t64, e->n64, stage == 0 ? 1 : 0);
+        ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC64, 2, (int)S_T0, (int)G_STAGE);
         if (stage == 0) ELAUNCH(ke_tol, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.rtol, o.atol);
         ELAUNCH(ke_outer, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.inner_rtol, o.inner_maxit, o.maxit);
         // pcg_core
@@ -706,8 +711,8 @@ int sched_solve(wpf_ensemble* e)
         }
         ELAUNCH(ke_add, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
     }
-    ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t1, e->n1, 0);
-    ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 2, (int)S_T0, (int)G_STAGE);
+    ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t64, e->n64, 0);
+    ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC64, 2, (int)S_T0, (int)G_STAGE);
     ELAUNCH(ke_final, S.mblocks(), S.mthreads(), 0, e->d_mem, nm);
     return WPF_OK;
 }
@@ -1030,7 +1035,7 @@ int wpf_ensemble_destroy(wpf_ensemble* e)
     auto F = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    F(e->d_mem); F(e->d_state); F(e->t1); F(e->tv); F(e->tz); F(e->ts);
+    F(e->d_mem); F(e->d_state); F(e->t1); F(e->tv); F(e->tz); F(e->ts); F(e->t64);
     for (int l = 0; l < kEnsMaxLevels; l++) F(e->tl[l]);
     if (e->h_state) (void)hipHostFree(e->h_state);
     if (e->stream) (void)hipStreamDestroy(e->stream);

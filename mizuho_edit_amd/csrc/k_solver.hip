@@ -978,15 +978,29 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
     set_shape(H, c->tuned_zc[0]);
     float* e = c->x32.p;
     double* x = c->p.p;
+    // the FP64 residual in vector form (2 FP64 cells per thread, the tile
+    // depth of the FP32 kernels) where rows split into pairs
+    const VZ Z64 = make_vz<2>(c->g, c->tuned_zc[0]);
+    const bool vec64 = c->g.nx % 2 == 0;
+    const int nb64 = vec64 ? vz_blocks(Z64) : blocks_for(n);
     int its = 0, outer = 0;
     double rn = 0.0, rn0 = -1.0;
     for (;;) {
         {
             const int kid_ = WPF_KERNEL_TRUE_RESID;
-            LAUNCH1D(k_true_resid, n, L, (const double*)x, (const double*)c->fsrc.p, (double*)nullptr,
-                     (double*)nullptr, F.r, s, RED(S_T0), bnorm < 0.0 ? 1 : 0);
+            const int bn = bnorm < 0.0 ? 1 : 0;
+            if (vec64) {
+                prof_mark(c, kid_, true);
+                hipLaunchKernelGGL(k_vtrue_resid<2>, dim3(nb64), dim3(256), 0, c->stream, Z64, L, (const double*)x,
+                                   (const double*)c->fsrc.p, F.r, s, bn, c->red.partials);
+                prof_mark(c, kid_, false);
+                WPF_HIP(hipGetLastError());
+            } else {
+                LAUNCH1D(k_true_resid, n, L, (const double*)x, (const double*)c->fsrc.p, (double*)nullptr,
+                         (double*)nullptr, F.r, s, RED(S_T0), bn);
+            }
         }
-        FINALIZE(blocks_for(n), 2, S_T0);
+        FINALIZE(nb64, 2, S_T0);
         if ((rc = read_results(c, S_T0, 2))) return rc;
         rn = std::sqrt(c->red.host[S_T0]);
         if (bnorm < 0.0) {

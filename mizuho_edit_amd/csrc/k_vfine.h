@@ -20,7 +20,7 @@
 // contiguous run of (chunk, z-run) tiles, chunks fastest.
 #pragma once
 
-#include "k_fine.h"
+#include "k_levels.h"  // LV, bnorm_cell (and k_fine.h)
 
 namespace wpf {
 
@@ -760,6 +760,90 @@ __global__ void __launch_bounds__(256) k_vdir_mv(VZ Z, const C* __restrict__ tx,
     double acc[1] = {t_vdir_mv<R, C, V>(Z, blockIdx.x, tx, ty, tz, mask, z, pold, pnew, qv, res, num, den,
                                         first)};
     block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
+}
+
+// The FP64 true residual r = f - A x of the mixed solver in vector form
+// (V = 2 FP64 cells per thread): writes the scaled FP32 copy s32 r, reduces
+// ||invD r||^2 into acc[0] and, with bn, ||D^-1 b||^2 of the reference
+// system (bnorm_cell, per cell) into acc[1].  d and s in the order of
+// row_parts; fixed cells get r = 0.
+template <int V>
+__device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<double>& L,
+                                              const double* __restrict__ x, const double* __restrict__ f,
+                                              float* __restrict__ r32, double s32, bool bn, double (&acc)[2])
+{
+    int64_t q;
+    int k0, k1;
+    const bool act = vz_at<V>(Z, bi, q, k0, k1);
+    const int lane = threadIdx.x & 63;
+    const int64_t nx = Z.nx, pl = Z.plane;
+    acc[0] = acc[1] = 0.0;
+    double xm[V], x0[V], xp[V], tzm[V];
+    int64_t c = q + pl * k0;
+    vld<V>(x + c - pl, xm);
+    vld<V>(x + c, x0);
+    vld<V>(L.tz + c - pl, tzm);
+    for (int k = k0; k < k1; k++, c += pl) {
+        vld<V>(x + c + pl, xp);
+        double xyp[V], xym[V], txv[V], tyv[V], tym[V], tzv[V], di[V], fv[V];
+        uint8_t m[V];
+        vld<V>(x + c + nx, xyp);
+        vld<V>(x + c - nx, xym);
+        vld<V>(L.tx + c, txv);
+        vld<V>(L.ty + c, tyv);
+        vld<V>(L.ty + c - nx, tym);
+        vld<V>(L.tz + c, tzv);
+        vld<V>(L.invD + c, di);
+        vld<V>(L.mask + c, m);
+        if (f) {
+            vld<V>(f + c, fv);
+        } else {
+#pragma unroll
+            for (int h = 0; h < V; h++) fv[h] = 0.0;
+        }
+        double xl = from_left(x0[V - 1]), xr = from_right(x0[0]), txl = from_left(txv[V - 1]);
+        if (lane == 0) {
+            xl = x[c - 1];
+            txl = L.tx[c - 1];
+        }
+        if (lane == 63) xr = x[c + V];
+        float rv32[V];
+#pragma unroll
+        for (int h = 0; h < V; h++) {
+            const double txm = h ? txv[h - 1] : txl;
+            const double xle = h ? x0[h - 1] : xl;
+            const double xre = (h + 1 < V) ? x0[h + 1] : xr;
+            const double d = txv[h] + txm + tyv[h] + tym[h] + tzv[h] + tzm[h];
+            const double s = txv[h] * xre + txm * xle + tyv[h] * xyp[h] + tym[h] * xym[h] + tzv[h] * xp[h] +
+                             tzm[h] * xm[h];
+            double rv = fv[h] - (d * x0[h] - s);
+            if (m[h] & WPF_MASK_FIXED) rv = 0.0;
+            const double zv = rv * di[h];
+            if (act) acc[0] += zv * zv;
+            rv32[h] = (float)(rv * s32);
+            if (bn && act) {
+                const double v = bnorm_cell(L, x, f, c + h);
+                acc[1] += v * v;
+            }
+        }
+        if (act) vst<V>(r32 + c, rv32);
+#pragma unroll
+        for (int h = 0; h < V; h++) {
+            xm[h] = x0[h];
+            x0[h] = xp[h];
+            tzm[h] = tzv[h];
+        }
+    }
+}
+
+template <int V>
+__global__ void __launch_bounds__(256) k_vtrue_resid(VZ Z, LV<double> L, const double* __restrict__ x,
+                                                     const double* __restrict__ f, float* __restrict__ r32,
+                                                     double s32, int bn, double* __restrict__ partials)
+{
+    double acc[2];
+    t_vtrue_resid<V>(Z, blockIdx.x, L, x, f, r32, s32, bn != 0, acc);
+    block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
 }
 
 }  // namespace wpf

@@ -92,7 +92,9 @@ typedef struct wpf_well {
 #define WPF_SOLVER_MIXED     2  /* FP32 MG-PCG inside FP64 refinement        */
 
 #define WPF_FLAG_LEARNED_WARM  1  /* learned first stage (needs a net)      */
-#define WPF_FLAG_LEARNED_FCG   2  /* same stage, named for flexible CG       */
+#define WPF_FLAG_LEARNED_FCG   2  /* WPF_SOLVER_CG: the learned stage inside
+                                     flexible CG; other solvers: the first
+                                     stage, as WPF_FLAG_LEARNED_WARM        */
 #define WPF_FLAG_EXTRAPOLATE   4  /* linear-in-time initial guess            */
 #define WPF_FLAG_CAPTURE       8  /* keep (input, correction) for training   */
 #define WPF_FLAG_CHEBYSHEV    16  /* degree-2 Chebyshev smoothing (sweeps 2) */
@@ -124,7 +126,8 @@ typedef struct wpf_solve_stats {
     int32_t iterations;   /* Krylov iterations (FP32 inner total if mixed)   */
     int32_t outer;        /* refinement steps (mixed), 0 otherwise           */
     int32_t converged;    /* 1 when the test above holds on the true residual*/
-    int32_t learned;      /* 1 when a learned stage was accepted             */
+    int32_t learned;      /* learned first stage: 1 when accepted; flexible
+                             CG: the number of learned stage applications  */
     double  rnorm0;       /* initial true residual                           */
     double  rnorm;        /* final true residual                             */
     double  bnorm;        /* right-hand side                                  */
@@ -238,6 +241,13 @@ int wpf_field_write(const char* path, double time, const wpf_params* p,
  * direction with an exact line search on the FP64 operator, keeps it only if
  * the FP64 residual drops, and then runs its ordinary iteration to the same
  * FP64 tolerance: the network can speed a solve up, never decide it.
+ * With WPF_SOLVER_CG and WPF_FLAG_LEARNED_FCG the solve is flexible CG
+ * (truncation 1) with the Jacobi preconditioner, where every 8th
+ * preconditioner application (the first included) is the learned stage:
+ * y = network correction for the current residual, s = (r, y)/(y, Ay),
+ * z = s y + D^-1 (r - s A y).  It stops on the same FP64 residual test; if
+ * that pass stalls (the residual fails to halve over 32 iterations) or does
+ * not converge, plain CG continues from its iterate without the network.
  * Grid sizes used with the network must be multiples of 32.
  * Fields for wpf_net_forward / _gradient / _train are planar FP32
  * [3][nx*ny*nz] (x fastest); targets [nx*ny*nz]; the output is the level-1

@@ -21,8 +21,8 @@ static constexpr int kMaxBlocks = 2048;
 // result slots: S_RN and S_T0 hold two values each; S_GRAM up to 27 (the
 // packed Gram matrix and V^T r of the projection), S_PROJY up to 7;
 // kTuneSlot holds 1 while the tile-depth tuner runs
-enum Slot { S_PQ = 0, S_RN = 1, S_RZ0 = 4, S_RZ1 = 5, S_BN = 6, S_T0 = 8, S_LR = 10, S_PN = 12,
-            S_GRAM = 16, S_PROJY = 48 };
+enum Slot { S_PQ = 0, S_RN = 1, S_RZ0 = 4, S_RZ1 = 5, S_BN = 6, S_T0 = 8, S_LR = 10, S_YAY = 11, S_PN = 12,
+            S_FCG = 13, S_GRAM = 16, S_PROJY = 48 };
 static constexpr int kTuneSlot = 63;
 
 template <typename R>

@@ -286,7 +286,7 @@ static int sweep(wpf_ctx* c, const Fine<R, C>& F, double a, double w, const R* x
 // in the last bits.
 template <typename R, typename C>
 static int dir_mv(wpf_ctx* c, const Fine<R, C>& F, const R* z, const R* pold, R* pnew, int num, int den,
-                  int first);
+                  int first, int slot = S_PQ);
 
 // The CG update fused with the first pre-smoothing step (k_vupsweep): x +=
 // alpha p, r2 = r - alpha q, y (into F.T) from z0 = w1 invD r2; alpha =
@@ -393,7 +393,7 @@ static int sweep(wpf_ctx* c, const Fine<R, C>& F, double a, double w, const R* x
 // p_new = z (+ beta p_old), q = A p_new, reduces (p_new, q) into S_PQ.
 template <typename R, typename C>
 static int dir_mv(wpf_ctx* c, const Fine<R, C>& F, const R* z, const R* pold, R* pnew, int num, int den,
-                  int first)
+                  int first, int slot)
 {
     constexpr int V = vwidth<R>();
     prof_mark(c, WPF_KERNEL_MATVEC, true);
@@ -407,7 +407,7 @@ static int dir_mv(wpf_ctx* c, const Fine<R, C>& F, const R* z, const R* pold, R*
                            (const double*)c->red.result, num, den, first, RED(S_PQ));
     prof_mark(c, WPF_KERNEL_MATVEC, false);
     WPF_HIP(hipGetLastError());
-    FINALIZE(fine_blocks(F), 1, S_PQ);
+    FINALIZE(fine_blocks(F), 1, slot);
     return WPF_OK;
 }
 
@@ -527,6 +527,142 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
             zres = F.Z;
             WPF_HIP(hipMemcpyAsync(c->red.result + rz_cur, c->red.result + S_RN + 1, sizeof(double),
                                    hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    *its_out = its;
+    *rn_out = rn;
+    return WPF_OK;
+}
+
+// Flexible-CG update: x += alpha p, r -= alpha q (alpha = res[num]/res[den]),
+// z = invD r.  Reduces ||invD r||^2, (r, z) and -(z, q), the numerator of the
+// flexible beta -(z_new, q_old)/(p_old, q_old).
+static __global__ void __launch_bounds__(kThreads) k_fupdate(int64_t n, const double* __restrict__ res, int num,
+                                                             int den, double* __restrict__ x,
+                                                             const double* __restrict__ p, double* __restrict__ r,
+                                                             const double* __restrict__ q,
+                                                             const double* __restrict__ invD,
+                                                             double* __restrict__ z, double* __restrict__ partials)
+{
+    const double alpha = res[num] / res[den];
+    const Chunk ch = my_chunk(n);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        x[c] += alpha * p[c];
+        const double qv = q[c];
+        const double rv = r[c] - alpha * qv;
+        r[c] = rv;
+        const double zv = rv * invD[c];
+        z[c] = zv;
+        acc[0] += zv * zv;
+        acc[1] += rv * zv;
+        acc[2] -= zv * qv;
+    }
+    block_reduce_store<3>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
+}
+
+// The learned stage as a preconditioner application: y is the network's
+// correction for the residual r and ay = A y; with s = res[ry]/res[yay] (the
+// exact line search, 0 unless (y, Ay) > 0 and s is finite) it sets
+// z = s y + invD (r - s ay), a coarse correction followed by one Jacobi step.
+// Reduces (r, z) and, when qold is given, -(z, qold).
+static __global__ void __launch_bounds__(kThreads) k_fcg_stage(int64_t n, const double* __restrict__ res, int ry,
+                                                               int yay, const double* __restrict__ y,
+                                                               const double* __restrict__ ay,
+                                                               const double* __restrict__ r,
+                                                               const double* __restrict__ invD,
+                                                               const double* __restrict__ qold,
+                                                               double* __restrict__ z, double* __restrict__ partials)
+{
+    double s = res[yay] > 0.0 ? res[ry] / res[yay] : 0.0;
+    if (!(fabs(s) < 1e300)) s = 0.0;
+    const Chunk ch = my_chunk(n);
+    double acc[2] = {0.0, 0.0};
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
+        const double rv = r[c];
+        const double zv = s * y[c] + invD[c] * (rv - s * ay[c]);
+        z[c] = zv;
+        acc[0] += rv * zv;
+        if (qold) acc[1] -= zv * qold[c];
+    }
+    block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
+}
+
+// Flexible CG with truncation 1 (beta = -(z_new, q_old)/(p_old, q_old)) on
+// the FP64 operator with the Jacobi preconditioner, where every
+// kFcgPeriod-th preconditioner application, the first included, is the
+// learned stage of k_fcg_stage.  The network's output depends nonlinearly
+// on r, which is why the Fletcher-Reeves beta of pcg_core does not apply.
+// Starts from the FP64 residual in F.r (norm rn); stops when the recursive
+// ||invD r|| <= tol, after maxit iterations, or when the residual fails to
+// halve over kFcgWindow iterations: the stage disturbs the conjugacy of the
+// earlier directions, which on residuals unlike the network's training data
+// (the first steps of a run) can stall the iteration, and the caller then
+// restarts plain CG from the current iterate.  *stages counts the learned
+// applications.
+static constexpr int kFcgPeriod = 8, kFcgWindow = 32;
+
+static int fcg_learned(wpf_ctx* c, const Fine<double, double>& F, double* x, double rn, double tol, int maxit,
+                       int* its_out, double* rn_out, int* stages)
+{
+    const int64_t n = F.g.n;
+    const int nb = blocks_for(n);
+    double* const y = F.T;    // network correction
+    double* const ay = F.r2;  // A y
+    double* pc = F.pa;
+    double* pn = F.pb;
+    // the reduction slots of (r, z) and of the flexible beta's numerator
+    int rz = S_FCG, bnum = S_FCG + 1;
+    auto stage = [&](double rnow, const double* qold) -> int {
+        const int kid_ = WPF_KERNEL_CG_UPDATE;
+        int rc = net_correction<double>(c, F.r, rnow, y);
+        if (rc) return rc;
+        Fine<double, double> Fy = F;
+        Fy.q = ay;
+        // pn is free here: the next direction update overwrites it
+        if ((rc = dir_mv(c, Fy, (const double*)y, (const double*)y, pn, S_PQ, S_PQ, 1, S_YAY))) return rc;
+        LAUNCH1D(k_dot<double>, n, n, (const double*)F.r, (const double*)y, RED(S_LR));
+        FINALIZE(nb, 1, S_LR);
+        LAUNCH1D(k_fcg_stage, n, n, (const double*)c->red.result, (int)S_LR, (int)S_YAY, (const double*)y,
+                 (const double*)ay, (const double*)F.r, (const double*)F.invD, qold, F.Z, c->red.partials);
+        FINALIZE(nb, 2, S_FCG);
+        rz = S_FCG;
+        bnum = S_FCG + 1;
+        (*stages)++;
+        return WPF_OK;
+    };
+    int rc = stage(rn, nullptr);
+    if (rc) return rc;
+    int its = 0;
+    int first = 1;
+    double rn_window = rn;
+    while (its < maxit) {
+        if ((rc = dir_mv(c, F, (const double*)F.Z, (const double*)pc, pn, bnum, S_PQ, first))) return rc;
+        std::swap(pc, pn);
+        {
+            const int kid_ = WPF_KERNEL_CG_UPDATE;
+            LAUNCH1D(k_fupdate, n, n, (const double*)c->red.result, rz, (int)S_PQ, x, (const double*)pc, F.r,
+                     (const double*)F.q, (const double*)F.invD, F.Z, c->red.partials);
+        }
+        FINALIZE(nb, 3, S_RN);
+        its++;
+        first = 0;
+        if ((rc = read_results(c, S_RN, 1))) return rc;
+        rn = std::sqrt(c->red.host[S_RN]);
+        if (!std::isfinite(rn)) {
+            set_error("pressure residual is not finite");
+            return WPF_ERR_NAN;
+        }
+        if (rn <= tol) break;
+        if (its % kFcgWindow == 0) {
+            if (rn > 0.5 * rn_window) break;
+            rn_window = rn;
+        }
+        if (its % kFcgPeriod == 0) {
+            if ((rc = stage(rn, (const double*)F.q))) return rc;
+        } else {
+            rz = S_RN + 1;
+            bnum = S_RN + 2;
         }
     }
     *its_out = its;
@@ -861,8 +997,11 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
     if (rc) return rc;
     double bnorm;
     if ((rc = prepare(c, o, &bnorm))) return rc;
+    // Jacobi CG with WPF_FLAG_LEARNED_FCG runs the learned stage inside
+    // flexible CG (fcg_learned); otherwise the flag asks for the first stage
+    const bool fcg = !mg && (o.flags & WPF_FLAG_LEARNED_FCG) && c->net;
     int learned = 0;
-    if ((rc = start_stage(c, o, &learned, true))) return rc;
+    if ((rc = start_stage(c, o, &learned, !fcg || (o.flags & WPF_FLAG_LEARNED_WARM)))) return rc;
     const double tol = std::max(o.rtol * bnorm, o.atol);
     MGLevel<double> F0;
     F0.g = c->g; F0.tx = c->tx; F0.ty = c->ty; F0.tz = c->tz; F0.invD = c->invD;
@@ -896,8 +1035,16 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
         if (rn <= tol || its >= o.maxit || restarts > 3) break;
         int k = 0;
         double rr = 0.0;
-        rc = pcg_core<double, double, double>(c, mg ? &c->mg64 : nullptr, F, mo, x, tol, o.maxit - its,
-                                              &k, &rr);
+        // plain CG is the fallback: restarts after a flexible pass do not
+        // use the network
+        if (fcg && restarts == 0) {
+            int stages = 0;
+            rc = fcg_learned(c, F, x, rn, tol, o.maxit - its, &k, &rr, &stages);
+            learned += stages;
+        } else {
+            rc = pcg_core<double, double, double>(c, mg ? &c->mg64 : nullptr, F, mo, x, tol, o.maxit - its,
+                                                  &k, &rr);
+        }
         if (rc) return rc;
         its += k;
         restarts++;
@@ -1127,7 +1274,10 @@ int solve_pressure_continue(wpf_ctx* c, const wpf_solver_opts* in, wpf_solve_sta
 {
     wpf_solver_opts o;
     fill_defaults(in, o);
-    o.flags &= ~(WPF_FLAG_EXTRAPOLATE | WPF_FLAG_CAPTURE | WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG);
+    // the guess is already in place; the ensemble keeps the projection
+    // space, and its saved previous solution must not be overwritten
+    o.flags &= ~(WPF_FLAG_EXTRAPOLATE | WPF_FLAG_PROJECT | WPF_FLAG_CAPTURE | WPF_FLAG_LEARNED_WARM |
+                 WPF_FLAG_LEARNED_FCG);
     wpf_solve_stats local{};
     if (!st) st = &local;
     *st = wpf_solve_stats{};

@@ -159,11 +159,13 @@ def make_training_set(nreal: int = 8, grid=(64, 64, 64), nsteps: int = 40, every
 
 
 def compare(case, net: Optional["PCNet"], nsteps: int, skip: int = 3, solver: str = "mixed",
-            inner_rtol: float = 1e-4, extrapolate: bool = True):
+            inner_rtol: float = 1e-4, extrapolate: bool = True, fcg: bool = False):
     """Run `case` for nsteps from the extrapolated start (or, with
     extrapolate=False, from the previous pressure), without and with the
-    learned stage.  Returns per-run iterations (steps after `skip`),
-    accepted learned stages, wall seconds and the final pressures."""
+    learned stage: the first stage, or with fcg=True (and solver "cg") the
+    stage inside flexible CG.  Returns per-run iterations (steps after
+    `skip`), learned stages (accepted first stages, or applications inside
+    flexible CG), wall seconds and the final pressures."""
     import time
     from .wpfaw3d import WPFAW3D, solver_opts
     out = {}
@@ -171,7 +173,8 @@ def compare(case, net: Optional["PCNet"], nsteps: int, skip: int = 3, solver: st
         g = WPFAW3D(case)
         if use:
             g.attach_net(net)
-        flags = (_lib.FLAG_EXTRAPOLATE if extrapolate else 0) | (_lib.FLAG_LEARNED_WARM if use else 0)
+        stage = _lib.FLAG_LEARNED_FCG if fcg else _lib.FLAG_LEARNED_WARM
+        flags = (_lib.FLAG_EXTRAPOLATE if extrapolate else 0) | (stage if use else 0)
         o = solver_opts(solver, rtol=1e-10, inner_rtol=inner_rtol, flags=flags)
         its, acc, wall = [], 0, 0.0
         for s in range(nsteps):
@@ -204,6 +207,9 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-extrapolate", action="store_true",
                     help="eval: start each solve from the previous pressure, not the extrapolated one")
+    ap.add_argument("--fcg", action="store_true",
+                    help="eval: the stage inside flexible CG with the Jacobi preconditioner "
+                         "(against plain CG + Jacobi) instead of the first stage of the mixed solver")
     a = ap.parse_args(argv)
     n = a.grid
     if a.mode == "train":
@@ -218,8 +224,9 @@ def main(argv=None):
         net = PCNet()
         net.load(a.weights)
         case = io.workload("plain-small", seed=a.seed + 1000, grid=(n, n, n), corr_len=4.0)
-        r = compare(case, net, a.nsteps, extrapolate=not a.no_extrapolate)
-        r["grid"], r["extrapolate"] = n, not a.no_extrapolate
+        r = compare(case, net, a.nsteps, extrapolate=not a.no_extrapolate,
+                    solver="cg" if a.fcg else "mixed", fcg=a.fcg)
+        r["grid"], r["extrapolate"], r["stage"] = n, not a.no_extrapolate, "fcg" if a.fcg else "first"
         p0, p1 = r["plain"].pop("pressure"), r["net"].pop("pressure")
         r["p_rel_diff"] = float(np.abs(p1 - p0).max() / np.abs(p0).max())
         print(json.dumps(r), flush=True)

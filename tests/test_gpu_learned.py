@@ -98,3 +98,39 @@ def test_training_and_learned_stage_gain(gpu):
     assert np.abs(p1 - p0).max() <= 1e-7 * np.abs(p0).max()
     assert acc1 >= 1
     net.close()
+
+
+def test_learned_stage_inside_flexible_cg_gain(gpu):
+    """The shipped weights as a stage inside flexible CG with the Jacobi
+    preconditioner, against CG + Jacobi (the reference's KSP settings) on a
+    realisation the weights were not trained on: every solve meets the same
+    FP64 test, the answers agree, and from step 6 on (residuals like the
+    training data) the solves need fewer iterations.  The first steps, where
+    the stage can stall the iteration and plain CG takes over, are reported."""
+    net = learned.PCNet()
+    net.load()
+    case = io.workload("plain-small", seed=300, grid=(64, 64, 64), corr_len=4.0)
+    runs = {}
+    for fcg in (False, True):
+        g = WPFAW3D(case)
+        if fcg:
+            g.attach_net(net)
+        o = solver_opts("cg", rtol=1e-10, flags=_lib.FLAG_LEARNED_FCG if fcg else 0)
+        recs = [g.step(o) for _ in range(12)]
+        assert all(r["converged"] == 1 for r in recs)
+        runs[fcg] = (recs, g.pressure)
+        g.close()
+    its0 = [r["iterations"] for r in runs[False][0]]
+    its1 = [r["iterations"] for r in runs[True][0]]
+    stages = sum(r["learned"] for r in runs[True][0][5:])
+    t0 = sum(r["seconds"] for r in runs[False][0][5:])
+    t1 = sum(r["seconds"] for r in runs[True][0][5:])
+    print(f"iterations per step, CG + Jacobi: {its0}", flush=True)
+    print(f"iterations per step, flexible CG with the learned stage: {its1}", flush=True)
+    print(f"steps 6..12: {sum(its0[5:])} vs {sum(its1[5:])} iterations ({stages} stage applications), "
+          f"solve time {t0:.3f} s vs {t1:.3f} s", flush=True)
+    p0, p1 = runs[False][1], runs[True][1]
+    assert np.abs(p1 - p0).max() <= 1e-7 * np.abs(p0).max()
+    assert stages >= 7 * 10
+    assert sum(its1[5:]) < 0.97 * sum(its0[5:])
+    net.close()

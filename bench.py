@@ -192,14 +192,14 @@ def bench_ensemble(a):
     launches over all members, captured graphs).  Prints one JSON line."""
     import numpy as np
     from mizuho_edit_amd import io
-    from mizuho_edit_amd._lib import FLAG_CHEBYSHEV, FLAG_EXTRAPOLATE, FLAG_OP16
+    from mizuho_edit_amd._lib import FLAG_CHEBYSHEV, FLAG_EXTRAPOLATE, FLAG_OP16, FLAG_PROJECT
     from mizuho_edit_amd.ensemble import Ensemble
     from mizuho_edit_amd.wpfaw3d import WPFAW3D, solver_opts
     log(f"building the ensemble workload (shrink {a.ens_shrink})")
     cases = io.ensemble_cases(shrink=a.ens_shrink)
     cells = sum(c.params.nx * c.params.ny * c.params.nz for c in cases)
     opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4,
-                       flags=FLAG_EXTRAPOLATE | FLAG_CHEBYSHEV | FLAG_OP16)
+                       flags=FLAG_EXTRAPOLATE | FLAG_CHEBYSHEV | FLAG_OP16 | (0 if a.no_project else FLAG_PROJECT))
     res = {}
     for mode in ("per_instance", "ensemble"):
         grids = [WPFAW3D(c, device=0, opts=opts) for c in cases]
@@ -246,7 +246,9 @@ def bench_ensemble(a):
         "data": "synthetic: 48 seeded GRF rocks, correlation length 2-8 cells, mean porosity 0.12-0.25",
         "config": {"workload": "ensemble: 16x64^3 + 16x96x64x64 + 8x128x96x96 + 8x160x128x128"
                                + (f" (each dimension / {a.ens_shrink})" if a.ens_shrink > 1 else ""),
-                   "cells": cells, "solver": "FP32 MG-PCG (Chebyshev) in FP64 refinement, rtol 1e-10",
+                   "cells": cells,
+                   "solver": ("" if a.no_project else "projected initial guess (last 4 solution changes) + ")
+                   + "FP32 MG-PCG (Chebyshev) in FP64 refinement, rtol 1e-10",
                    **info},
     }
     print(json.dumps(out), flush=True)

@@ -24,6 +24,7 @@
 namespace wpf {
 
 constexpr int kEnsMaxLevels = 8;
+constexpr int kEnsProjDepth = 4;  // the single-grid ring depth
 
 // FP32 vectors of a member's inner solve
 enum EnsVec { EV_R = 0, EV_Z, EV_T, EV_PA, EV_PB, EV_Q, EV_E, EV_R2, EV_COUNT };
@@ -43,6 +44,9 @@ struct EnsState {
     double bnorm, tol, rn, rn0, inner_tol, dt0, dt1, ratio;
     int itime, cur, sel, have_prev, extrap;
     int outer_active, inner_active, its, stage_its, outer, converged;
+    // projected guess: ring position and fill of the last solution changes,
+    // and whether this solve projects (the first stage, with a non-empty ring)
+    int proj_next, proj_count, proj_on;
 };
 
 struct EnsMember {
@@ -64,6 +68,8 @@ struct EnsMember {
     float *tx32, *ty32, *tz32, *invD32;
     _Float16 *htx, *hty, *htz;
     float* v[EV_COUNT];
+    // the last solution changes (FP32 ring; slots never filled are zero)
+    float* proj[kEnsProjDepth];
     EnsLevel lv[kEnsMaxLevels];
     int nlev;
     // reductions, taumin keys, time scalars, breakthrough

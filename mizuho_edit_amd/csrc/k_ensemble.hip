@@ -5,12 +5,14 @@
 //   graph A  begin (step count, extrapolation test) -> permeability with
 //            the FP32 operator -> Dirichlet values -> extrapolated guess ->
 //            ||D^-1 b|| and the tolerance -> Galerkin levels -> S refinement
-//            stages, each: FP64 true residual, stage test, K iterations of
-//            FP32 MG-PCG (every launch skips members whose stage or inner
-//            solve is done), e added to p -> final FP64 residual test
+//            stages, each: FP64 true residual, stage test, (first stage,
+//            with PROJECT) projected guess, K iterations of FP32 MG-PCG
+//            (every launch skips members whose stage or inner solve is
+//            done), e added to p -> final FP64 residual test
 //   host     one copy of the member states; a member the schedule left
 //            unconverged is finished by the single-grid solver
-//   graph B  fused field update with the device dt rule -> end of step
+//   graph B  (PROJECT) the solution change into the ring -> fused field
+//            update with the device dt rule -> end of step
 // The launch sequence, its parameters and the reduction layouts are those
 // of the single-grid mixed solver (k_solver.hip), so a member's iterates
 // are the single-grid iterates.
@@ -33,7 +35,7 @@ struct wpf_ensemble {
     wpf_ensemble_opts eo{};
     int nlev = 0;
     int zc = 8, grid = 1024;
-    bool op16 = false, extrap = false;
+    bool op16 = false, extrap = false, project = false;
     std::vector<double> s32;                 // FP32 system scale per member
     wpf::EnsMember* d_mem = nullptr;         // device member table
     wpf::EnsState* d_state = nullptr;        // device states
@@ -50,13 +52,14 @@ struct wpf_ensemble {
 
 namespace wpf {
 
-enum EnsGate { G_ALL = 0, G_INNER = 1, G_STAGE = 2 };
+enum EnsGate { G_ALL = 0, G_INNER = 1, G_STAGE = 2, G_PROJ = 3 };
 enum EnsFamily { F_1D = 0, F_VEC = 1, F_VEC64 = 2 };
 
 __device__ __forceinline__ bool ens_on(const EnsMember& M, int gate)
 {
     if (gate == G_INNER) return M.st->inner_active != 0;
     if (gate == G_STAGE) return M.st->outer_active != 0;
+    if (gate == G_PROJ) return M.st->proj_on != 0;
     return true;
 }
 
@@ -208,6 +211,56 @@ __global__ void ke_final(const EnsMember* __restrict__ Ms, int n)
     s.outer_active = 0;
 }
 
+// ---- projected initial guess (project / finish of the single-grid solver) --
+// A solve projects in its first stage when the ring holds a solution change;
+// the ring always has kEnsProjDepth slots, and slots never filled are zero,
+// so their Gram rows are zero and proj_solve gives them y = 0.
+__global__ void ke_proj_begin(const EnsMember* __restrict__ Ms, int n)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    EnsState& s = *Ms[m].st;
+    s.proj_on = s.outer_active && s.outer == 1 && s.proj_count > 0;
+}
+
+// the ring, most recent change first
+__device__ __forceinline__ ProjVecs<float> ens_proj_vecs(const EnsMember& M)
+{
+    ProjVecs<float> V{};
+    const int next = M.st->proj_next;
+    for (int j = 0; j < kEnsProjDepth; j++) V.v[j] = M.proj[(next - 1 - j + 2 * kEnsProjDepth) % kEnsProjDepth];
+    return V;
+}
+
+__global__ void ke_proj_solve(const EnsMember* __restrict__ Ms, int n)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const EnsMember& M = Ms[m];
+    if (M.st->proj_on) proj_solve<kEnsProjDepth>(M.result + S_GRAM, M.result + S_PROJY);
+}
+
+// the inner tolerance from the projected residual
+__global__ void ke_proj_tol(const EnsMember* __restrict__ Ms, int n, double inner_rtol)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const EnsMember& M = Ms[m];
+    EnsState& s = *M.st;
+    if (!s.proj_on) return;
+    s.inner_tol = fmax(inner_rtol * sqrt(M.result[S_PN]), 0.5 * s.tol);
+    s.proj_on = 0;
+}
+
+__global__ void ke_proj_advance(const EnsMember* __restrict__ Ms, int n)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    EnsState& s = *Ms[m].st;
+    s.proj_next = (s.proj_next + 1) % kEnsProjDepth;
+    s.proj_count = min(s.proj_count + 1, kEnsProjDepth);
+}
+
 __global__ void ke_end_step(const EnsMember* __restrict__ Ms, int n, int extrap)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -253,6 +306,19 @@ __global__ void __launch_bounds__(256) ke_extrapolate(ENS_ARGS)
         psave[c] = v;
         if (s.extrap && !(M.mask[c] & WPF_MASK_FIXED)) M.p[c] = v + s.ratio * (v - pprev[c]);
     }
+    ENS_END
+}
+
+// this solve's change of the solution joins the ring (k_proj_push)
+__global__ void __launch_bounds__(256) ke_proj_push(ENS_ARGS)
+{
+    ENS_TILES(G_ALL)
+    const EnsState& s = *M.st;
+    const double* psave = M.xb[1 - s.sel];
+    float* v = M.proj[s.proj_next];
+    const Chunk ch = my_chunk_at(M.n, T.b, M.nb1);
+    for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads)
+        v[c] = (M.mask[c] & WPF_MASK_FIXED) ? 0.0f : (float)(M.p[c] - psave[c]);
     ENS_END
 }
 
@@ -341,6 +407,33 @@ __global__ void __launch_bounds__(256) ke_dir_mv(ENS_ARGS, int rz, int rpold, in
     ens_faces<C>(M, tx, ty, tz);
     double acc[1] = {t_vdir_mv<float, C, 4>(M.Z, T.b, tx, ty, tz, M.mask, M.v[rz], M.v[rpold], M.v[rpnew],
                                             M.v[EV_Q], M.result, num, den, first)};
+    block_reduce_store<1>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);
+    ENS_END
+}
+
+// Gram matrix of the ring and (v_j, r) (k_vgram) over vector tiles
+template <typename C>
+__global__ void __launch_bounds__(256) ke_vgram(ENS_ARGS)
+{
+    constexpr int NV = kEnsProjDepth * (kEnsProjDepth + 1) / 2 + kEnsProjDepth;
+    ENS_TILES(G_PROJ)
+    const C *tx, *ty, *tz;
+    ens_faces<C>(M, tx, ty, tz);
+    double acc[NV];
+    t_vgram<float, C, 4, kEnsProjDepth>(M.Z, T.b, ens_proj_vecs(M), M.v[EV_R], tx, ty, tz, M.mask, acc);
+    block_reduce_store<NV>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);
+    ENS_END
+}
+
+// x += V y, r -= A V y, ||invD r||^2 (k_vproj_update) over vector tiles
+template <typename C>
+__global__ void __launch_bounds__(256) ke_vproj_update(ENS_ARGS)
+{
+    ENS_TILES(G_PROJ)
+    const C *tx, *ty, *tz;
+    ens_faces<C>(M, tx, ty, tz);
+    double acc[1] = {t_vproj_update<float, C, 4, kEnsProjDepth>(M.Z, T.b, ens_proj_vecs(M), M.result + S_PROJY, M.p,
+                                                               M.v[EV_R], tx, ty, tz, M.invD32, M.mask)};
     block_reduce_store<1>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);
     ENS_END
 }
@@ -501,6 +594,7 @@ int build_tables(wpf_ensemble* e)
         X.htx = c->h_tx.p; X.hty = c->h_ty.p; X.htz = c->h_tz.p;
         X.v[EV_R] = c->r32.p; X.v[EV_Z] = c->z32.p; X.v[EV_T] = c->t32.p; X.v[EV_PA] = c->p32.p;
         X.v[EV_PB] = c->p32b.p; X.v[EV_Q] = c->q32.p; X.v[EV_E] = c->x32.p; X.v[EV_R2] = c->r32b.p;
+        for (int j = 0; j < kEnsProjDepth; j++) X.proj[j] = c->proj[j].p;
         X.nlev = e->nlev;
         for (int l = 1; l < e->nlev; l++) {
             MGLevel<float>& L = c->mg32[l];
@@ -655,6 +749,8 @@ int sched_vcycle(const Sched& S, int dot_slot, int* zres, int rb, bool pre_done)
     return WPF_OK;
 }
 
+static_assert(kEnsProjDepth == wpf_ctx::kProjDepth, "the ensemble shares the members' projection rings");
+
 // graph A: permeability and the pressure solve of every member
 template <typename C>
 int sched_solve(wpf_ensemble* e)
@@ -666,7 +762,8 @@ int sched_solve(wpf_ensemble* e)
     // permeability, faces, 1/D and the scaled FP32 operator in one pass
     ECHK(ens_perm(e->stream, e->d_mem, e->ts, e->ns, e->grid, e->op16 ? PERM_F16 : PERM_F32));
     ELAUNCH(ke_dirichlet, nm, 256, 0, e->d_mem);
-    if (e->extrap) ELAUNCH(ke_extrapolate, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
+    // saves the previous solution (extrapolation and projection) and extrapolates
+    if (e->extrap || e->project) ELAUNCH(ke_extrapolate, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
     for (int l = 1; l < e->nlev; l++)
         ELAUNCH(ke_coarsen, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, e->op16 ? 1 : 0);
     const float w1 = (float)S.mo.w1;
@@ -676,6 +773,16 @@ int sched_solve(wpf_ensemble* e)
         ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC64, 2, (int)S_T0, (int)G_STAGE);
         if (stage == 0) ELAUNCH(ke_tol, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.rtol, o.atol);
         ELAUNCH(ke_outer, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.inner_rtol, o.inner_maxit, o.maxit);
+        if (stage == 0 && e->project) {
+            constexpr int NV = kEnsProjDepth * (kEnsProjDepth + 1) / 2 + kEnsProjDepth;
+            ELAUNCH(ke_proj_begin, S.mblocks(), S.mthreads(), 0, e->d_mem, nm);
+            ELAUNCH((ke_vgram<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
+            ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, NV, (int)S_GRAM, (int)G_PROJ);
+            ELAUNCH(ke_proj_solve, S.mblocks(), S.mthreads(), 0, e->d_mem, nm);
+            ELAUNCH((ke_vproj_update<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
+            ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, 1, (int)S_PN, (int)G_PROJ);
+            ELAUNCH(ke_proj_tol, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.inner_rtol);
+        }
         // pcg_core
         int rz_cur = S_RZ0, rz_prev = S_RZ1;
         int zres = EV_Z;
@@ -721,8 +828,14 @@ int sched_solve(wpf_ensemble* e)
 int sched_fields(wpf_ensemble* e)
 {
     const int nm = (int)e->mem.size();
+    // the final pressure of every member is in place here, also of members
+    // the host finished after graph A
+    if (e->project) {
+        ELAUNCH(ke_proj_push, e->grid, 256, 0, e->d_mem, e->t1, e->n1);
+        ELAUNCH(ke_proj_advance, (nm + 63) / 64, 64, 0, e->d_mem, nm);
+    }
     ECHK(ens_step_fields(e->stream, e->d_mem, nm, e->ts, e->ns, e->grid));
-    ELAUNCH(ke_end_step, (nm + 63) / 64, 64, 0, e->d_mem, nm, e->extrap ? 1 : 0);
+    ELAUNCH(ke_end_step, (nm + 63) / 64, 64, 0, e->d_mem, nm, (e->extrap || e->project) ? 1 : 0);
     return WPF_OK;
 }
 
@@ -778,7 +891,6 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
     wpf_solver_opts o;
     solver_defaults(opts, &o);
     WPF_CHECK(o.solver == WPF_SOLVER_MIXED, WPF_ERR_ARG, "ensembles run the mixed solver");
-    WPF_CHECK(!(o.flags & WPF_FLAG_PROJECT), WPF_ERR_ARG, "the projected initial guess is single-grid only");
     WPF_CHECK(!(o.flags & (WPF_FLAG_LEARNED_WARM | WPF_FLAG_LEARNED_FCG | WPF_FLAG_CAPTURE | WPF_FLAG_F32_STENCIL)),
               WPF_ERR_ARG, "ensembles step without the learned stage, sample capture or FP32 stencils");
     WPF_CHECK(!(o.flags & WPF_FLAG_CHEBYSHEV) || o.mg_sweeps == 2, WPF_ERR_ARG,
@@ -804,6 +916,7 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
     e->eo = eo;
     e->op16 = (o.flags & WPF_FLAG_OP16) != 0;
     e->extrap = (o.flags & WPF_FLAG_EXTRAPOLATE) != 0;
+    e->project = (o.flags & WPF_FLAG_PROJECT) != 0;
     e->grid = eo.grid_blocks > 0 ? (eo.grid_blocks + 7) / 8 * 8 : 1024;
     e->zc = eo.tile_depth > 0 ? eo.tile_depth : 8;
     e->mem.assign(members, members + n);
@@ -828,6 +941,12 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
             if ((rc = dev_alloc(c->h_tx, c->g)) || (rc = dev_alloc(c->h_ty, c->g)) ||
                 (rc = dev_alloc(c->h_tz, c->g)))
                 return fail(rc);
+        }
+        // the projection ring of the member, shared with its own solver
+        if (e->project && !c->proj[0].raw) {
+            for (int j = 0; j < wpf_ctx::kProjDepth; j++)
+                if ((rc = dev_alloc(c->proj[j], c->g))) return fail(rc);
+            c->proj_count = c->proj_next = 0;
         }
         e->s32[m] = op32_scale(c, e->op16);
         const int L = (int)c->mg32.size();
@@ -872,6 +991,8 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
         s.have_prev = c->have_prev ? 1 : 0;
         s.dt0 = c->dt_hist[0];
         s.dt1 = c->dt_hist[1];
+        s.proj_next = c->proj_next;
+        s.proj_count = c->proj_count;
         const double sc[2] = {time[m], dtime[m]};
         if (hipMemcpy(c->scal, sc, sizeof(sc), hipMemcpyHostToDevice) != hipSuccess) return fail(WPF_ERR_HIP);
     }
@@ -1002,6 +1123,8 @@ int wpf_ensemble_sync(wpf_ensemble* e, int32_t* itime, double* time, double* dti
         c->have_prev = s.have_prev != 0;
         c->dt_hist[0] = s.dt0;
         c->dt_hist[1] = s.dt1;
+        c->proj_next = s.proj_next;
+        c->proj_count = s.proj_count;
         if (s.sel) {
             std::swap(c->xprev, c->psave);
             s.sel = 0;

@@ -31,7 +31,8 @@ def _pair(flags, zc=4):
     return cases, opts, ens_m, solo
 
 
-@pytest.mark.parametrize("flags", [FLAGS, FLAGS | _lib.FLAG_OP16])
+@pytest.mark.parametrize("flags", [FLAGS, FLAGS | _lib.FLAG_OP16,
+                                   FLAGS | _lib.FLAG_OP16 | _lib.FLAG_PROJECT])
 def test_ensemble_equals_single_grid_steps(gpu, flags):
     cases, opts, ens_m, solo = _pair(flags)
     # two single-grid steps first: the ensemble takes over mid-run
@@ -57,8 +58,9 @@ def test_ensemble_equals_single_grid_steps(gpu, flags):
     ens.close()
 
 
-def test_short_schedule_falls_back_to_the_single_grid_solver(gpu):
-    cases, opts, ens_m, solo = _pair(FLAGS)
+@pytest.mark.parametrize("flags", [FLAGS, FLAGS | _lib.FLAG_PROJECT])
+def test_short_schedule_falls_back_to_the_single_grid_solver(gpu, flags):
+    cases, opts, ens_m, solo = _pair(flags)
     ens = Ensemble(ens_m, opts, stages=1, inner_iters=2, tile_depth=4, grid_blocks=256)
     st = ens.step(6)
     ens.sync()

@@ -765,8 +765,9 @@ __global__ void __launch_bounds__(256) k_vdir_mv(VZ Z, const C* __restrict__ tx,
 // The FP64 true residual r = f - A x of the mixed solver in vector form
 // (V = 2 FP64 cells per thread): writes the scaled FP32 copy s32 r, reduces
 // ||invD r||^2 into acc[0] and, with bn, ||D^-1 b||^2 of the reference
-// system (bnorm_cell, per cell) into acc[1].  d and s in the order of
-// row_parts; fixed cells get r = 0.
+// system (the per-cell values of bnorm_cell, from the neighbours' x and
+// masks in registers) into acc[1].  d and s in the order of row_parts;
+// fixed cells get r = 0.
 template <int V>
 __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<double>& L,
                                               const double* __restrict__ x, const double* __restrict__ f,
@@ -779,10 +780,12 @@ __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<doub
     const int64_t nx = Z.nx, pl = Z.plane;
     acc[0] = acc[1] = 0.0;
     double xm[V], x0[V], xp[V], tzm[V];
+    uint8_t mzm[V];  // masks of the plane below (bn)
     int64_t c = q + pl * k0;
     vld<V>(x + c - pl, xm);
     vld<V>(x + c, x0);
     vld<V>(L.tz + c - pl, tzm);
+    if (bn) vld<V>(L.mask + c - pl, mzm);
     for (int k = k0; k < k1; k++, c += pl) {
         vld<V>(x + c + pl, xp);
         double xyp[V], xym[V], txv[V], tyv[V], tym[V], tzv[V], di[V], fv[V];
@@ -807,6 +810,18 @@ __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<doub
             txl = L.tx[c - 1];
         }
         if (lane == 63) xr = x[c + V];
+        // bn: the neighbours' masks, for the Dirichlet terms of b
+        uint8_t mzp[V], myp[V], mym[V];
+        int ml = 0, mr = 0;
+        if (bn) {
+            vld<V>(L.mask + c + pl, mzp);
+            vld<V>(L.mask + c + nx, myp);
+            vld<V>(L.mask + c - nx, mym);
+            ml = from_left((int)m[V - 1]);
+            mr = from_right((int)m[0]);
+            if (lane == 0) ml = L.mask[c - 1];
+            if (lane == 63) mr = L.mask[c + V];
+        }
         float rv32[V];
 #pragma unroll
         for (int h = 0; h < V; h++) {
@@ -822,7 +837,25 @@ __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<doub
             if (act) acc[0] += zv * zv;
             rv32[h] = (float)(rv * s32);
             if (bn && act) {
-                const double v = bnorm_cell(L, x, f, c + h);
+                // bnorm_cell from the values in registers: the same terms
+                // in the same order
+                double v;
+                if (m[h] & WPF_MASK_FIXED) {
+                    v = x0[h];
+                } else {
+                    const int mxl = h ? m[h - 1] : ml, mxr = (h + 1 < V) ? m[h + 1] : mr;
+                    double sb = 0.0;
+                    auto add = [&](double t, int mk, double xv) {
+                        if (t != 0.0) sb += t * ((mk & WPF_MASK_FIXED) ? xv : 0.0);
+                    };
+                    add(txv[h], mxr, xre);
+                    add(txm, mxl, xle);
+                    add(tyv[h], myp[h], xyp[h]);
+                    add(tym[h], mym[h], xym[h]);
+                    add(tzv[h], mzp[h], xp[h]);
+                    add(tzm[h], mzm[h], xm[h]);
+                    v = (fv[h] + sb) * di[h];
+                }
                 acc[1] += v * v;
             }
         }
@@ -832,6 +865,7 @@ __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<doub
             xm[h] = x0[h];
             x0[h] = xp[h];
             tzm[h] = tzv[h];
+            if (bn) mzm[h] = m[h];
         }
     }
 }

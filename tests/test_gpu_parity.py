@@ -165,9 +165,12 @@ def test_pressure_solve(gpu, solver, flags):
     assert info["converged"] == 1
     st.calc_permeability()
     st.pressure[...] = p0 * 0.5
-    st.solve_pressure(rtol=TIGHT, maxit=20000)
+    _, _, _, bn_ref = st.solve_pressure(rtol=TIGHT, maxit=20000)
     pg, pr = g.pressure, st.pressure
     assert np.abs(pg - pr).max() <= 1e-10 * np.abs(pr).max()
+    # ||D^-1 b||, the scale of the stopping test (the mixed solver sums it
+    # in its first residual pass, the others in a pass of its own)
+    assert info["bnorm"] == pytest.approx(bn_ref, rel=1e-12)
     # the GPU's own residual claim checked by the oracle's residual
     st.calc_permeability()
     assert st.pressure_residual(pg) <= 1.01 * TIGHT * info["bnorm"] + 1e-300

@@ -456,12 +456,13 @@ __global__ void __launch_bounds__(256) ke_coarsen(ENS_ARGS, int l, int h16)
     ENS_TILES(G_ALL)
     const EnsLevel& C = M.lv[l];
     const LV<float> F = ens_lv32(M, l - 1);
+    // members have nx % 4 == 0: level 1 reads the fine rows as pairs
     if (l == 1 && h16)
-        t_coarsen<float, true, _Float16>(F, M.htx, M.hty, M.htz, C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D,
+        t_coarsen_pairs<float, _Float16>(F, M.htx, M.hty, M.htz, C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D,
                                          C.invD, T.b, C.nb);
     else if (l == 1)
-        t_coarsen<float, true>(F, F.tx, F.ty, F.tz, C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D, C.invD, T.b,
-                               C.nb);
+        t_coarsen_pairs<float, float>(F, F.tx, F.ty, F.tz, C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D,
+                                      C.invD, T.b, C.nb);
     else
         t_coarsen<float, false>(F, F.tx, F.ty, F.tz, C.nx, C.ny, C.nz, C.plane, C.tx, C.ty, C.tz, C.D, C.invD,
                                 T.b, C.nb);

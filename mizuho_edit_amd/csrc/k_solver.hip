@@ -142,7 +142,15 @@ static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv, bool h16 = false
         int64_t nb = (C.g.n + kThreads - 1) / kThreads;
         if (nb >= 8) nb = nb / 8 * 8;
         const dim3 grid((unsigned)nb), block(kThreads);
-        if (fine && h16)
+        const bool pairs = F.nx % 2 == 0;  // aggregate rows as aligned pairs
+        if (fine && h16 && pairs)
+            hipLaunchKernelGGL((k_coarsen_pairs<R, _Float16>), grid, block, 0, c->stream, F,
+                               (const _Float16*)c->h_tx.p, (const _Float16*)c->h_ty.p, (const _Float16*)c->h_tz.p,
+                               C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
+        else if (fine && pairs)
+            hipLaunchKernelGGL((k_coarsen_pairs<R, R>), grid, block, 0, c->stream, F, F.tx, F.ty, F.tz, C.g.nx,
+                               C.g.ny, C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
+        else if (fine && h16)
             hipLaunchKernelGGL((k_coarsen<R, true, _Float16>), grid, block, 0, c->stream, F,
                                (const _Float16*)c->h_tx.p, (const _Float16*)c->h_ty.p, (const _Float16*)c->h_tz.p,
                                C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);

@@ -880,4 +880,87 @@ __global__ void __launch_bounds__(256) k_vtrue_resid(VZ Z, LV<double> L, const d
     block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
 }
 
+// Galerkin level 1 (t_coarsen with FINE) where fine rows have even length:
+// the two fine cells of an aggregate row are one aligned pair, so each face
+// and mask array is read with one 2-value load per fine row (and one value
+// for the -x face and one mask for the +x neighbour) instead of one load
+// per cell and face.  Same chunking, sums and order of additions as
+// t_coarsen, hence the same coarse operator.
+template <typename R, typename CF>
+__device__ __forceinline__ void t_coarsen_pairs(LV<R> L, const CF* __restrict__ fx, const CF* __restrict__ fy,
+                                                const CF* __restrict__ fz, int cnx, int cny, int cnz, int64_t cplane,
+                                                R* __restrict__ ctx, R* __restrict__ cty, R* __restrict__ ctz,
+                                                R* __restrict__ cD, R* __restrict__ cinvD, int b_, int nb_)
+{
+    const int64_t nc = (int64_t)cnx * cny * cnz;
+    const Chunk ch = my_chunk_at(nc, b_, nb_);
+    for (int64_t C = ch.begin + threadIdx.x; C < ch.end; C += kThreads) {
+        const int I = (int)(C % cnx);
+        const int J = (int)((C / cnx) % cny);
+        const int K = (int)(C / cplane);
+        const int i = 2 * I;
+        R D = 0, tx = 0, ty = 0, tz = 0;
+        for (int dk = 0; dk < 2; dk++) {
+            const int k = 2 * K + dk;
+            if (k >= L.nz) break;
+            for (int dj = 0; dj < 2; dj++) {
+                const int j = 2 * J + dj;
+                if (j >= L.ny) break;
+                const int64_t c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
+                CF ftx[2], fty[2], ftym[2], ftz[2], ftzm[2];
+                uint8_t m[2], myp[2], mzp[2];
+                vld<2>(fx + c, ftx);
+                vld<2>(fy + c, fty);
+                vld<2>(fy + c - L.nx, ftym);
+                vld<2>(fz + c, ftz);
+                vld<2>(fz + c - L.plane, ftzm);
+                vld<2>(L.mask + c, m);
+                vld<2>(L.mask + c + L.nx, myp);
+                vld<2>(L.mask + c + L.plane, mzp);
+                const R txl0 = (R)fx[c - 1];
+                const uint8_t mx2 = L.mask[c + 2];
+#pragma unroll
+                for (int di = 0; di < 2; di++) {
+                    if (m[di] & WPF_MASK_FIXED) continue;
+                    const R txc = (R)ftx[di], txl = di ? (R)ftx[0] : txl0;
+                    const R tyc = (R)fty[di], tzc = (R)ftz[di];
+                    D += txc + txl + tyc + (R)ftym[di] + tzc + (R)ftzm[di];
+                    const uint8_t mx = di ? mx2 : m[1];
+                    if (i + di + 1 < L.nx && !(mx & WPF_MASK_FIXED)) {
+                        if (di == 0) D -= (R)2 * txc;
+                        else tx += txc;
+                    }
+                    if (j + 1 < L.ny && !(myp[di] & WPF_MASK_FIXED)) {
+                        if (dj == 0) D -= (R)2 * tyc;
+                        else ty += tyc;
+                    }
+                    if (k + 1 < L.nz && !(mzp[di] & WPF_MASK_FIXED)) {
+                        if (dk == 0) D -= (R)2 * tzc;
+                        else tz += tzc;
+                    }
+                }
+            }
+        }
+        if (I == cnx - 1) tx = 0;
+        if (J == cny - 1) ty = 0;
+        if (K == cnz - 1) tz = 0;
+        ctx[C] = tx;
+        cty[C] = ty;
+        ctz[C] = tz;
+        cD[C] = D;
+        cinvD[C] = D > (R)0 ? (R)1 / D : (R)0;
+    }
+}
+template <typename R, typename CF>
+__global__ void __launch_bounds__(kThreads) k_coarsen_pairs(LV<R> L, const CF* __restrict__ fx,
+                                                            const CF* __restrict__ fy, const CF* __restrict__ fz,
+                                                            int cnx, int cny, int cnz, int64_t cplane,
+                                                            R* __restrict__ ctx, R* __restrict__ cty,
+                                                            R* __restrict__ ctz, R* __restrict__ cD,
+                                                            R* __restrict__ cinvD)
+{
+    t_coarsen_pairs<R, CF>(L, fx, fy, fz, cnx, cny, cnz, cplane, ctx, cty, ctz, cD, cinvD, (int)blockIdx.x,
+                           (int)gridDim.x);
+}
+
 }  // namespace wpf

@@ -133,8 +133,9 @@ static int ensure_hierarchy(std::vector<MGLevel<R>>& lv, int nlevels)
 // its FP16-stored copy (whose decoded values the FP32 faces would hold).
 // (no reduction: one coarse cell per thread, as many blocks as that takes)
 template <typename R>
-static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv, bool h16 = false)
+static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv, bool h16 = false, hipStream_t st = nullptr)
 {
+    if (!st) st = c->stream;
     for (size_t l = 1; l < lv.size(); l++) {
         MGLevel<R>& C = lv[l];
         const bool fine = (l == 1);
@@ -144,21 +145,21 @@ static int coarsen_all(wpf_ctx* c, std::vector<MGLevel<R>>& lv, bool h16 = false
         const dim3 grid((unsigned)nb), block(kThreads);
         const bool pairs = F.nx % 2 == 0;  // aggregate rows as aligned pairs
         if (fine && h16 && pairs)
-            hipLaunchKernelGGL((k_coarsen_pairs<R, _Float16>), grid, block, 0, c->stream, F,
+            hipLaunchKernelGGL((k_coarsen_pairs<R, _Float16>), grid, block, 0, st, F,
                                (const _Float16*)c->h_tx.p, (const _Float16*)c->h_ty.p, (const _Float16*)c->h_tz.p,
                                C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
         else if (fine && pairs)
-            hipLaunchKernelGGL((k_coarsen_pairs<R, R>), grid, block, 0, c->stream, F, F.tx, F.ty, F.tz, C.g.nx,
+            hipLaunchKernelGGL((k_coarsen_pairs<R, R>), grid, block, 0, st, F, F.tx, F.ty, F.tz, C.g.nx,
                                C.g.ny, C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
         else if (fine && h16)
-            hipLaunchKernelGGL((k_coarsen<R, true, _Float16>), grid, block, 0, c->stream, F,
+            hipLaunchKernelGGL((k_coarsen<R, true, _Float16>), grid, block, 0, st, F,
                                (const _Float16*)c->h_tx.p, (const _Float16*)c->h_ty.p, (const _Float16*)c->h_tz.p,
                                C.g.nx, C.g.ny, C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
         else if (fine)
-            hipLaunchKernelGGL((k_coarsen<R, true>), grid, block, 0, c->stream, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny,
+            hipLaunchKernelGGL((k_coarsen<R, true>), grid, block, 0, st, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny,
                                C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
         else
-            hipLaunchKernelGGL((k_coarsen<R, false>), grid, block, 0, c->stream, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny,
+            hipLaunchKernelGGL((k_coarsen<R, false>), grid, block, 0, st, F, F.tx, F.ty, F.tz, C.g.nx, C.g.ny,
                                C.g.nz, C.g.plane, C.tx.p, C.ty.p, C.tz.p, C.D.p, C.invD.p);
         WPF_HIP(hipGetLastError());
     }
@@ -1113,7 +1114,30 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         else
             LAUNCH1D(k_to_f32, n, L, s, F32.tx.p, F32.ty.p, F32.tz.p, F32.invD.p);
     }
-    if ((rc = coarsen_all(c, c->mg32, op16))) return rc;
+    // the Galerkin levels are built on the second stream while the main
+    // stream runs the first residual and the projection; the first V-cycle
+    // waits for them
+    if (!c->aux) {
+        WPF_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+        WPF_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        WPF_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
+    WPF_HIP(hipEventRecord(c->ev_fork, c->stream));
+    WPF_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    if ((rc = coarsen_all(c, c->mg32, op16, c->aux))) return rc;
+    WPF_HIP(hipEventRecord(c->ev_join, c->aux));
+    // every way out of this solve joins the second stream first: the next
+    // permeability pass rewrites the faces the Galerkin build reads
+    struct Join {
+        wpf_ctx* c;
+        bool done = false;
+        void now()
+        {
+            if (!done) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+            done = true;
+        }
+        ~Join() { now(); }
+    } join{c};
     const MGOpts mo = mg_opts(o);
     Fine<float, float> F;
     F.g = c->g;
@@ -1185,6 +1209,7 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         int k = 0;
         double rr = 0.0;
         const int cap = std::min(o.inner_maxit, o.maxit - its);
+        join.now();
         rc = op16 ? pcg_core<float, float, _Float16>(c, &c->mg32, H, mo, e, inner_tol, cap, &k, &rr)
                   : pcg_core<float, float, float>(c, &c->mg32, F, mo, e, inner_tol, cap, &k, &rr);
         if (rc) return rc;

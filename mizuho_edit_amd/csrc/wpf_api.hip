@@ -303,6 +303,10 @@ int wpf_destroy(wpf_ctx* c)
     dev_free(c->r); dev_free(c->z); dev_free(c->pp); dev_free(c->q); dev_free(c->xprev);
     reducer_free(c->red);
     for (auto& e : c->prof_ev) (void)hipEventDestroy(e);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return WPF_OK;

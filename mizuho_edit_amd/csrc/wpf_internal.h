@@ -100,6 +100,10 @@ struct Reducer {
 struct wpf_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
+    // second stream (default priority) for work that overlaps the main
+    // stream inside one call, forked and joined with the two events
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     wpf_params prm{};
     wpf::Grid g;
 

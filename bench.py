@@ -201,10 +201,13 @@ def bench_ensemble(a):
     opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4,
                        flags=FLAG_EXTRAPOLATE | FLAG_CHEBYSHEV | FLAG_OP16 | (0 if a.no_project else FLAG_PROJECT))
     res = {}
+    # the ensemble takes the last warm-up steps itself, so that its adaptive
+    # schedule has settled before the timed steps (the same steps both ways)
+    settle = min(10, a.warmup)
     for mode in ("per_instance", "ensemble"):
         grids = [WPFAW3D(c, device=0, opts=opts) for c in cases]
         for g in grids:
-            g.run(a.warmup)
+            g.run(a.warmup - (settle if mode == "ensemble" else 0))
         sync()
         if mode == "per_instance":
             t0 = time.perf_counter()
@@ -218,12 +221,14 @@ def bench_ensemble(a):
         else:
             ens = Ensemble(grids, opts)
             tg, tz = ens.tune()
-            ens.step(1)  # captures the graphs
-            log(f"  ensemble tuned: {tg} workgroups, {tz} planes per tile")
+            s0 = ens.step(settle)  # captures the graphs, adapts the schedule
+            log(f"  ensemble tuned: {tg} workgroups, {tz} planes per tile; schedule "
+                f"{s0['stages']} stages, {s0['inner_iters']} iteration slots")
             st = ens.step(a.steps)
             wall = st["seconds"]
             ens.sync()
             info = {"grid_blocks": tg, "tile_depth": tz, "fallbacks": st["fallbacks"],
+                    "schedule": f"{st['stages']} stages, {st['inner_iters']} FP32 iteration slots in all (adaptive)",
                     "fp32_iterations_per_member_step": st["iterations"] / (a.steps * len(grids))}
             ens.close()
         res[mode] = (wall, info)

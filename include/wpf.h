@@ -322,8 +322,13 @@ int wpf_set_tile_depth(wpf_ctx* ctx, int32_t planes);
  * the members back for single-grid use. */
 typedef struct wpf_ensemble wpf_ensemble;
 typedef struct wpf_ensemble_opts {
-    int32_t stages;       /* refinement stages per solve in the schedule (3)  */
-    int32_t inner_iters;  /* FP32 iterations per stage in the schedule (16)   */
+    int32_t stages;       /* refinement stages per solve in the schedule      */
+    int32_t inner_iters;  /* FP32 iterations per stage in the schedule; with
+                             both 0 the schedule adapts between steps: it
+                             starts at 3 x 16, a stage grows when a member
+                             used all of it, and every 8 steps each stage
+                             shrinks to its most used + 2 iterations, with
+                             one spare stage of 4                           */
     int32_t grid_blocks;  /* workgroups of every batched launch (0: tune)     */
     int32_t tile_depth;   /* planes per tile of the vector kernels (0: tune)  */
 } wpf_ensemble_opts;
@@ -332,6 +337,8 @@ typedef struct wpf_ensemble_stats {
     int32_t fallbacks;    /* member solves finished by the single-grid solver */
     int64_t iterations;   /* FP32 inner iterations, all members and steps     */
     double  seconds;      /* wall time of the steps                           */
+    int32_t stages;       /* the schedule after the last step: stages and     */
+    int32_t inner_iters;  /* FP32 iteration slots summed over them            */
 } wpf_ensemble_stats;
 int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opts* opts,
                         const wpf_ensemble_opts* eopts, const int32_t* itime,

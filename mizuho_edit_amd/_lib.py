@@ -65,7 +65,8 @@ class WpfEnsembleOpts(ctypes.Structure):
 
 class WpfEnsembleStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int32), ("fallbacks", ctypes.c_int32),
-                ("iterations", ctypes.c_int64), ("seconds", ctypes.c_double)]
+                ("iterations", ctypes.c_int64), ("seconds", ctypes.c_double),
+                ("stages", ctypes.c_int32), ("inner_iters", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -25,6 +25,7 @@ namespace wpf {
 
 constexpr int kEnsMaxLevels = 8;
 constexpr int kEnsProjDepth = 4;  // the single-grid ring depth
+constexpr int kEnsMaxStages = 8;  // refinement stages of a schedule
 
 // FP32 vectors of a member's inner solve
 enum EnsVec { EV_R = 0, EV_Z, EV_T, EV_PA, EV_PB, EV_Q, EV_E, EV_R2, EV_COUNT };
@@ -47,6 +48,7 @@ struct EnsState {
     // projected guess: ring position and fill of the last solution changes,
     // and whether this solve projects (the first stage, with a non-empty ring)
     int proj_next, proj_count, proj_on;
+    int stage_used[kEnsMaxStages];  // FP32 iterations each stage of this step used
 };
 
 struct EnsMember {

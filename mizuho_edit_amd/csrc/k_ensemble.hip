@@ -48,6 +48,13 @@ struct wpf_ensemble {
     size_t coarsest_lds = 0;
     size_t up_lds = 0;  // LDS of the staged fused update (0: the global-load form)
     hipGraphExec_t gA = nullptr, gB = nullptr;
+    // the schedule: FP32 iteration slots of each refinement stage
+    std::vector<int> sched;
+    // adaptive schedule: the most used stages and iterations of each stage
+    // over the current window of steps
+    bool adaptive = false;
+    int win_stages = 0, win_steps = 0;
+    int win_used[wpf::kEnsMaxStages] = {};
 };
 
 namespace wpf {
@@ -134,6 +141,7 @@ __global__ void ke_begin(const EnsMember* __restrict__ Ms, int n, int extrap)
     s.converged = 0;
     s.outer_active = 1;
     s.inner_active = 0;
+    for (int k = 0; k < kEnsMaxStages; k++) s.stage_used[k] = 0;
 }
 
 __global__ void ke_tol(const EnsMember* __restrict__ Ms, int n, double rtol, double atol)
@@ -186,6 +194,7 @@ __global__ void ke_inner(const EnsMember* __restrict__ Ms, int n, int inner_maxi
     const double rn = sqrt(M.result[S_RN]);
     s.stage_its++;
     s.its++;
+    if (s.outer >= 1 && s.outer <= kEnsMaxStages) s.stage_used[s.outer - 1] = s.stage_its;
     if (!isfinite(rn)) {
         s.converged = -1;
         s.inner_active = 0;
@@ -752,6 +761,45 @@ int sched_vcycle(const Sched& S, int dot_slot, int* zres, int rb, bool pre_done)
 
 static_assert(kEnsProjDepth == wpf_ctx::kProjDepth, "the ensemble shares the members' projection rings");
 
+// Adaptive schedule, after a step whose members used at most need_stages
+// stages and need[k] iterations in stage k (cut: a member was not converged
+// by the schedule).  A stage some member used up grows at once (and a
+// stage is added when all were used); otherwise every 8 steps each stage
+// shrinks to the window's most used plus two iterations, and the schedule
+// to the most used stages plus one spare stage of 4, so the graphs carry
+// few launches that every member skips.  A change drops the graphs
+// (captured again on the next step).  Members never see a stage cut short
+// while each stage is longer than what they use, so their arithmetic stays
+// the single-grid one.
+void adapt_schedule(wpf_ensemble* e, int need_stages, const int* need, bool cut)
+{
+    std::vector<int> s = e->sched;
+    const int ns = (int)s.size();
+    e->win_stages = std::max(e->win_stages, need_stages);
+    for (int k = 0; k < ns; k++) e->win_used[k] = std::max(e->win_used[k], need[k]);
+    e->win_steps++;
+    bool grow = cut || need_stages >= ns;
+    for (int k = 0; k < ns; k++) grow |= need[k] >= s[k];
+    if (grow) {
+        for (int k = 0; k < ns; k++)
+            if (cut || need[k] >= s[k]) s[k] = std::min(64, std::max(need[k] + 4, s[k] * 3 / 2));
+        if ((cut || need_stages >= ns) && ns < kEnsMaxStages) s.push_back(std::max(4, s.back()));
+    } else if (e->win_steps >= 8) {
+        const int keep = std::min(ns, std::max(1, e->win_stages + 1));
+        s.resize(keep);
+        for (int k = 0; k < keep; k++) s[k] = std::min(s[k], std::max(2, e->win_used[k] + 2));
+        if (keep > e->win_stages) s[keep - 1] = std::min(s[keep - 1], 4);
+    } else {
+        return;
+    }
+    e->win_stages = e->win_steps = 0;
+    for (int k = 0; k < kEnsMaxStages; k++) e->win_used[k] = 0;
+    if (s != e->sched) {
+        e->sched = s;
+        drop_graphs(e);
+    }
+}
+
 // graph A: permeability and the pressure solve of every member
 template <typename C>
 int sched_solve(wpf_ensemble* e)
@@ -768,7 +816,8 @@ int sched_solve(wpf_ensemble* e)
     for (int l = 1; l < e->nlev; l++)
         ELAUNCH(ke_coarsen, e->grid, 256, 0, e->d_mem, e->tl[l], e->nl[l], l, e->op16 ? 1 : 0);
     const float w1 = (float)S.mo.w1;
-    for (int stage = 0; stage < e->eo.stages; stage++) {
+    for (int stage = 0; stage < (int)e->sched.size(); stage++) {
+        const int inner_iters = e->sched[stage];
         // the first stage's residual pass also gives ||D^-1 b|| and the tolerance
         ELAUNCH(ke_true_resid, e->grid, 256, 0, e->d_mem, e->t64, e->n64, stage == 0 ? 1 : 0);
         ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC64, 2, (int)S_T0, (int)G_STAGE);
@@ -792,7 +841,7 @@ int sched_solve(wpf_ensemble* e)
         ELAUNCH(ke_zero_smooth0, e->grid, 256, 0, e->d_mem, e->t1, e->n1, rr, w1);
         ECHK(sched_vcycle<C>(S, rz_cur, &zres, rr, false));
         int pc = EV_PA, pn = EV_PB;
-        for (int it = 0; it < e->eo.inner_iters; it++) {
+        for (int it = 0; it < inner_iters; it++) {
             ELAUNCH((ke_dir_mv<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, zres, pc, pn, rz_cur, rz_prev,
                     it == 0 ? 1 : 0);
             ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, 1, (int)S_PQ, (int)G_INNER);
@@ -813,7 +862,7 @@ int sched_solve(wpf_ensemble* e)
                 ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_1D, 2, (int)S_RN, (int)G_INNER);
             }
             ELAUNCH(ke_inner, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.inner_maxit, o.maxit);
-            if (it + 1 == e->eo.inner_iters) break;
+            if (it + 1 == inner_iters) break;
             std::swap(rz_cur, rz_prev);
             ECHK(sched_vcycle<C>(S, rz_cur, &zres, rr, fuse));
         }
@@ -903,11 +952,13 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
         WPF_CHECK(members[m]->g.nx % 4 == 0, WPF_ERR_ARG, "member %d: nx = %d is not a multiple of 4", m,
                   members[m]->g.nx);
     }
+    WPF_CHECK(!eopts || eopts->stages <= kEnsMaxStages, WPF_ERR_ARG, "at most %d stages", kEnsMaxStages);
     WPF_HIP(hipSetDevice(dev));
     wpf_ensemble* e = new wpf_ensemble();
     e->dev = dev;
     e->o = o;
     wpf_ensemble_opts eo{3, 16, 0, 0};
+    e->adaptive = !eopts || (eopts->stages <= 0 && eopts->inner_iters <= 0);
     if (eopts) {
         if (eopts->stages > 0) eo.stages = eopts->stages;
         if (eopts->inner_iters > 0) eo.inner_iters = eopts->inner_iters;
@@ -915,6 +966,7 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
         eo.tile_depth = eopts->tile_depth;
     }
     e->eo = eo;
+    e->sched.assign(eo.stages, eo.inner_iters);
     e->op16 = (o.flags & WPF_FLAG_OP16) != 0;
     e->extrap = (o.flags & WPF_FLAG_EXTRAPOLATE) != 0;
     e->project = (o.flags & WPF_FLAG_PROJECT) != 0;
@@ -1079,17 +1131,22 @@ int wpf_ensemble_step(wpf_ensemble* e, int32_t nsteps, wpf_ensemble_stats* st)
     wpf_ensemble_stats local{};
     if (!st) st = &local;
     *st = wpf_ensemble_stats{};
-    int rc = ensure_graphs(e);
-    if (rc) return rc;
+    int rc;
     const int nm = (int)e->mem.size();
     const auto t0 = std::chrono::steady_clock::now();
     for (int s = 0; s < nsteps; s++) {
+        if ((rc = ensure_graphs(e))) return rc;
         WPF_HIP(hipGraphLaunch(e->gA, e->stream));
         WPF_HIP(hipMemcpyAsync(e->h_state, e->d_state, sizeof(EnsState) * nm, hipMemcpyDeviceToHost, e->stream));
         WPF_HIP(hipStreamSynchronize(e->stream));
+        int need_stages = 0, need[kEnsMaxStages] = {};
+        bool cut = false;
         for (int m = 0; m < nm; m++) {
             const EnsState& x = e->h_state[m];
             st->iterations += x.its;
+            need_stages = std::max(need_stages, x.outer);
+            for (int k = 0; k < kEnsMaxStages; k++) need[k] = std::max(need[k], x.stage_used[k]);
+            cut |= x.converged != 1;
             if (x.converged == 1) continue;
             if (x.converged < 0) {
                 set_error("ensemble member %d: pressure residual is not finite", m);
@@ -1104,7 +1161,11 @@ int wpf_ensemble_step(wpf_ensemble* e, int32_t nsteps, wpf_ensemble_stats* st)
         }
         WPF_HIP(hipGraphLaunch(e->gB, e->stream));
         st->steps++;
+        if (e->adaptive) adapt_schedule(e, need_stages, need, cut);
     }
+    st->stages = (int32_t)e->sched.size();
+    st->inner_iters = 0;
+    for (int k : e->sched) st->inner_iters += k;
     WPF_HIP(hipStreamSynchronize(e->stream));
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return WPF_OK;

@@ -21,7 +21,7 @@ class Ensemble:
     needed).  After sync() the members may be stepped alone; the ensemble
     itself is then done with (make a new one to batch them again)."""
 
-    def __init__(self, members: Sequence[WPFAW3D], opts, stages: int = 3, inner_iters: int = 16,
+    def __init__(self, members: Sequence[WPFAW3D], opts, stages: int = 0, inner_iters: int = 0,
                  grid_blocks: int = 0, tile_depth: int = 0):
         self.members: List[WPFAW3D] = list(members)
         n = len(self.members)

@@ -88,3 +88,26 @@ def test_tuner_picks_a_candidate_and_steps(gpu):
     ens.close()
     for m in ens_m:
         assert m.itime == 3
+
+
+def test_adaptive_schedule_shrinks_and_matches(gpu):
+    """With no schedule given the ensemble adapts it between steps: after 8
+    steps it shrinks to what the members used, and the members still follow
+    the single-grid results."""
+    cases, opts, ens_m, solo = _pair(FLAGS | _lib.FLAG_OP16 | _lib.FLAG_PROJECT)
+    for g in ens_m + solo:
+        g.run(2)
+    ens = Ensemble(ens_m, opts, tile_depth=4, grid_blocks=256)
+    st = ens.step(12)
+    ens.sync()
+    print("ensemble stats", st, flush=True)
+    assert st["steps"] == 12
+    assert st["inner_iters"] < 3 * 16  # iteration slots summed over the stages
+    for g in solo:
+        g.run(12)
+    for a, b in zip(ens_m, solo):
+        assert a.itime == b.itime
+        for name in ("pressure", "phase", "concent"):
+            x, y = getattr(a, name), getattr(b, name)
+            assert np.abs(x - y).max() <= 1e-6 * np.abs(y).max(), name
+    ens.close()

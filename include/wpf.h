@@ -306,14 +306,14 @@ int wpf_set_tile_depth(wpf_ctx* ctx, int32_t planes);
  * independent grids of different sizes on one GPU, stepped together.  Every
  * operation of a time step is one launch over the blocks of all members
  * (a fixed, tuned grid of workgroups walking a table of member blocks); the
- * pressure solve runs a fixed schedule of FP64 refinement stages and FP32
- * MG-PCG iterations with per-member convergence decided on the device, so a
- * whole step is replayed from captured HIP graphs with one host
- * synchronisation.  A member the schedule leaves unconverged is finished by
+ * pressure solve runs a schedule of FP64 refinement stages and FP32 MG-PCG
+ * iterations (fixed, or adapted between steps) with per-member convergence
+ * decided on the device, so a whole step is replayed from captured HIP
+ * graphs with one host synchronisation.  A member the schedule leaves unconverged is finished by
  * the single-grid solver before its fields are advanced.  Each member
  * follows the arithmetic of the same grid stepped alone with wpf_step and
- * the same options (WPF_SOLVER_MIXED; WPF_FLAG_CHEBYSHEV, _OP16 and
- * _EXTRAPOLATE honoured; FP64 stencils; no learned stage).
+ * the same options (WPF_SOLVER_MIXED; WPF_FLAG_CHEBYSHEV, _OP16,
+ * _EXTRAPOLATE and _PROJECT honoured; FP64 stencils; no learned stage).
  *
  * Members are contexts created by wpf_create on one device, each nx a
  * multiple of 4; they stay owned by the caller and must outlive the

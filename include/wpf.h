@@ -270,6 +270,13 @@ int wpf_attach_net(wpf_ctx* ctx, wpf_net* net);
 /* After a solve with WPF_FLAG_CAPTURE: the network input of its start
  * state (planar [3][n]) and the correction the solver found (n values). */
 int wpf_capture_get(wpf_ctx* ctx, float* fields, float* target);
+/* After a CG + Jacobi solve with WPF_FLAG_LEARNED_FCG | WPF_FLAG_CAPTURE:
+ * the network input at every second learned-stage point of flexible CG (up
+ * to 16) and the correction from the iterate there to the final solution,
+ * the training pairs of the stage; count, then pair j (fields [3][n],
+ * target n). */
+int wpf_capture_fcg_count(wpf_ctx* ctx, int32_t* count);
+int wpf_capture_fcg_get(wpf_ctx* ctx, int32_t j, float* fields, float* target);
 
 /* ---- measurement ---------------------------------------------------------
  * Time every launch of one kernel with HIP events on the context's stream

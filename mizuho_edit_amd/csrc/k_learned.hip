@@ -726,7 +726,7 @@ int solver_input(wpf_ctx* c, const double* r64, double rnorm, float* fields)
 }
 
 template <typename R>
-int net_correction(wpf_ctx* c, const double* r64, double rnorm, R* out)
+int net_correction(wpf_ctx* c, const double* r64, double rnorm, R* out, float* fields_copy)
 {
     wpf_net* n = c->net;
     WPF_CHECK(n, WPF_ERR_STATE, "no network attached");
@@ -743,6 +743,9 @@ int net_correction(wpf_ctx* c, const double* r64, double rnorm, R* out)
                        rms_inv, 1.0e5, (const double*)c->phi[c->cur], (const double*)c->krock,
                        (const double*)c->p.p, r64, (const double*)c->invD.p, n->fields);
     WPF_HIP(hipGetLastError());
+    if (fields_copy)
+        WPF_HIP(hipMemcpyAsync(fields_copy, n->fields, sizeof(float) * 3 * (size_t)g.n, hipMemcpyDeviceToDevice,
+                               c->stream));
     rc = forward(n, n->fields);
     if (!rc) {
         hipLaunchKernelGGL(k_upsample_out<R>, dim3(nblk(g.n)), dim3(256), 0, c->stream, n->lv[0], n->lv[1],
@@ -779,8 +782,8 @@ int net_correction32(wpf_ctx* c, const float* r32, const float* invD32, double r
     return rc;
 }
 
-template int net_correction<double>(wpf_ctx*, const double*, double, double*);
-template int net_correction<float>(wpf_ctx*, const double*, double, float*);
+template int net_correction<double>(wpf_ctx*, const double*, double, double*, float*);
+template int net_correction<float>(wpf_ctx*, const double*, double, float*, float*);
 
 void free_learned(wpf_ctx* c) { c->net = nullptr; }  // the network is owned by its creator
 

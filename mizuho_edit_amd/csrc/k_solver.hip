@@ -612,7 +612,7 @@ static __global__ void __launch_bounds__(kThreads) k_fcg_stage(int64_t n, const 
 static constexpr int kFcgPeriod = 8, kFcgWindow = 32;
 
 static int fcg_learned(wpf_ctx* c, const Fine<double, double>& F, double* x, double rn, double tol, int maxit,
-                       int* its_out, double* rn_out, int* stages)
+                       int* its_out, double* rn_out, int* stages, bool capture)
 {
     const int64_t n = F.g.n;
     const int nb = blocks_for(n);
@@ -624,7 +624,19 @@ static int fcg_learned(wpf_ctx* c, const Fine<double, double>& F, double* x, dou
     int rz = S_FCG, bnum = S_FCG + 1;
     auto stage = [&](double rnow, const double* qold) -> int {
         const int kid_ = WPF_KERNEL_CG_UPDATE;
-        int rc = net_correction<double>(c, F.r, rnow, y);
+        // capture: every second stage point, the network input and the iterate
+        float* keep = nullptr;
+        if (capture && *stages % 2 == 0 && c->fcap_n < wpf_ctx::kFcapMax) {
+            const int j = c->fcap_n++;
+            if (!c->fcap_fields[j]) {
+                WPF_HIP(hipMalloc((void**)&c->fcap_fields[j], sizeof(float) * 3 * (size_t)n));
+                WPF_HIP(hipMalloc((void**)&c->fcap_x[j], sizeof(double) * (size_t)n));
+                WPF_HIP(hipMalloc((void**)&c->fcap_target[j], sizeof(float) * (size_t)n));
+            }
+            keep = c->fcap_fields[j];
+            WPF_HIP(hipMemcpyAsync(c->fcap_x[j], x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+        }
+        int rc = net_correction<double>(c, F.r, rnow, y, keep);
         if (rc) return rc;
         Fine<double, double> Fy = F;
         Fy.q = ay;
@@ -1009,6 +1021,8 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
     // Jacobi CG with WPF_FLAG_LEARNED_FCG runs the learned stage inside
     // flexible CG (fcg_learned); otherwise the flag asks for the first stage
     const bool fcg = !mg && (o.flags & WPF_FLAG_LEARNED_FCG) && c->net;
+    const bool capture = fcg && (o.flags & WPF_FLAG_CAPTURE);
+    if (capture) c->fcap_n = 0;
     int learned = 0;
     if ((rc = start_stage(c, o, &learned, !fcg || (o.flags & WPF_FLAG_LEARNED_WARM)))) return rc;
     const double tol = std::max(o.rtol * bnorm, o.atol);
@@ -1048,7 +1062,7 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
         // use the network
         if (fcg && restarts == 0) {
             int stages = 0;
-            rc = fcg_learned(c, F, x, rn, tol, o.maxit - its, &k, &rr, &stages);
+            rc = fcg_learned(c, F, x, rn, tol, o.maxit - its, &k, &rr, &stages, capture);
             learned += stages;
         } else {
             rc = pcg_core<double, double, double>(c, mg ? &c->mg64 : nullptr, F, mo, x, tol, o.maxit - its,
@@ -1058,6 +1072,10 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
         its += k;
         restarts++;
     }
+    // the captured iterates' corrections to the final solution
+    for (int j = 0; capture && j < c->fcap_n; j++)
+        hipLaunchKernelGGL(k_cap_target, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                           (const double*)x, (const double*)c->fcap_x[j], c->fcap_target[j]);
     st->iterations = its;
     st->outer = 0;
     st->rnorm0 = rn0;

@@ -299,6 +299,9 @@ int wpf_destroy(wpf_ctx* c)
     F(c->krock); F(c->dphidt); F(c->dcdt); F(c->vx); F(c->vy); F(c->vz);
     F(c->scal); F(c->tau_key); F(c->bt_step);
     F(c->cap_fields); F(c->cap_target); F(c->cap_guess);
+    for (int j = 0; j < wpf_ctx::kFcapMax; j++) {
+        F(c->fcap_fields[j]); F(c->fcap_x[j]); F(c->fcap_target[j]);
+    }
     dev_free(c->p); dev_free(c->tx); dev_free(c->ty); dev_free(c->tz); dev_free(c->invD);
     dev_free(c->r); dev_free(c->z); dev_free(c->pp); dev_free(c->q); dev_free(c->xprev);
     reducer_free(c->red);
@@ -671,6 +674,25 @@ int wpf_capture_get(wpf_ctx* c, float* fields, float* target)
     const size_t n = (size_t)c->g.n;
     WPF_HIP(hipMemcpyAsync(fields, c->cap_fields, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
     WPF_HIP(hipMemcpyAsync(target, c->cap_target, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipStreamSynchronize(c->stream));
+    return WPF_OK;
+}
+
+int wpf_capture_fcg_count(wpf_ctx* c, int32_t* count)
+{
+    WPF_CHECK(c && count, WPF_ERR_ARG, "null argument");
+    *count = c->fcap_n;
+    return WPF_OK;
+}
+
+int wpf_capture_fcg_get(wpf_ctx* c, int32_t j, float* fields, float* target)
+{
+    WPF_CHECK(c && fields && target, WPF_ERR_ARG, "null argument");
+    WPF_CHECK(j >= 0 && j < c->fcap_n, WPF_ERR_ARG, "capture %d of %d", j, c->fcap_n);
+    WPF_HIP(hipSetDevice(c->dev));
+    const size_t n = (size_t)c->g.n;
+    WPF_HIP(hipMemcpyAsync(fields, c->fcap_fields[j], sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipMemcpyAsync(target, c->fcap_target[j], sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     WPF_HIP(hipStreamSynchronize(c->stream));
     return WPF_OK;
 }

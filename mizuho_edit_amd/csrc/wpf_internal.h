@@ -166,6 +166,14 @@ struct wpf_ctx {
     float* cap_target = nullptr;              // [n]
     double* cap_guess = nullptr;              // [n]
     bool have_capture = false;
+    // captures of the learned stage inside flexible CG: the network input
+    // at up to kFcapMax stage points of the last such solve, the iterate
+    // there, and the correction to the final solution
+    static constexpr int kFcapMax = 16;
+    float* fcap_fields[kFcapMax] = {};        // [3][n] each
+    double* fcap_x[kFcapMax] = {};            // [n]
+    float* fcap_target[kFcapMax] = {};        // [n]
+    int fcap_n = 0;
 
     // device scalars for the fused step: [0] time [1] dtime [2] taumin_prev
     // [3] taumin_cur, plus an unsigned-key min slot and the breakthrough step
@@ -257,7 +265,7 @@ int solve_pressure_continue(wpf_ctx* c, const wpf_solver_opts* o, wpf_solve_stat
 // learned part (k_learned.hip)
 void free_learned(wpf_ctx* c);
 template <typename R>
-int net_correction(wpf_ctx* c, const double* r64, double rnorm, R* out);
+int net_correction(wpf_ctx* c, const double* r64, double rnorm, R* out, float* fields_copy = nullptr);
 // the same correction from the FP32 residual of the mixed solver
 int net_correction32(wpf_ctx* c, const float* r32, const float* invD32, double rnorm, float* out);
 // network input fields ([3][n] FP32) of the current solver state

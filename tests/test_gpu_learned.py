@@ -2,6 +2,8 @@
 hand-written BF16/MFMA network against the FP64 numpy oracle
 (oracle/pcnet.py), training on solver-generated data, and what the trained
 network gains inside the solve over the same solve without it."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -133,4 +135,38 @@ def test_learned_stage_inside_flexible_cg_gain(gpu):
     assert np.abs(p1 - p0).max() <= 1e-7 * np.abs(p0).max()
     assert stages >= 7 * 10
     assert sum(its1[5:]) < 0.97 * sum(its0[5:])
+    net.close()
+
+
+def test_flexible_cg_stage_point_capture(gpu):
+    """The training pairs of the stage inside flexible CG: at most 16 per
+    solve, the residual channel normalised to unit rms, and the first pair
+    (the solve's start) equal to the start-of-solve capture."""
+    net = learned.PCNet()
+    net.load()
+    case = io.workload("plain-small", seed=301, grid=(64, 64, 64), corr_len=4.0)
+    g = WPFAW3D(case)
+    g.attach_net(net)
+    g.run(3, solver_opts("cg", rtol=1e-10))
+    rec = g.step(solver_opts("cg", rtol=1e-10, flags=_lib.FLAG_LEARNED_FCG | _lib.FLAG_CAPTURE))
+    assert rec["converged"] == 1 and rec["learned"] >= 2
+    L = _lib.lib()
+    cnt = ctypes.c_int32()
+    assert L.wpf_capture_fcg_count(g._ctx, ctypes.byref(cnt)) == 0
+    assert 1 <= cnt.value <= 16 and cnt.value == min(16, (rec["learned"] + 1) // 2)
+    n = case.params.ncell
+    f0, t0 = np.zeros(3 * n, np.float32), np.zeros(n, np.float32)
+    fs, ts = np.zeros(3 * n, np.float32), np.zeros(n, np.float32)
+    assert L.wpf_capture_fcg_get(g._ctx, 0, f0.ctypes.data_as(_lib.F), t0.ctypes.data_as(_lib.F)) == 0
+    assert L.wpf_capture_get(g._ctx, fs.ctypes.data_as(_lib.F), ts.ctypes.data_as(_lib.F)) == 0
+    assert np.array_equal(t0, ts)
+    assert np.allclose(f0, fs, rtol=1e-6, atol=1e-6)
+    for j in range(cnt.value):
+        f, t = np.zeros(3 * n, np.float32), np.zeros(n, np.float32)
+        assert L.wpf_capture_fcg_get(g._ctx, j, f.ctypes.data_as(_lib.F), t.ctypes.data_as(_lib.F)) == 0
+        assert abs(np.sqrt(np.mean(f[2 * n:].astype(np.float64) ** 2)) - 1.0) < 1e-3
+        assert np.isfinite(t).all() and t.any()
+    assert L.wpf_capture_fcg_get(g._ctx, cnt.value, f0.ctypes.data_as(_lib.F),
+                                 t0.ctypes.data_as(_lib.F)) == _lib.WPF_ERR_ARG
+    g.close()
     net.close()

@@ -150,11 +150,73 @@ def capture_samples(case, nsteps: int, every: int, opts_kw=None, start: int = 3)
     return out
 
 
-def capture_fcg_samples(case, net: "PCNet", nsteps: int, start: int = 3):
+def _upsample_1d(nf: int):
+    """The network's trilinear N/4 -> N upsampling along one axis (the
+    cell-centred weights of k_upsample_out), as a sparse matrix."""
+    import scipy.sparse as sp
+    nc = nf // 4
+    cc = np.clip((np.arange(nf) + 0.5) / 4 - 0.5, 0, nc - 1)
+    i0 = np.minimum(cc.astype(int), nc - 1)
+    i1 = np.minimum(i0 + 1, nc - 1)
+    f = cc - i0
+    rows = np.concatenate([np.arange(nf), np.arange(nf)])
+    return sp.csr_matrix((np.concatenate([1 - f, f]), (rows, np.concatenate([i0, i1]))), shape=(nf, nc))
+
+
+class CoarseSolve:
+    """The best correction the network's output space holds: for a residual
+    r, y = P (P^T A P)^-1 P^T r, the A-orthogonal projection of the error
+    onto the trilinear upsamplings of N/4 fields.  A is the SPD operator of
+    the solve (free cells, from the device's face coefficients), P the
+    network's upsampling with Dirichlet rows zero."""
+
+    def __init__(self, case, faces):
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        p = case.params
+        nx, ny, nz = p.nx, p.ny, p.nz
+        n = nx * ny * nz
+        lw = np.asarray(case.lw)[1:-1, 1:-1, 1:-1].reshape(-1, order="F")
+        press = np.array([0.0] + [w.pressure for w in case.wells])
+        fixed = (lw > 0) & (press[np.maximum(lw, 0)] != 0.0)
+        t = [np.asarray(faces[d])[1:-1, 1:-1, 1:-1].reshape(-1, order="F") for d in range(3)]
+        idx = np.arange(n)
+        rows, cols, vals = [], [], []
+        diag = np.zeros(n)
+        for d, step in enumerate((1, nx, nx * ny)):
+            a, b = idx[:-step] if step < n else idx[:0], idx[step:]
+            tt = t[d][a]
+            keep = tt != 0.0
+            a, b, tt = a[keep], b[keep], tt[keep]
+            diag[a] += tt
+            diag[b] += tt
+            free = ~fixed[a] & ~fixed[b]
+            rows += [a[free], b[free]]
+            cols += [b[free], a[free]]
+            vals += [-tt[free], -tt[free]]
+        diag[fixed] = 1.0
+        A = sp.csr_matrix((np.concatenate(vals + [diag]), (np.concatenate(rows + [idx]), np.concatenate(cols + [idx]))),
+                          shape=(n, n))
+        P = sp.kron(sp.kron(_upsample_1d(nz), _upsample_1d(ny)), _upsample_1d(nx)).tocsr()
+        self.P = sp.diags((~fixed).astype(float)) @ P
+        self.D = diag
+        self.lu = spla.splu((self.P.T @ A @ self.P).tocsc())
+        self.shape = (nx, ny, nz)
+
+    def target(self, fields):
+        """The coarse-solve correction (full resolution) for the residual of
+        a network input (its channel 2 is D^-1 r up to a scale)."""
+        r = np.asarray(fields[2], np.float64).reshape(-1, order="F") * self.D
+        y = self.P @ self.lu.solve(self.P.T @ r)
+        return y.reshape(self.shape, order="F").astype(np.float32)
+
+
+def capture_fcg_samples(case, net: "PCNet", nsteps: int, start: int = 3, coarse: bool = False):
     """Run `case` with CG + Jacobi and the learned stage inside flexible CG
     (`net`), keeping from step `start` on the pairs of every solve's stage
     points: the network input there and the correction from that iterate to
-    the solve's final solution."""
+    the solve's final solution (coarse=True: the CoarseSolve correction of
+    that residual instead)."""
     from .wpfaw3d import WPFAW3D, solver_opts
     g = WPFAW3D(case)
     g.attach_net(net)
@@ -163,32 +225,40 @@ def capture_fcg_samples(case, net: "PCNet", nsteps: int, start: int = 3):
     out = []
     for s in range(1, nsteps + 1):
         cap = s >= start
+        faces = None
+        if cap and coarse:
+            # the operator of this step's solve: the permeability pass of
+            # the current fields, which the step repeats
+            g.calcPermeability()
+            faces = g.download("face_t")
         g.step(solver_opts("cg", rtol=1e-10,
                            flags=_lib.FLAG_LEARNED_FCG | (_lib.FLAG_CAPTURE if cap else 0)))
         if not cap:
             continue
         cnt = ctypes.c_int32()
         check(_lib.lib().wpf_capture_fcg_count(g._ctx, ctypes.byref(cnt)), "capture_fcg_count")
+        cs = CoarseSolve(case, faces) if coarse and cnt.value else None
         for j in range(cnt.value):
             f = np.zeros(3 * n, np.float32)
             t = np.zeros(n, np.float32)
             check(_lib.lib().wpf_capture_fcg_get(g._ctx, j, _fp(f), _fp(t)), "capture_fcg_get")
             if t.any():
-                out.append((f.reshape((3, p.nx, p.ny, p.nz), order="F"),
-                            t.reshape((p.nx, p.ny, p.nz), order="F")))
+                f = f.reshape((3, p.nx, p.ny, p.nz), order="F")
+                out.append((f, cs.target(f) if cs else t.reshape((p.nx, p.ny, p.nz), order="F")))
     g.close()
     return out
 
 
 def make_fcg_training_set(net: "PCNet", nreal: int = 8, grid=(64, 64, 64), nsteps: int = 12,
-                          seed0: int = 100):
+                          seed0: int = 100, coarse: bool = False):
     """Stage-point pairs from seeded realisations (correlation lengths
     2..8 cells), generated with `net` as the stage."""
     samples = []
     for i in range(nreal):
         corr = 2.0 + 6.0 * (i / max(1, nreal - 1))
         case = io.workload("plain-small", seed=seed0 + i, grid=grid, corr_len=corr)
-        samples.extend(capture_fcg_samples(case, net, nsteps))
+        samples.extend(capture_fcg_samples(case, net, nsteps, coarse=coarse))
+        print(f"  realisation {i + 1}/{nreal}: {len(samples)} pairs", flush=True)
     return samples
 
 
@@ -253,6 +323,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-extrapolate", action="store_true",
                     help="eval: start each solve from the previous pressure, not the extrapolated one")
+    ap.add_argument("--coarse", action="store_true",
+                    help="train-fcg: the coarse-solve correction of each residual as the target")
     ap.add_argument("--fcg", action="store_true",
                     help="eval: the stage inside flexible CG with the Jacobi preconditioner "
                          "(against plain CG + Jacobi) instead of the first stage of the mixed solver")
@@ -263,11 +335,14 @@ def main(argv=None):
         # stage, then a network trained on them from scratch
         gen = PCNet()
         gen.load(WEIGHTS)
-        samples = make_fcg_training_set(gen, a.nreal, (n, n, n), a.nsteps)
+        samples = make_fcg_training_set(gen, a.nreal, (n, n, n), a.nsteps, coarse=a.coarse)
         gen.close()
         print(f"{len(samples)} stage-point samples at {n}^3", flush=True)
         net = PCNet(seed=a.seed)
-        losses = net.train(samples, epochs=a.epochs, lr=a.lr)
+        losses = []
+        for e in range(a.epochs):  # one epoch per call (the optimiser state stays in the network)
+            losses += net.train(samples, epochs=1, lr=a.lr).tolist()
+            print(f"  epoch {e + 1}/{a.epochs}: loss {losses[-1]:.4f}", flush=True)
         print("loss per epoch:", np.round(losses, 4).tolist(), flush=True)
         out = a.weights if a.weights != WEIGHTS else WEIGHTS_FCG
         net.save(out)

@@ -443,8 +443,18 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>
         MGLevel<R>& C1 = lv[1];
         {
             const int kid_ = WPF_KERNEL_RESTRICT;
-            LAUNCHZ((k_fresid_restrict<R, C>), F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask,
-                    (const R*)F.r, (const R*)cur, C1.g.nx, C1.g.ny, C1.b.p);
+            constexpr int V = vwidth<R>();
+            if (F.vec && vfresid_fits<V>(F.V)) {
+                prof_mark(c, kid_, true);
+                hipLaunchKernelGGL((k_vfresid_restrict<R, C, V>), dim3(vz_blocks(F.V)), dim3(256), 0, c->stream, F.V,
+                                   F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (const R*)F.r, (const R*)cur,
+                                   C1.g.nx, C1.g.ny, C1.b.p);
+                prof_mark(c, kid_, false);
+                WPF_HIP(hipGetLastError());
+            } else {
+                LAUNCHZ((k_fresid_restrict<R, C>), F.G, F.G, F.tx, F.ty, F.tz, (const uint8_t*)c->mask,
+                        (const R*)F.r, (const R*)cur, C1.g.nx, C1.g.ny, C1.b.p);
+            }
         }
         if ((rc = vcycle_coarse(c, lv, 1, o))) return rc;
         pro = Pro<R>{C1.x.p, C1.g.nx, C1.g.plane, (R)o.cscale};

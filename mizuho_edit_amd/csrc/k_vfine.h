@@ -963,4 +963,103 @@ __global__ void __launch_bounds__(kThreads) k_coarsen_pairs(LV<R> L, const CF* _
                            (int)gridDim.x);
 }
 
+// Residual of the pre-smoothed iterate restricted 2x2x2 to the next level
+// (t_fresid_restrict) in vector form: each thread owns V cells of a row
+// (V/2 aggregates in x), the chunk of 256*V cells covers whole pairs of
+// rows (256*V % (2 nx) == 0), and the odd row's sums reach the even row's
+// threads through LDS.  Per aggregate the sums run in the order of
+// t_fresid_restrict (plane k then k+1 per cell, then the x pair, then the
+// y pair), so the coarse right-hand side is the same.
+template <int V>
+inline bool vfresid_fits(const VZ& Z)
+{
+    return Z.nx % (2 * V) == 0 && (256 * V) % (2 * Z.nx) == 0 && Z.zc % 2 == 0;
+}
+template <typename R, typename C, int V>
+__device__ __forceinline__ void t_vfresid_restrict(const VZ& Z, int bi, const C* __restrict__ tx,
+                                                   const C* __restrict__ ty, const C* __restrict__ tz,
+                                                   const uint8_t* __restrict__ mask, const R* __restrict__ b,
+                                                   const R* __restrict__ x, int cnx, int cny, R* __restrict__ bc)
+{
+    __shared__ R sh[256][V / 2];
+    int64_t q;
+    int k0, k1;
+    const bool act = vz_at<V>(Z, bi, q, k0, k1);
+    const int lane = threadIdx.x & 63;
+    const int64_t nx = Z.nx, pl = Z.plane;
+    const int i = (int)(q % nx), j = (int)(q / nx);
+    const int64_t cplane = (int64_t)cnx * cny;
+    const int partner = (int)threadIdx.x + (int)(nx / V);  // same cells one row up
+    R xm[V], x0[V], tzm[V];
+    int64_t c = q + pl * k0;
+    vld<V>(x + c - pl, xm);
+    vld<V>(x + c, x0);
+    vldc<V>(tz + c - pl, tzm);
+    for (int k = k0; k < k1; k += 2) {
+        R v[V];
+#pragma unroll
+        for (int h = 0; h < V; h++) v[h] = (R)0;
+        for (int dz = 0; dz < 2; dz++) {
+            if (k + dz >= k1) break;
+            R xp[V], xyp[V], xym[V], txv[V], tyv[V], tym[V], tzv[V], bv[V];
+            uint8_t m[V];
+            vld<V>(x + c + pl, xp);
+            vld<V>(x + c + nx, xyp);
+            vld<V>(x + c - nx, xym);
+            vldc<V>(tx + c, txv);
+            vldc<V>(ty + c, tyv);
+            vldc<V>(ty + c - nx, tym);
+            vldc<V>(tz + c, tzv);
+            vld<V>(b + c, bv);
+            vld<V>(mask + c, m);
+            R xl = from_left(x0[V - 1]), xr = from_right(x0[0]), txl = from_left(txv[V - 1]);
+            if (lane == 0) {
+                xl = x[c - 1];
+                txl = (R)tx[c - 1];
+            }
+            if (lane == 63) xr = x[c + V];
+#pragma unroll
+            for (int h = 0; h < V; h++) {
+                const R txp = txv[h], txm = h ? txv[h - 1] : txl;
+                const R d = txp + txm + tyv[h] + tym[h] + tzv[h] + tzm[h];
+                const R s = txp * ((h + 1 < V) ? x0[h + 1] : xr) + txm * (h ? x0[h - 1] : xl) +
+                            tyv[h] * xyp[h] + tym[h] * xym[h] + tzv[h] * xp[h] + tzm[h] * xm[h];
+                if (!(m[h] & WPF_MASK_FIXED)) v[h] += bv[h] - (d * x0[h] - s);
+            }
+#pragma unroll
+            for (int h = 0; h < V; h++) {
+                xm[h] = x0[h];
+                x0[h] = xp[h];
+                tzm[h] = tzv[h];
+            }
+            c += pl;
+        }
+        // threads past the plane (their loads were clamped into it) add 0,
+        // as the rows past the grid do in t_fresid_restrict
+        R pv[V / 2];
+#pragma unroll
+        for (int h = 0; h < V / 2; h++) {
+            pv[h] = act ? v[2 * h] + v[2 * h + 1] : (R)0;
+            sh[threadIdx.x][h] = pv[h];
+        }
+        __syncthreads();
+        if (act && !(j & 1)) {
+            R o[V / 2];
+#pragma unroll
+            for (int h = 0; h < V / 2; h++) o[h] = pv[h] + sh[partner][h];
+            const int64_t Ci = (i >> 1) + (int64_t)cnx * (j >> 1) + cplane * (k >> 1);
+            vst<V / 2>(bc + Ci, o);
+        }
+        __syncthreads();
+    }
+}
+template <typename R, typename C, int V>
+__global__ void __launch_bounds__(256) k_vfresid_restrict(VZ Z, const C* __restrict__ tx, const C* __restrict__ ty,
+                                                          const C* __restrict__ tz, const uint8_t* __restrict__ mask,
+                                                          const R* __restrict__ b, const R* __restrict__ x, int cnx,
+                                                          int cny, R* __restrict__ bc)
+{
+    t_vfresid_restrict<R, C, V>(Z, blockIdx.x, tx, ty, tz, mask, b, x, cnx, cny, bc);
+}
+
 }  // namespace wpf

@@ -227,10 +227,23 @@ int wpf_config_load(const char* cfg_path, wpf_params* p, int32_t* nwell,
 int wpf_config_wells(wpf_well* wells);
 int wpf_config_fields(int32_t* lw_haloed, double* krock_haloed,
                       double* phase_haloed, double* concent_haloed);
+/* The output counts of the last loaded .cfg: gid_output (n_res_out, results
+ * per time_total, 1:318-320) and scr_output (n_dat_out, 1:321-323); 0 when
+ * the key is absent. */
+int wpf_config_output(int32_t* n_res_out, int32_t* n_dat_out);
 /* saveField, wPFAW3D.f90:2096-2141 (haloed host arrays in). */
 int wpf_field_write(const char* path, double time, const wpf_params* p,
                     const double* krock_haloed, const double* phase_haloed,
                     const double* concent_haloed);
+/* saveResult, wPFAW3D.f90:1842-2091: the GiD post-processing binary
+ * (GiDPostEx1.1).  first != 0 creates the file with the hexahedral mesh
+ * (nodes every dx, one cell-centre Gauss point); every call then appends the
+ * results Pressure, Phase-field, Concentration and Permeability (the node
+ * value permeab(0,...), as FP32) at `time`.  Haloed host arrays in; the
+ * file is closed after each call, which makes doneResult a no-op. */
+int wpf_result_write(const char* path, double time, const wpf_params* p, int32_t first,
+                     const double* pressure_haloed, const double* phase_haloed,
+                     const double* concent_haloed, const double* permeab0_haloed);
 
 /* ---- learned pressure correction ------------------------------------------
  * No counterpart in the reference (its solve is PETSc's).  A 3-D conv

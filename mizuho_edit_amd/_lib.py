@@ -135,6 +135,8 @@ def lib():
         "wpf_net_train": ([P, I32, I32, I32, I32, F, F, I32, ctypes.c_float, D], I32),
         "wpf_attach_net": ([P, P], I32),
         "wpf_capture_get": ([P, F, F], I32),
+        "wpf_config_output": ([ctypes.POINTER(I32), ctypes.POINTER(I32)], I32),
+        "wpf_result_write": ([ctypes.c_char_p, ctypes.c_double, ctypes.POINTER(WpfParams), I32, D, D, D, D], I32),
         "wpf_capture_fcg_count": ([P, ctypes.POINTER(I32)], I32),
         "wpf_capture_fcg_get": ([P, I32, F, F], I32),
         "wpf_set_tile_depth": ([P, I32], I32),

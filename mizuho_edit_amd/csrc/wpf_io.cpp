@@ -26,6 +26,7 @@ namespace {
 struct Staged {
     wpf_params p{};
     double time_start = 0.0, time_total = 0.0;
+    int32_t n_res_out = 0, n_dat_out = 0;  // gid_output, scr_output
     std::vector<wpf_well> wells;
     std::string init_file;
     double init_k = 0.0, init_phase = -1.0, init_conc = 0.0;
@@ -101,13 +102,12 @@ int wpf_config_load(const char* path, wpf_params* out, int32_t* nwell, double* t
         auto rd = [&](double& v) { ok = ok && static_cast<bool>(is >> v); };
         auto ri = [&](int32_t& v) { ok = ok && static_cast<bool>(is >> v); };
         double dummy;
-        int32_t idummy;
         if (key == "grid") { ri(S.p.nx); ri(S.p.ny); ri(S.p.nz); }
         else if (key == "dt") rd(S.p.dtime);
         else if (key == "tfac") rd(S.p.tfac);
         else if (key == "time_total") rd(S.time_total);
-        else if (key == "gid_output") ri(idummy);
-        else if (key == "scr_output") ri(idummy);
+        else if (key == "gid_output") ri(S.n_res_out);
+        else if (key == "scr_output") ri(S.n_dat_out);
         else if (key == "w") rd(S.p.epsilon);
         else if (key == "dxw") rd(dxw);
         else if (key == "lambda") rd(S.p.lambda);
@@ -348,6 +348,147 @@ int wpf_field_write(const char* path, double time, const wpf_params* p, const do
             }
     fclose(fp);
     if (!ok) {
+        wpf::set_error("write failed: %s", path);
+        return WPF_ERR_IO;
+    }
+    return WPF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GiD post-processing result file (saveResult, wPFAW3D.f90:1842-2091)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct GidOut {
+    FILE* fp;
+    bool ok = true;
+    void raw(const void* d, size_t n) { ok = ok && fwrite(d, 1, n, fp) == n; }
+    void i32(int32_t v) { raw(&v, 4); }
+    // write_string (1:2045-2051): length + 1, then the text and a NUL
+    void str(const std::string& t)
+    {
+        i32((int32_t)t.size() + 1);
+        raw(t.c_str(), t.size() + 1);
+    }
+};
+
+// Fortran's ES12.5 edit of the time: d.ddddd, then E+xx, or +xxx (no E) for
+// a three-digit exponent, right-justified in 12 characters
+std::string es12_5(double v)
+{
+    char b[64];
+    snprintf(b, sizeof b, "%.5E", v);
+    std::string t(b);
+    const size_t e = t.find('E');
+    if (e != std::string::npos && t.size() - e - 2 > 2) t.erase(e, 1);
+    if (t.size() < 12) t.insert(0, 12 - t.size(), ' ');
+    return t;
+}
+
+}  // namespace
+
+int wpf_config_output(int32_t* n_res_out, int32_t* n_dat_out)
+{
+    if (!g_cfg.loaded) {
+        wpf::set_error("wpf_config_output before wpf_config_load");
+        return WPF_ERR_STATE;
+    }
+    if (n_res_out) *n_res_out = g_cfg.n_res_out;
+    if (n_dat_out) *n_dat_out = g_cfg.n_dat_out;
+    return WPF_OK;
+}
+
+int wpf_result_write(const char* path, double time, const wpf_params* p, int32_t first, const double* pressure,
+                     const double* phase, const double* concent, const double* permeab0)
+{
+    if (!path || !p || !pressure || !phase || !concent || !permeab0) {
+        wpf::set_error("wpf_result_write: null argument");
+        return WPF_ERR_ARG;
+    }
+    FILE* fp = fopen(path, first ? "wb" : "ab");
+    if (!fp) {
+        wpf::set_error("can not create result file: %s", path);
+        return WPF_ERR_IO;
+    }
+    GidOut o{fp};
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    // serial indices of mesh points (with the extra point per axis) and of
+    // elements, 1-based, x fastest (mindex3D, eindex3D: 1:2071-2089)
+    auto mindex = [&](int i, int j, int k) {
+        return (int32_t)((i - 1) + (int64_t)(j - 1) * (nx + 1) + (int64_t)(k - 1) * (nx + 1) * (ny + 1) + 1);
+    };
+    auto eindex = [&](int i, int j, int k) {
+        return (int32_t)((i - 1) + (int64_t)(j - 1) * nx + (int64_t)(k - 1) * nx * ny + 1);
+    };
+    if (first) {
+        o.i32(0x091d);  // ByteOrderCheck
+        o.str("GiDPostEx1.1");
+        o.str("MESH \"quadmesh\" dimension 3 ElemType Hexahedra Nnode 8");
+        o.str("# color 0.7 0.7 0.4");
+        o.str("Unit \"m\"");
+        o.str("Coordinates -1 Indexed");
+        struct Coord {
+            int32_t index;
+            float x, y, z;
+        };
+        std::vector<Coord> cr(nx + 1);
+        for (int k = 1; k <= nz + 1; k++)
+            for (int j = 1; j <= ny + 1; j++) {
+                for (int i = 1; i <= nx + 1; i++)
+                    cr[i - 1] = Coord{mindex(i, j, k), (float)((i - 1) * p->dx), (float)((j - 1) * p->dx),
+                                      (float)((k - 1) * p->dx)};
+                o.raw(cr.data(), sizeof(Coord) * cr.size());
+            }
+        o.str("End Coordinates");
+        o.str("Elements -1 Indexed");
+        struct Elem {
+            int32_t index, nid[8], conn;
+        };
+        std::vector<Elem> er(nx);
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++) {
+                for (int i = 1; i <= nx; i++)
+                    er[i - 1] = Elem{eindex(i, j, k),
+                                     {mindex(i, j, k), mindex(i + 1, j, k), mindex(i + 1, j + 1, k),
+                                      mindex(i, j + 1, k), mindex(i, j, k + 1), mindex(i + 1, j, k + 1),
+                                      mindex(i + 1, j + 1, k + 1), mindex(i, j + 1, k + 1)},
+                                     1};
+                o.raw(er.data(), sizeof(Elem) * er.size());
+            }
+        o.str("End Elements");
+        o.str("GaussPoints \"Cell_Center\" ElemType Hexahedra");
+        o.str("Number Of Gauss Points: 1");
+        o.str("Natural Coordinates: Given");
+        const float zeros[3] = {0.0f, 0.0f, 0.0f};
+        o.raw(zeros, sizeof zeros);
+        o.str("End GaussPoints");
+    }
+    // the four results of each call: haloed Fortran-order host arrays
+    struct Value {
+        int32_t index;
+        float value;
+    };
+    const char* names[4] = {"Pressure", "Phase-field", "Concentration", "Permeability"};
+    const char* units[4] = {"Pa", "-", "-", "-"};
+    const double* src[4] = {pressure, phase, concent, permeab0};
+    const int64_t sx = nx + 2, sy = ny + 2;
+    std::vector<Value> vr(nx);
+    for (int f = 0; f < 4; f++) {
+        o.str(std::string("Result \"") + names[f] + "\" \"Analysis\" " + es12_5(time) +
+              " Scalar OnGaussPoints \"Cell_Center\"");
+        o.str(std::string("Unit \"") + units[f] + "\"");
+        o.str("Values -1 Indexed");
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++) {
+                for (int i = 1; i <= nx; i++)
+                    vr[i - 1] = Value{eindex(i, j, k), (float)src[f][i + sx * (j + sy * k)]};
+                o.raw(vr.data(), sizeof(Value) * vr.size());
+            }
+        o.i32(-1);
+        o.str("End Values");
+    }
+    const bool ok = o.ok;
+    if (fclose(fp) != 0 || !ok) {
         wpf::set_error("write failed: %s", path);
         return WPF_ERR_IO;
     }

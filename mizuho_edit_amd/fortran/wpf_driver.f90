@@ -4,13 +4,15 @@
 !!
 !! Usage: wpf_driver <name>.cfg [nsteps]
 !! The loop ends at time_end as in the reference, or after nsteps steps when
-!! given.  At the end the state is written with saveField to <name>.last.fld
-!! and a progress line per step goes to stdout.
+!! given.  With gid_output = n in the .cfg, results go to <name>.res with
+!! saveResult at the start and every time_total/n (1:2217-2221, 2288-2292).
+!! At the end the state is written with saveField to <name>.last.fld and a
+!! progress line per step goes to stdout.
 program wpf_driver
    use wpfaw3d_gpu
    implicit none
    integer :: itime, nsteps, i, nargs
-   real(8) :: time, time_end, taumin
+   real(8) :: time, time_end, taumin, time_res, dtime_res
    character(len=512) :: filename, arg
    type(wpf_diag) :: d
 
@@ -37,6 +39,12 @@ program wpf_driver
    time = time_start
    time_end = time_start + time_total
    taumin = 0.0d0
+   dtime_res = 0.0d0
+   if (n_res_out > 0) then
+      dtime_res = time_total/n_res_out
+      time_res = time + dtime_res
+      call saveResult(filename(1:i-1)//".res", time)
+   end if
    do while (time < time_end)
       if (nsteps > 0 .and. itime >= nsteps) exit
       itime = itime + 1
@@ -57,7 +65,13 @@ program wpf_driver
 
       write(*,'(a,i8,a,es22.15,a,es22.15,a,es22.15,a,i6)') " ITIME", itime, " TIME", time, &
          " DT", gparam%dtime, " TAUMIN", taumin, " ITS", solve_stats%iterations
+
+      if (n_res_out > 0 .and. time > time_res) then
+         call saveResult(filename(1:i-1)//".res", time)
+         time_res = time_res + dtime_res
+      end if
    end do
+   if (n_res_out > 0) call doneResult()
 
    call showDiag(d)
    write(*,'(a,4es22.14)') " FINAL pv vcal rwhmax pinj", d%pv, d%vcal, d%rwhmax, d%pinj

@@ -44,7 +44,8 @@ module wpfaw3d_gpu
    end type wpf_diag
 
    integer(c_int32_t), parameter, public :: WPF_FIELD_PERMEAB_ROCK = 0, WPF_FIELD_PRESSURE = 1, &
-      WPF_FIELD_PHASE = 2, WPF_FIELD_CONCENT = 3, WPF_FIELD_DPHASEDT = 4, WPF_FIELD_DCONCENTDT = 5
+      WPF_FIELD_PHASE = 2, WPF_FIELD_CONCENT = 3, WPF_FIELD_DPHASEDT = 4, WPF_FIELD_DCONCENTDT = 5, &
+      WPF_FIELD_PERMEAB = 6
    integer(c_int32_t), parameter :: WPF_OK = 0, WPF_ERR_NOCONV = 3
 
    interface
@@ -70,6 +71,19 @@ module wpfaw3d_gpu
          real(c_double), value :: t
          type(wpf_params), intent(in) :: p
          real(c_double), intent(in) :: kr(*), ph(*), cc(*)
+      end function
+      integer(c_int32_t) function wpf_config_output(nres, ndat) bind(C, name="wpf_config_output")
+         import :: c_int32_t
+         integer(c_int32_t), intent(out) :: nres, ndat
+      end function
+      integer(c_int32_t) function wpf_result_write(path, t, p, first, pr, ph, cc, k0) &
+            bind(C, name="wpf_result_write")
+         import :: c_char, c_int32_t, c_double, wpf_params
+         character(kind=c_char), intent(in) :: path(*)
+         real(c_double), value :: t
+         type(wpf_params), intent(in) :: p
+         integer(c_int32_t), value :: first
+         real(c_double), intent(in) :: pr(*), ph(*), cc(*), k0(*)
       end function
       integer(c_int32_t) function wpf_create(p, lw, w, nwell, dev, ctx) bind(C, name="wpf_create")
          import :: c_int32_t, c_ptr, wpf_params, wpf_well
@@ -152,9 +166,11 @@ module wpfaw3d_gpu
    integer(c_int32_t), allocatable, public :: lw(:,:,:)
    real(c_double), allocatable, public :: permeab_rock(:,:,:), pressure(:,:,:), phase(:,:,:), concent(:,:,:)
    real(c_double), public :: time_start = 0.0d0, time_total = 0.0d0
+   ! gid_output and scr_output of the .cfg (0 when absent)
+   integer(c_int32_t), public :: n_res_out = 0, n_dat_out = 0
    type(c_ptr) :: ctx = c_null_ptr
 
-   public :: loadConfig, syncHost, saveField, freeField, showDiag, getPressure
+   public :: loadConfig, syncHost, saveField, saveResult, doneResult, freeField, showDiag, getPressure
    public :: calcPermeability, solvePressure, calcVelocity, calcPhaseDeriv, calcConcentDeriv
    public :: evolvePhase, evolveConcent, updateHalo
 
@@ -183,6 +199,7 @@ contains
       integer(c_int32_t) :: nwell
       call check(wpf_config_load(trim(filename)//c_null_char, gparam, nwell, time_start, time_total), &
                  "loadConfig")
+      call check(wpf_config_output(n_res_out, n_dat_out), "loadConfig(output)")
       allocate(well(max(1, nwell)))
       call check(wpf_config_wells(well), "loadConfig(wells)")
       allocate(lw(0:gparam%nx+1, 0:gparam%ny+1, 0:gparam%nz+1))
@@ -215,6 +232,28 @@ contains
       call check(wpf_field_write(trim(filename)//c_null_char, time, gparam, permeab_rock, phase, &
                                  concent), "saveField")
    end subroutine saveField
+
+   !> saveResult (wPFAW3D.f90:1842-2091) from the current device state: the
+   !> first call writes the GiD mesh, every call appends the four results
+   subroutine saveResult(filename, time)
+      character(len=*), intent(in) :: filename
+      real(c_double), intent(in) :: time
+      logical, save :: firsttime = .true.
+      real(c_double), allocatable :: perm(:,:,:,:), k0(:,:,:)
+      call syncHost()
+      allocate(perm(0:3, 0:gparam%nx+1, 0:gparam%ny+1, 0:gparam%nz+1))
+      call check(wpf_download(ctx, WPF_FIELD_PERMEAB, perm), "download(permeab)")
+      k0 = perm(0,:,:,:)
+      call check(wpf_result_write(trim(filename)//c_null_char, time, gparam, &
+                                  merge(1_c_int32_t, 0_c_int32_t, firsttime), pressure, phase, concent, k0), &
+                 "saveResult")
+      firsttime = .false.
+      deallocate(perm, k0)
+   end subroutine saveResult
+
+   !> doneResult: the result file is closed after every saveResult
+   subroutine doneResult()
+   end subroutine doneResult
 
    subroutine freeField()
       integer(c_int32_t) :: ierr

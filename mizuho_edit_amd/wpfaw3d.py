@@ -152,6 +152,18 @@ class WPFAW3D:
               "getPressure")
         return v.value
 
+    def saveResult(self, filename: str, time: Optional[float] = None):
+        """saveResult (wPFAW3D.f90:1842-2091): the first call writes the GiD
+        mesh, every call appends Pressure, Phase-field, Concentration and
+        Permeability (the node value) at `time` (default: the current time)."""
+        first = not getattr(self, "_result_written", False)
+        k0 = np.asfortranarray(self.permeab[0])
+        arrs = [np.asfortranarray(a) for a in (self.pressure, self.phase, self.concent)] + [k0]
+        check(_lib.lib().wpf_result_write(filename.encode(), self.time if time is None else time,
+                                          ctypes.byref(self._cp), 1 if first else 0,
+                                          *[a.ctypes.data_as(_lib.D) for a in arrs]), "saveResult")
+        self._result_written = True
+
     def showProgress(self) -> WpfDiag:
         d = WpfDiag()
         check(_lib.lib().wpf_diagnostics(self._ctx, ctypes.byref(d)), "showProgress")

@@ -22,6 +22,7 @@ def test_fortran_driver_matches_python_mirror(gpu, tmp_path):
     case = io.workload("plain-small", seed=5, grid=(32, 16, 16))
     p = case.params
     p.time_total = 1.0e30
+    p.n_res_out = 1  # saveResult at the start only (the next output is time_total later)
     io.write_wells(str(tmp_path / "c.well"), case.wells)
     io.write_field(str(tmp_path / "c.fld"), 0.0, case.krock[1:-1, 1:-1, 1:-1],
                    case.phase[1:-1, 1:-1, 1:-1], case.concent[1:-1, 1:-1, 1:-1])
@@ -34,6 +35,13 @@ def test_fortran_driver_matches_python_mirror(gpu, tmp_path):
     lines = [l for l in r.stdout.splitlines() if l.startswith(" ITIME")]
     assert len(lines) == nsteps
     t_drv, _, ph_drv, c_drv = io.read_field(str(tmp_path / "c.last.fld"), (p.nx, p.ny, p.nz))
+
+    # the driver's GiD result file (saveResult at the start) is the Python
+    # mirror's, byte for byte
+    g0 = WPFAW3D(case)
+    g0.saveResult(str(tmp_path / "py.res"), 0.0)
+    g0.close()
+    assert (tmp_path / "c.res").read_bytes() == (tmp_path / "py.res").read_bytes()
 
     g = WPFAW3D(case, opts=solver_opts("mgcg", rtol=1e-10, maxit=1000))
     for _ in range(nsteps):

@@ -206,3 +206,88 @@ def test_fortran_driver_reports_missing_gpu(tmp_path):
     r = subprocess.run([drv, cfg, "2"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "wpf_create" in r.stdout and "no GPU" in r.stdout
+
+
+def _gid_read(raw):
+    """Reader of the GiD post-processing binary as saveResult writes it
+    (wPFAW3D.f90:1842-2091): int32 strings (length incl. NUL), int32/float
+    records.  Returns the token list: ('s', text) or raw record blocks."""
+    pos = 0
+
+    def i32():
+        nonlocal pos
+        v = struct.unpack_from("<i", raw, pos)[0]
+        pos += 4
+        return v
+
+    def string():
+        nonlocal pos
+        n = i32()
+        t = raw[pos:pos + n]
+        pos += n
+        assert t.endswith(b"\0")
+        return t[:-1].decode()
+
+    return i32, string, lambda: pos
+
+
+def test_gid_result_file_layout(tmp_path):
+    """wpf_result_write (saveResult): the mesh block once, then the four
+    results of each call, in the layout of wPFAW3D.f90:1842-2091."""
+    nx, ny, nz = 3, 2, 2
+    p = io.physics_params(nx, ny, nz)
+    cp = p.to_c()
+    sh = (nx + 2, ny + 2, nz + 2)
+    rng = np.random.default_rng(3)
+    f = [np.asfortranarray(rng.normal(size=sh)) for _ in range(4)]
+    L = _lib.lib()
+    path = str(tmp_path / "r.res")
+    for first, t in ((1, 1.25e-3), (0, 1.5e100)):
+        assert L.wpf_result_write(path.encode(), t, ctypes.byref(cp), first,
+                                  *[a.ctypes.data_as(_lib.D) for a in f]) == 0
+    raw = open(path, "rb").read()
+    i32, string, at = _gid_read(raw)
+    assert i32() == 0x091d
+    assert [string() for _ in range(5)] == [
+        "GiDPostEx1.1", 'MESH "quadmesh" dimension 3 ElemType Hexahedra Nnode 8', "# color 0.7 0.7 0.4",
+        'Unit "m"', "Coordinates -1 Indexed"]
+    npt = (nx + 1) * (ny + 1) * (nz + 1)
+    pts = np.frombuffer(raw, dtype=[("i", "<i4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4")], count=npt,
+                        offset=at())
+    assert np.array_equal(pts["i"], np.arange(1, npt + 1))
+    k, j, i = np.unravel_index(np.arange(npt), (nz + 1, ny + 1, nx + 1))
+    assert np.array_equal(pts["x"], (i * p.dx).astype(np.float32))
+    assert np.array_equal(pts["z"], (k * p.dx).astype(np.float32))
+    i32, string, _ = _gid_read(raw[at() + 16 * npt:])
+    base = len(raw) - len(raw[at() + 16 * npt:])
+    assert string() == "End Coordinates" and string() == "Elements -1 Indexed"
+    off = base + (4 + len("End Coordinates") + 1) + (4 + len("Elements -1 Indexed") + 1)
+    ne = nx * ny * nz
+    el = np.frombuffer(raw, dtype="<i4", count=10 * ne, offset=off).reshape(ne, 10)
+    assert np.array_equal(el[:, 0], np.arange(1, ne + 1))
+    mi = lambda a, b, c: a + (nx + 1) * (b + (ny + 1) * c) + 1  # noqa: E731  (0-based a, b, c)
+    assert list(el[0, 1:9]) == [mi(0, 0, 0), mi(1, 0, 0), mi(1, 1, 0), mi(0, 1, 0),
+                                mi(0, 0, 1), mi(1, 0, 1), mi(1, 1, 1), mi(0, 1, 1)]
+    assert np.all(el[:, 9] == 1)
+    rest = raw[off + 40 * ne:]
+    i32, string, at = _gid_read(rest)
+    assert [string() for _ in range(4)] == ["End Elements", 'GaussPoints "Cell_Center" ElemType Hexahedra',
+                                            "Number Of Gauss Points: 1", "Natural Coordinates: Given"]
+    assert struct.unpack_from("<3f", rest, at()) == (0.0, 0.0, 0.0)
+    rest = rest[at() + 12:]
+    i32, string, at = _gid_read(rest)
+    assert string() == "End GaussPoints"
+    names = [("Pressure", "Pa"), ("Phase-field", "-"), ("Concentration", "-"), ("Permeability", "-")]
+    for t_str in (" 1.25000E-03", " 1.50000+100"):
+        for q, (name, unit) in enumerate(names):
+            assert string() == f'Result "{name}" "Analysis" {t_str} Scalar OnGaussPoints "Cell_Center"'
+            assert string() == f'Unit "{unit}"'
+            assert string() == "Values -1 Indexed"
+            vals = np.frombuffer(rest, dtype=[("i", "<i4"), ("v", "<f4")], count=ne, offset=at())
+            assert np.array_equal(vals["i"], np.arange(1, ne + 1))
+            interior = f[q][1:-1, 1:-1, 1:-1].reshape(-1, order="F").astype(np.float32)
+            assert np.array_equal(vals["v"], interior)
+            rest = rest[at() + 8 * ne:]
+            i32, string, at = _gid_read(rest)
+            assert i32() == -1 and string() == "End Values"
+    assert at() == len(rest)

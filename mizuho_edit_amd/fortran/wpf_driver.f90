@@ -12,7 +12,7 @@ program wpf_driver
    use wpfaw3d_gpu
    implicit none
    integer :: itime, nsteps, i, nargs
-   real(8) :: time, time_end, taumin, time_res, dtime_res
+   real(8) :: time, time_end, taumin, time_res, dtime_res, time_dat, dtime_dat
    character(len=512) :: filename, arg
    type(wpf_diag) :: d
 
@@ -45,6 +45,12 @@ program wpf_driver
       time_res = time + dtime_res
       call saveResult(filename(1:i-1)//".res", time)
    end if
+   dtime_dat = 0.0d0
+   if (n_dat_out > 0) then
+      dtime_dat = time_total/n_dat_out
+      time_dat = time + dtime_dat
+      call showProgress(filename(1:i-1)//".dat", time, itime, getPressure(1, 1, 1))
+   end if
    do while (time < time_end)
       if (nsteps > 0 .and. itime >= nsteps) exit
       itime = itime + 1
@@ -66,12 +72,18 @@ program wpf_driver
       write(*,'(a,i8,a,es22.15,a,es22.15,a,es22.15,a,i6)') " ITIME", itime, " TIME", time, &
          " DT", gparam%dtime, " TAUMIN", taumin, " ITS", solve_stats%iterations
 
+      if (n_dat_out > 0 .and. time > time_dat) then
+         call showProgress(filename(1:i-1)//".dat", time, itime, &
+                           getPressure(gparam%ix_ref, gparam%iy_ref, gparam%iz_ref))
+         time_dat = time_dat + dtime_dat
+      end if
       if (n_res_out > 0 .and. time > time_res) then
          call saveResult(filename(1:i-1)//".res", time)
          time_res = time_res + dtime_res
       end if
    end do
    if (n_res_out > 0) call doneResult()
+   if (n_dat_out > 0) call doneProgress()
 
    call showDiag(d)
    write(*,'(a,4es22.14)') " FINAL pv vcal rwhmax pinj", d%pv, d%vcal, d%rwhmax, d%pinj

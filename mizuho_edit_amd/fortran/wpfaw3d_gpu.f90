@@ -168,9 +168,15 @@ module wpfaw3d_gpu
    real(c_double), public :: time_start = 0.0d0, time_total = 0.0d0
    ! gid_output and scr_output of the .cfg (0 when absent)
    integer(c_int32_t), public :: n_res_out = 0, n_dat_out = 0
+   ! the .dat file of showProgress and its start clocks
+   integer :: progress_unit = 0
+   logical :: progress_open = .false.  ! (newunit= numbers are negative)
+   integer(8) :: progress_c0 = 0, progress_rate = 1
+   real(c_double) :: progress_cpu0 = 0.0d0
    type(c_ptr) :: ctx = c_null_ptr
 
    public :: loadConfig, syncHost, saveField, saveResult, doneResult, freeField, showDiag, getPressure
+   public :: showProgress, doneProgress
    public :: calcPermeability, solvePressure, calcVelocity, calcPhaseDeriv, calcConcentDeriv
    public :: evolvePhase, evolveConcent, updateHalo
 
@@ -266,6 +272,93 @@ contains
       type(wpf_diag), intent(out) :: d
       call check(wpf_diagnostics(ctx, d), "showProgress")
    end subroutine showDiag
+
+   !> showProgress (wPFAW3D.f90:1665-1810): the .dat file (input summary on
+   !> the first call, then one line per call) and the progress line on
+   !> stdout, from the device reductions of wpf_diagnostics
+   subroutine showProgress(filename, time, itime, pinj)
+      character(len=*), intent(in) :: filename
+      real(c_double), intent(in) :: time, pinj
+      integer, intent(in) :: itime
+      logical, save :: firsttime = .true.
+      real(c_double), save :: pv
+      real(c_double) :: xchl, qinj, vacid, vcaco3
+      type(wpf_diag) :: d
+      integer :: iw
+      call check(wpf_diagnostics(ctx, d), "showProgress")
+      xchl = gparam%cwtp*gparam%b100*gparam%rhoa/gparam%rhos
+      if (firsttime) then
+         open(newunit=progress_unit, file=filename, status="replace")
+         progress_open = .true.
+         call system_clock(progress_c0, progress_rate)
+         call cpu_time(progress_cpu0)
+         pv = d%pv
+         call progress_header(pv, xchl)
+         firsttime = .false.
+      end if
+      qinj = 0.0d0
+      do iw = 1, size(well)
+         if (well(iw)%injection /= 0.0d0) qinj = qinj + well(iw)%injection
+      end do
+      vacid = qinj*time
+      vcaco3 = qinj*time*xchl
+      write(progress_unit, '(1x,i8,1x,7es20.10)') itime, time, vacid/pv, pinj, d%rwhmax, vacid, vcaco3, d%vcal
+      write(*, "(i8,es13.5,es11.3,1x,4(f8.3),es13.5,es12.4,es10.2)") itime, time, vacid/pv, &
+         d%phase_max, d%phase_min, d%concent_max, d%concent_min, pinj, d%rwhmax, gparam%dtime
+   end subroutine showProgress
+
+   subroutine progress_header(pv, xchl)
+      real(c_double), intent(in) :: pv, xchl
+      real(c_double) :: b, kc
+      integer :: u
+      u = progress_unit
+      b = gparam%rhoa/gparam%rhos
+      kc = gparam%cwtp*gparam%b100/(1.0d0 - gparam%porosity)
+      write(u,*) "***** CALCULATION SUMMARY FOR PFACID PROGRAM *****"
+      write(u,*) "/// Input data ///"
+      write(u,*) "*** Grid data"
+      write(u,"(3x,'Num of grids, nx,ny,nz =',3i6)") gparam%nx, gparam%ny, gparam%nz
+      write(u,"(3x,'Grid size, dx       =',es14.6,' [m]')") gparam%dx
+      write(u,"(3x,'Core length, Lx, Ly, Lz =',3f10.4,' [m]')") gparam%dx*gparam%nx, gparam%dx*gparam%ny, &
+         gparam%dx*gparam%nz
+      write(u,*) "*** Rock and fluid data"
+      write(u,"(3x,'acid con., cwtp      =',f12.4,' [kg HCl/kg solution]')") gparam%cwtp
+      write(u,"(3x,'acid density, rhoa   =',f12.1,' [kg/m3]')") gparam%rhoa
+      write(u,"(3x,'rock density, rhos   =',f12.1,' [kg/m3]')") gparam%rhos
+      write(u,"(3x,'viscosity, visco     =',es14.6,' [Pa*s]')") gparam%viscosity
+      write(u,"(3x,'rock porosity, poro  =',f12.6,' [dec]')") gparam%porosity
+      write(u,"(3x,'diffusivity coef. ,d =',es12.4,' [m2/s]')") gparam%diffusion
+      write(u,"(3x,'gravimetric dissolving power, b100 =',f8.3,' [kg CaCO3/kg HCl]')") gparam%b100
+      write(u,"(3x,'volumetric dissolving power, Xacid =',f8.3,' [m3 CaCO3/m3 HCl]')") xchl
+      write(u,*) "*** Phase-field related parameters"
+      write(u,"(3x,'lambda =',f12.6,' [-]')") gparam%lambda
+      write(u,"(3x,'kc     =',f12.6,' [-]')") kc
+      write(u,"(3x,'b      =',f12.6,' [-]')") b
+      write(u,"(3x,'alpha  =',f12.6,' [-]')") gparam%alpha
+      write(u,*) "*** Injection data"
+      write(u,"(3x,'time step size, dt     =',es12.4,' [s]')") gparam%dtime
+      write(u,"(3x,'initial pressure, pini =',es12.4,' [m/s]')") gparam%pini
+      write(u,"(1x,'pore volume =',es20.6,' [m3]')") pv
+      write(u,"(//'/// SIMULATION RESULTS ///')")
+      write(u,"(13x,'ITIME',2x,'TIME',15x,'PVINJ',13x,'WELL_PRESS'," // &
+         "10x,'WH_LENGTH',10x,'INJ_ACID',11x,'DISSOLVED_CaCO3',10x,'DISS.CaCO3_c')")
+      write(u,"(13x,'(-)',4x,'(sec)',15x,'(v/v)',15x,'(Pa)'," // &
+         "16x,'(m)',14x,'(m3/m)',14x,'(m3/m)',14x,'(m3/m)')")
+      write(u,"(5x,'-----',2x,6('   --------------   '))")
+   end subroutine progress_header
+
+   !> doneProgress (wPFAW3D.f90:1812-1825): the run times, then close
+   subroutine doneProgress()
+      integer(8) :: c1
+      real(c_double) :: cpu1
+      if (.not. progress_open) return
+      call system_clock(c1)
+      call cpu_time(cpu1)
+      write(progress_unit, "(/'CPU time =',es12.4,' [min]')") (cpu1 - progress_cpu0)/60.0d0
+      write(progress_unit, "('Cal time =',es12.4,' [min]')") dble(c1 - progress_c0)/progress_rate/60.0d0
+      close(progress_unit)
+      progress_open = .false.
+   end subroutine doneProgress
 
    !> getPressure (wPFAW3D.f90:2151-2166)
    function getPressure(ix_g, iy_g, iz_g) result(p)

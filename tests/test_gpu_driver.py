@@ -23,6 +23,7 @@ def test_fortran_driver_matches_python_mirror(gpu, tmp_path):
     p = case.params
     p.time_total = 1.0e30
     p.n_res_out = 1  # saveResult at the start only (the next output is time_total later)
+    p.n_dat_out = 1  # showProgress likewise
     io.write_wells(str(tmp_path / "c.well"), case.wells)
     io.write_field(str(tmp_path / "c.fld"), 0.0, case.krock[1:-1, 1:-1, 1:-1],
                    case.phase[1:-1, 1:-1, 1:-1], case.concent[1:-1, 1:-1, 1:-1])
@@ -40,8 +41,20 @@ def test_fortran_driver_matches_python_mirror(gpu, tmp_path):
     # mirror's, byte for byte
     g0 = WPFAW3D(case)
     g0.saveResult(str(tmp_path / "py.res"), 0.0)
+    d0 = g0.showProgress()
     g0.close()
     assert (tmp_path / "c.res").read_bytes() == (tmp_path / "py.res").read_bytes()
+    # its .dat file (showProgress at the start, scr_output = 1): the summary,
+    # then the line of step 0 from the device reductions
+    dat = (tmp_path / "c.dat").read_text().splitlines()
+    assert dat[0].strip() == "***** CALCULATION SUMMARY FOR PFACID PROGRAM *****"
+    row = [l for l in dat if l.strip() and l.split()[0] == "0"]
+    assert len(row) == 1
+    itime_d, t_d, pvinj, pinj, rwh, vacid, vcaco3, vcal = row[0].split()
+    assert float(t_d) == 0.0 and float(vacid) == 0.0
+    assert float(vcal) == pytest.approx(d0.vcal, rel=1e-9)
+    assert float(rwh) == pytest.approx(d0.rwhmax, rel=1e-9, abs=1e-300)
+    assert any(l.startswith("Cal time =") for l in dat)
 
     g = WPFAW3D(case, opts=solver_opts("mgcg", rtol=1e-10, maxit=1000))
     for _ in range(nsteps):

@@ -973,9 +973,9 @@ static int project_m(wpf_ctx* c, const Fine<float, C>& F, double* x, double* rn)
                        (const uint8_t*)c->mask, c->red.partials);
     WPF_HIP(hipGetLastError());
     FINALIZE(nb, 1, S_PN);
-    int rc = read_results(c, S_PN, 1);
-    if (rc) return rc;
-    rc = read_results(c, S_PROJY + M, 1);
+    // one copy for both values the host needs (slots S_PN .. S_PROJY + M)
+    static_assert(S_PN < S_PROJY, "slot order");
+    int rc = read_results(c, S_PN, S_PROJY + M + 1 - S_PN);
     if (rc) return rc;
     c->proj_used = c->red.host[S_PROJY + M] != 0.0;
     *rn = std::sqrt(c->red.host[S_PN]);

@@ -153,6 +153,8 @@ def test_native_config_loader_matches_python_reader(tmp_path):
     """wpf_config_load/_wells/_fields (host code of libwpf_hip.so, no GPU
     needed) against the Python reader and the case that wrote the files."""
     case, cfg = _write_case(tmp_path)
+    case.params.n_res_out, case.params.n_dat_out = 3, 5
+    io.write_config(cfg, case.params, "c.well", init_file="c.fld")
     L = _lib.lib()
     L.wpf_config_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(WpfParams),
                                   ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
@@ -167,6 +169,10 @@ def test_native_config_loader_matches_python_reader(tmp_path):
     for name, _ in WpfParams._fields_:
         assert getattr(cp, name) == getattr(ref, name), name
     assert nw.value == len(case.wells) and t0.value == 0.25 and tt.value == 1.0e30
+    # the output counts (gid_output, scr_output) the driver schedules with
+    nres, ndat = ctypes.c_int32(-1), ctypes.c_int32(-1)
+    assert L.wpf_config_output(ctypes.byref(nres), ctypes.byref(ndat)) == 0
+    assert (nres.value, ndat.value) == (case.params.n_res_out, case.params.n_dat_out)
     w = (WpfWell * nw.value)()
     assert L.wpf_config_wells(w) == 0
     assert all((w[i].ix_g, w[i].pressure, w[i].concent) ==

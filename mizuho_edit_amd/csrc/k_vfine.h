@@ -266,8 +266,8 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
                                            const R* __restrict__ p, const R* __restrict__ r,
                                            const R* __restrict__ q, R* __restrict__ r2,
                                            const C* __restrict__ tx, const C* __restrict__ ty,
-                                           const C* __restrict__ tz, const R* __restrict__ invD, R* y,
-                                           double (&acc)[2], R* zs = nullptr)
+                                           const C* __restrict__ tz, const R* __restrict__ invD,
+                                           R* __restrict__ y, double (&acc)[2], R* zs = nullptr)
 {
     const R al = (R)(res[num] / res[den]);
     int64_t qi;
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict
                                                   const R* __restrict__ r, const R* __restrict__ q,
                                                   R* __restrict__ r2, const C* __restrict__ tx,
                                                   const C* __restrict__ ty, const C* __restrict__ tz,
-                                                  const R* __restrict__ invD, R* y,
+                                                  const R* __restrict__ invD, R* __restrict__ y,
                                                   double* __restrict__ partials)
 {
     double acc[2];

@@ -205,8 +205,11 @@ int wpf_step(wpf_ctx* ctx, const wpf_solver_opts* opts, int32_t itime,
              wpf_solve_stats* stats);
 
 /* Breakthrough: first step at which a non-well interior cell in the plane
- * ix_g = nx-1 (next to the outlet plane) holds phase > 0.  Returns 0 while
- * it has not happened.  Checked by wpf_step after every Euler update. */
+ * ix_g = nx-2 holds phase > 0.  Returns 0 while it has not happened.
+ * Checked by wpf_step after every Euler update.  The plane is two before
+ * the outlet wells: wells hold their phase, and an outlet plane held at
+ * phase = -1 keeps the interface's zero level about 1.5 cells away from
+ * it, so the plane next to it approaches 0 from below without crossing. */
 int wpf_breakthrough_step(wpf_ctx* ctx, int32_t* step);
 /* Move the breakthrough plane to ix_g (1-based) and clear the recorded step.
  * The reference has no breakthrough event; this is the workload's own

@@ -376,7 +376,9 @@ __global__ void __launch_bounds__(256) k_phase_deriv(Dims d, PhaseConsts K,
     block_min_store(tau, mins, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
-int launch_phase_deriv(wpf_ctx* c, double* taumin)
+// calcPhaseDeriv with its block minima in mins[0..cap) and the taumin key
+// in *key (both device memory).
+static int phase_deriv_into(wpf_ctx* c, double* mins, int64_t cap, unsigned long long* key, double* taumin)
 {
     WPF_CHECK(c->have_velocity, WPF_ERR_STATE, "calcPhaseDeriv before calcVelocity");
     const Dims d = dims_of(c);
@@ -384,22 +386,28 @@ int launch_phase_deriv(wpf_ctx* c, double* taumin)
     // taumin starts at 1e6 (1:1223): k_min_key folds that in
     dim3 blk(64, 4, 1);
     const dim3 gr = grid3(d, blk);
-    const int nb = (int)(gr.x * gr.y * gr.z);
-    WPF_CHECK((int64_t)nb <= (int64_t)c->red.max_blocks * c->red.slots, WPF_ERR_ARG,
-              "grid too large for the reduction scratch");
+    const int64_t nb = (int64_t)gr.x * gr.y * gr.z;
+    WPF_CHECK(nb <= cap, WPF_ERR_ARG, "calcPhaseDeriv: the grid needs %lld block minima, the scratch holds %lld",
+              (long long)nb, (long long)cap);
     hipLaunchKernelGGL(k_phase_deriv, gr, blk, 0, c->stream, d, K, c->phi[c->cur], c->conc[c->cur], c->vx,
-                       c->vy, c->vz, c->mask, c->dphidt, c->red.partials);
+                       c->vy, c->vz, c->mask, c->dphidt, mins);
     WPF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_min_key, dim3(1), dim3(1024), 0, c->stream, (const double*)c->red.partials, nb,
-                       c->tau_key, 3);
+    hipLaunchKernelGGL(k_min_key, dim3(1), dim3(1024), 0, c->stream, (const double*)mins, (int)nb, key, 0);
     WPF_HIP(hipGetLastError());
     unsigned long long out;
-    WPF_HIP(hipMemcpyAsync(&out, c->tau_key + 3, 8, hipMemcpyDeviceToHost, c->stream));
+    WPF_HIP(hipMemcpyAsync(&out, key, 8, hipMemcpyDeviceToHost, c->stream));
     WPF_HIP(hipStreamSynchronize(c->stream));
     unsigned long long u = (out & 0x8000000000000000ull) ? (out & 0x7fffffffffffffffull) : ~out;
     memcpy(taumin, &u, 8);
     return WPF_OK;
 }
+
+int launch_phase_deriv(wpf_ctx* c, double* taumin)
+{
+    return phase_deriv_into(c, c->red.partials, c->red.cap, c->tau_key + 3, taumin);
+}
+
+
 
 // ---------------------------------------------------------------------------
 // calcConcentDeriv (per-procedure form)
@@ -691,8 +699,7 @@ int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt,
     const dim3 gr(gx, gy, gz);
     prof_mark(c, WPF_KERNEL_STEP_FIELDS, true);
     // the block minima use the reduction scratch as one flat array
-    WPF_CHECK((int64_t)nb <= (int64_t)c->red.max_blocks * c->red.slots, WPF_ERR_ARG,
-              "grid too large for the reduction scratch");
+    WPF_CHECK(reduction_fits(c->red, nb, 1), WPF_ERR_ARG, "grid too large for the reduction scratch");
     if (f32)
         hipLaunchKernelGGL(k_step_fields<float>, gr, blk, 0, c->stream, d, consts_as<float>(K), K.dx, S,
                            c->phi[c->cur], c->conc[c->cur], c->p.p, c->tx.p, c->ty.p, c->tz.p, c->mask,
@@ -777,7 +784,7 @@ int launch_diag(wpf_ctx* c, wpf_diag* out)
     dim3 blk(64, 4, 1);
     dim3 grd = grid3(d, blk);
     const int nb = (int)(grd.x * grd.y * grd.z);
-    WPF_CHECK(nb <= c->red.max_blocks, WPF_ERR_ARG, "diagnostics grid too large");
+    WPF_CHECK(reduction_fits(c->red, nb, 2), WPF_ERR_ARG, "diagnostics grid too large for the reduction scratch");
     unsigned long long init[5];
     const double big = 1.0e308;
     for (int q = 0; q < 5; q++) {

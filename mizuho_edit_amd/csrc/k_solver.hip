@@ -1281,6 +1281,9 @@ int solve_pressure(wpf_ctx* c, const wpf_solver_opts* in, wpf_solve_stats* st)
     WPF_HIP(hipStreamSynchronize(c->stream));
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     c->have_pressure = true;
+    if (rc == WPF_ERR_NOCONV)
+        set_error("pressure solve: residual %.3e above max(rtol*bnorm, atol) = %.3e after %d iterations",
+                  st->rnorm, std::max(o.rtol * st->bnorm, o.atol), st->iterations);
     return rc;
 }
 

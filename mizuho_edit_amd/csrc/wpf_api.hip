@@ -23,17 +23,29 @@ void set_error(const char* fmt, ...)
     va_end(ap);
 }
 
-int reducer_init(Reducer& r, int slots, int max_blocks)
+int reducer_init(Reducer& r, int slots, int64_t cap)
 {
     r.slots = slots;
-    r.max_blocks = max_blocks;
-    WPF_HIP(hipMalloc((void**)&r.partials, sizeof(double) * 4 * (size_t)max_blocks));
+    r.cap = cap;
+    WPF_HIP(hipMalloc((void**)&r.partials, sizeof(double) * (size_t)cap));
     WPF_HIP(hipMalloc((void**)&r.counter, sizeof(unsigned) * 4));
     WPF_HIP(hipMemset(r.counter, 0, sizeof(unsigned) * 4));
     WPF_HIP(hipMalloc((void**)&r.result, sizeof(double) * slots));
     WPF_HIP(hipMemset(r.result, 0, sizeof(double) * slots));
     WPF_HIP(hipHostMalloc((void**)&r.host, sizeof(double) * slots, hipHostMallocDefault));
     return WPF_OK;
+}
+
+int64_t reduction_capacity(const Grid& g)
+{
+    // block counts: 64 x 4 x 1 blocks of the per-procedure stencils (the
+    // fused step's and the scalar solver's tiles are coarser), the vector
+    // z-march kernels (16 bytes per thread, so at least 2 cells, and at least
+    // one plane per tile), and the capped 1-D grids
+    const int64_t b64x4 = (int64_t)((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
+    const int64_t bvec = ((g.plane + 511) / 512) * g.nz;
+    const int64_t nb = std::max<int64_t>({(int64_t)2048, b64x4, bvec});
+    return nb * kMaxValuesPerBlock;
 }
 
 void reducer_free(Reducer& r)
@@ -272,8 +284,8 @@ int wpf_create(const wpf_params* p, const int32_t* lw_haloed, const wpf_well* we
         int32_t v = 0x7fffffff;
         WPF_HIP(hipMemcpy(c->bt_step, &v, 4, hipMemcpyHostToDevice));
     }
-    c->bt_plane = g.nx >= 2 ? g.nx - 2 : -1;
-    if ((rc = reducer_init(c->red, 64, 65536))) {
+    c->bt_plane = g.nx >= 3 ? g.nx - 3 : -1;  // ix_g = nx-2
+    if ((rc = reducer_init(c->red, 64, reduction_capacity(g)))) {
         wpf_destroy(c);
         return rc;
     }

@@ -83,14 +83,20 @@ struct MGLevel {
 };
 
 // Reduction scratch: per-block partials + completion counter + results.
+// A launch of nb blocks that stores nv values per block (value v of block b
+// at partials[v * nb + b]) needs nb * nv <= cap; reduction_fits checks it.
 struct Reducer {
-    double* partials = nullptr;  // [slots * max_blocks]
-    unsigned* counter = nullptr; // [slots]
+    double* partials = nullptr;  // [cap]
+    unsigned* counter = nullptr; // [4]
     double* result = nullptr;    // [slots] device
     double* host = nullptr;      // pinned mirror of result
-    int max_blocks = 0;
-    int slots = 0;
+    int64_t cap = 0;             // doubles in partials
+    int slots = 0;               // result slots
 };
+inline bool reduction_fits(const Reducer& r, int64_t nb, int nv) { return nb * (int64_t)nv <= r.cap; }
+// The most values one block of any kernel stores (the projection's Gram pass:
+// 10 packed Gram entries + 4 right-hand sides).
+static constexpr int kMaxValuesPerBlock = 16;
 
 }  // namespace wpf
 
@@ -289,7 +295,10 @@ void dev_free(Dev<T>& a)
     a.raw = a.p = nullptr;
 }
 
-int reducer_init(Reducer& r, int slots, int max_blocks);
+int reducer_init(Reducer& r, int slots, int64_t cap);
+// Partials a context of this grid needs: the largest block count of any of
+// its launches times kMaxValuesPerBlock.
+int64_t reduction_capacity(const Grid& g);
 void reducer_free(Reducer& r);
 
 }  // namespace wpf

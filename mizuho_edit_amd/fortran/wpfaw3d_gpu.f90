@@ -163,6 +163,9 @@ module wpfaw3d_gpu
    type(wpf_solver_opts), public :: solver
    type(wpf_solve_stats), public :: solve_stats
    type(wpf_well), allocatable, public :: well(:)
+   ! param%nwell: the number of wells loaded (well(:) holds at least one
+   ! record, so loops over the wells run to nwell, not size(well))
+   integer(c_int32_t), public :: nwell = 0
    integer(c_int32_t), allocatable, public :: lw(:,:,:)
    real(c_double), allocatable, public :: permeab_rock(:,:,:), pressure(:,:,:), phase(:,:,:), concent(:,:,:)
    real(c_double), public :: time_start = 0.0d0, time_total = 0.0d0
@@ -202,7 +205,6 @@ contains
    !! GPU context and the upload of the loaded state.
    subroutine loadConfig(filename)
       character(len=*), intent(in) :: filename
-      integer(c_int32_t) :: nwell
       call check(wpf_config_load(trim(filename)//c_null_char, gparam, nwell, time_start, time_total), &
                  "loadConfig")
       call check(wpf_config_output(n_res_out, n_dat_out), "loadConfig(output)")
@@ -297,7 +299,7 @@ contains
          firsttime = .false.
       end if
       qinj = 0.0d0
-      do iw = 1, size(well)
+      do iw = 1, nwell
          if (well(iw)%injection /= 0.0d0) qinj = qinj + well(iw)%injection
       end do
       vacid = qinj*time

@@ -49,6 +49,14 @@ def lib():
                                             ctypes.c_double, ctypes.c_int, _D, _D, _D]
         L.oracle_solve_pressure.restype = ctypes.c_int
         L.oracle_pressure_residual.argtypes = [P, _I, P, _D, _D, _D, _D]
+        L.oracle_solve_pressure_bcgs.argtypes = [P, _I, P, _D, _D, _D, _D, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_int, _D, _D, _D,
+                                                 ctypes.POINTER(ctypes.c_int)]
+        L.oracle_solve_pressure_bcgs.restype = ctypes.c_int
+        L.oracle_sor_apply.argtypes = [P, _I, P, _D, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int]
+        L.oracle_sor_operator.argtypes = [P, _I, P, _D, _D, _D, _D, _D]
         L.oracle_pressure_residual.restype = ctypes.c_double
         L.oracle_calc_velocity.argtypes = [P, _D, _D, _D, _D, _D, _D, _D]
         L.oracle_phase_cell.argtypes = [P, _D, ctypes.c_double, _D, ctypes.c_int, _D, _D]
@@ -64,6 +72,8 @@ def lib():
         L.oracle_diagnostics.argtypes = [P, _D, _D, _D, P]
         L.oracle_breakthrough.argtypes = [P, _I, _D]
         L.oracle_breakthrough.restype = ctypes.c_int
+        L.oracle_breakthrough_plane.argtypes = [P, _I, _D, ctypes.c_int]
+        L.oracle_breakthrough_plane.restype = ctypes.c_int
         L.oracle_run.argtypes = ([P, _I, P, ctypes.c_int] + [_D] * 13 +
                                  [ctypes.c_int, ctypes.c_int, _D, _D, _D, ctypes.c_double,
                                   ctypes.c_int, _D, ctypes.c_int])
@@ -130,6 +140,37 @@ class OracleState:
                                           ctypes.byref(bn))
         return its, r0.value, rn.value, bn.value
 
+    def solve_pressure_bcgs(self, rtol=1e-10, atol=1e-50, maxit=1000, sor_its=10, omega=1.7,
+                            nblocks=0):
+        """The reference's own solver (wPFAW3D.f90:989-1001): BiCGStab with
+        SOR (its outer iterations of block-local symmetric sweeps over
+        nblocks row blocks, the ranks of an mpirun; 0: one per thread).
+        Returns (iterations, preconditioned residual, ||M^-1 b||, true
+        ||D^-1 r||, converged)."""
+        rn, bn, tr = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        conv = ctypes.c_int()
+        its = lib().oracle_solve_pressure_bcgs(_ref(self.cp), _i(self.lw), _ref(self.cw), _d(self.kx),
+                                               _d(self.ky), _d(self.kz), _d(self.pressure), rtol, atol,
+                                               maxit, sor_its, omega, nblocks, ctypes.byref(rn),
+                                               ctypes.byref(bn), ctypes.byref(tr), ctypes.byref(conv))
+        return its, rn.value, bn.value, tr.value, bool(conv.value)
+
+    def sor_apply(self, r, its=10, omega=1.7, nblocks=1):
+        """M^-1 r of the SOR preconditioner on an interior vector (x fastest)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        lib().oracle_sor_apply(_ref(self.cp), _i(self.lw), _ref(self.cw), _d(self.kx), _d(self.ky),
+                               _d(self.kz), _d(r), _d(z), its, omega, nblocks)
+        return z
+
+    def sor_operator(self, x):
+        """A x with the rows the SOR sweeps relax (identity at Dirichlet cells)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        lib().oracle_sor_operator(_ref(self.cp), _i(self.lw), _ref(self.cw), _d(self.kx), _d(self.ky),
+                                  _d(self.kz), _d(x), _d(y))
+        return y
+
     def pressure_residual(self, pressure=None):
         pr = self.pressure if pressure is None else pressure
         return lib().oracle_pressure_residual(_ref(self.cp), _i(self.lw), _ref(self.cw),
@@ -165,7 +206,30 @@ class OracleState:
                                  _d(self.pressure), _ref(d))
         return d
 
+    def phase_positive_on_plane(self, ix: int) -> bool:
+        """The breakthrough test: a non-well cell of plane ix (1-based) holds
+        phase > 0."""
+        return bool(lib().oracle_breakthrough_plane(_ref(self.cp), _i(self.lw), _d(self.phase), ix))
+
     # -- time loop ---------------------------------------------------------
+    def step_bcgs(self, rtol=1e-10, maxit=1000, nblocks=0) -> dict:
+        """One pass of the loop body of wPFAW3D.f90:2225-2294 with the
+        reference's own solver (BiCGStab + SOR(10, 1.7)), the procedures
+        called in the order of oracle_run.  Returns the solve's record."""
+        self.itime += 1
+        if self.itime >= 3:
+            self.dtime = self.dt_rule(self.taumin)
+        self.time += self.dtime
+        self.calc_permeability()
+        its, rn, bn, tr, conv = self.solve_pressure_bcgs(rtol=rtol, maxit=maxit, nblocks=nblocks)
+        self.calc_velocity()
+        self.taumin = self.calc_phase_deriv()
+        self.calc_concent_deriv()
+        self.evolve_phase(self.dtime)
+        self.evolve_concent(self.dtime)
+        self.hist.append([self.dtime, self.taumin, its, rn / (bn if bn > 0 else 1.0)])
+        return {"iterations": its, "converged": conv, "rnorm": rn, "bnorm": bn, "true_rnorm": tr}
+
     def run(self, nsteps: int, rtol=1e-10, maxit=1000, nthreads=0) -> int:
         t, dt, tm = ctypes.c_double(self.time), ctypes.c_double(self.dtime), ctypes.c_double(self.taumin)
         hist = np.zeros(4 * nsteps)

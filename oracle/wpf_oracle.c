@@ -16,6 +16,8 @@
  *   oracle_assemble            wPFAW3D.f90:603-936 (4 passes, COO)
  *   oracle_solve_pressure      wPFAW3D.f90:938-1050, solved as the SPD
  *                              equivalent by CG + Jacobi (see the function)
+ *   oracle_solve_pressure_bcgs wPFAW3D.f90:989-1001, the reference's own
+ *                              BiCGStab + SOR(10 its, omega 1.7)
  *   oracle_calc_velocity       wPFAW3D.f90:1164-1193
  *   oracle_phase_cell          wPFAW3D.f90:1231-1414 (one cell, intermediates)
  *   oracle_calc_phase_deriv    wPFAW3D.f90:1197-1475
@@ -366,6 +368,286 @@ int oracle_solve_pressure(const wpf_params* p, const int32_t* lw, const wpf_well
     if (rnorm_out) *rnorm_out = rnorm;
     if (bnorm_out) *bnorm_out = bnorm;
     free(x); free(r); free(z); free(pp); free(q); free(dinv); free(f); free(fixed);
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The reference's own KSP configuration, wPFAW3D.f90:989-1001: BiCGStab     */
+/* (KSPBCGS) with the SOR preconditioner, PCSORSetIterations(pc, 10, 1) and  */
+/* omega = 1.7, rtol 1e-10, x0 = the previous pressure.                     */
+/*                                                                          */
+/* PETSc is not in the reference tree; what follows restates the published  */
+/* algorithms the calls select (PETSc's manual pages for PCSOR, MatSOR and   */
+/* KSPBCGS), not PETSc's code:                                              */
+/*  - PCSOR's default sweep is SOR_LOCAL_SYMMETRIC_SWEEP, applied from a    */
+/*    zero initial guess.  On P ranks the matrix rows are split into P      */
+/*    contiguous blocks (PETSC_DECIDE: n/P rows each, the first n%P ranks   */
+/*    one more).  Each of the `its` outer iterations takes the coupling to  */
+/*    the other blocks from the iterate at its start and does one forward   */
+/*    and one backward Gauss-Seidel sweep with relaxation omega inside each */
+/*    block (lits = 1).  P = 1 is plain symmetric SOR.                      */
+/*  - KSPBCGS is van der Vorst's BiCGStab (SIAM J. Sci. Stat. Comput. 13,    */
+/*    1992) on the left-preconditioned system M^-1 A x = M^-1 b, with       */
+/*    PETSc's default test on the preconditioned residual,                 */
+/*    ||r|| <= max(rtol ||M^-1 b||, atol), also checked after the half step.*/
+/* Rows are the SPD form of the other solvers (free rows negated, Dirichlet */
+/* columns eliminated): SOR sweeps are unchanged by negating a row, and the */
+/* preconditioned system is the same, so the iterates are those of the      */
+/* reference's system.  Dirichlet rows are identity rows; x starts there at */
+/* the Dirichlet value (the reference's x0 holds it from step 2 on).        */
+/* ------------------------------------------------------------------------ */
+double oracle_pressure_residual(const wpf_params* p, const int32_t* lw, const wpf_well* w,
+                                const double* kx, const double* ky, const double* kz,
+                                const double* pressure);
+
+typedef struct {
+    int nx, ny, nz;
+    int64_t n, plane;
+    double* c[3];          /* coupling to the +x/+y/+z neighbour, 0 unless both cells are free */
+    double* d;             /* diagonal: sum of the cell's six face t (1 at Dirichlet cells) */
+    unsigned char* fixed;
+} sor_op;
+
+static void sor_op_init(sor_op* o, const wpf_params* p, const int32_t* lw, const wpf_well* w,
+                        const double* kx, const double* ky, const double* kz)
+{
+    DECL_DIMS(p);
+    const double mu = p->viscosity;
+    o->nx = NX; o->ny = NY; o->nz = NZ;
+    o->plane = (int64_t)NX * NY;
+    o->n = o->plane * NZ;
+    for (int a = 0; a < 3; a++) o->c[a] = malloc(sizeof(double) * o->n);
+    o->d = malloc(sizeof(double) * o->n);
+    o->fixed = malloc(o->n);
+    for (int k = 1; k <= NZ; k++)
+        for (int j = 1; j <= NY; j++)
+            for (int i = 1; i <= NX; i++)
+                o->fixed[(i - 1) + (int64_t)NX * ((j - 1) + (int64_t)NY * (k - 1))] =
+                    (unsigned char)on_boundary(lw, w, H(i, j, k));
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= NZ; k++)
+        for (int j = 1; j <= NY; j++)
+            for (int i = 1; i <= NX; i++) {
+                int64_t c = H(i, j, k);
+                int64_t m = (i - 1) + (int64_t)NX * ((j - 1) + (int64_t)NY * (k - 1));
+                double txp = (i < NX) ? face_t(kx, c, mu) : 0.0;
+                double txm = (i > 1) ? face_t(kx, H(i - 1, j, k), mu) : 0.0;
+                double typ = (j < NY) ? face_t(ky, c, mu) : 0.0;
+                double tym = (j > 1) ? face_t(ky, H(i, j - 1, k), mu) : 0.0;
+                double tzp = (k < NZ) ? face_t(kz, c, mu) : 0.0;
+                double tzm = (k > 1) ? face_t(kz, H(i, j, k - 1), mu) : 0.0;
+                o->d[m] = o->fixed[m] ? 1.0 : txp + txm + typ + tym + tzp + tzm;
+                o->c[0][m] = (i < NX && !o->fixed[m] && !o->fixed[m + 1]) ? txp : 0.0;
+                o->c[1][m] = (j < NY && !o->fixed[m] && !o->fixed[m + NX]) ? typ : 0.0;
+                o->c[2][m] = (k < NZ && !o->fixed[m] && !o->fixed[m + o->plane]) ? tzp : 0.0;
+            }
+}
+
+static void sor_op_free(sor_op* o)
+{
+    for (int a = 0; a < 3; a++) free(o->c[a]);
+    free(o->d);
+    free(o->fixed);
+}
+
+/* y = A x: identity rows at Dirichlet cells, d x - sum c x_nb elsewhere */
+static void sor_op_apply(const sor_op* o, const double* x, double* y)
+{
+    const int NX = o->nx, NY = o->ny, NZ = o->nz;
+    const int64_t P = o->plane;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < NZ; k++)
+        for (int j = 0; j < NY; j++)
+            for (int i = 0; i < NX; i++) {
+                int64_t m = i + (int64_t)NX * (j + (int64_t)NY * k);
+                if (o->fixed[m]) { y[m] = x[m]; continue; }
+                double s = 0.0;
+                if (i > 0) s += o->c[0][m - 1] * x[m - 1];
+                if (i < NX - 1) s += o->c[0][m] * x[m + 1];
+                if (j > 0) s += o->c[1][m - NX] * x[m - NX];
+                if (j < NY - 1) s += o->c[1][m] * x[m + NX];
+                if (k > 0) s += o->c[2][m - P] * x[m - P];
+                if (k < NZ - 1) s += o->c[2][m] * x[m + P];
+                y[m] = o->d[m] * x[m] - s;
+            }
+}
+
+/* One relaxation of row m from the current z (inside [r0, r1)) and the
+ * start-of-iteration zo (outside). */
+static inline double sor_row(const sor_op* o, const double* rhs, const double* z, const double* zo,
+                             int64_t m, int i, int j, int k, int64_t r0, int64_t r1, double omega)
+{
+    if (o->fixed[m]) return (1.0 - omega) * z[m] + omega * rhs[m];
+    const int64_t P = o->plane;
+    const int NX = o->nx;
+    double s = 0.0;
+#define NBV(q) (((q) >= r0 && (q) < r1) ? z[q] : zo[q])
+    if (i > 0) s += o->c[0][m - 1] * NBV(m - 1);
+    if (i < o->nx - 1) s += o->c[0][m] * NBV(m + 1);
+    if (j > 0) s += o->c[1][m - NX] * NBV(m - NX);
+    if (j < o->ny - 1) s += o->c[1][m] * NBV(m + NX);
+    if (k > 0) s += o->c[2][m - P] * NBV(m - P);
+    if (k < o->nz - 1) s += o->c[2][m] * NBV(m + P);
+#undef NBV
+    return (1.0 - omega) * z[m] + omega * (rhs[m] + s) / o->d[m];
+}
+
+/* z = M^-1 rhs: its outer iterations of block-local symmetric SOR from
+ * z = 0 over nblocks contiguous row blocks; zo is scratch (n). */
+static void sor_apply(const sor_op* o, const double* rhs, double* z, double* zo, int its, double omega,
+                      int nblocks)
+{
+    const int64_t n = o->n;
+    memset(z, 0, sizeof(double) * n);
+    for (int it = 0; it < its; it++) {
+        memcpy(zo, z, sizeof(double) * n);
+#pragma omp parallel for schedule(static, 1)
+        for (int b = 0; b < nblocks; b++) {
+            const int64_t q = n / nblocks, rem = n % nblocks;
+            const int64_t r0 = b * q + (b < rem ? b : rem);
+            const int64_t r1 = r0 + q + (b < rem ? 1 : 0);
+            if (r1 <= r0) continue;
+            int i = (int)(r0 % o->nx), j = (int)((r0 / o->nx) % o->ny), k = (int)(r0 / o->plane);
+            for (int64_t m = r0; m < r1; m++) {             /* forward */
+                z[m] = sor_row(o, rhs, z, zo, m, i, j, k, r0, r1, omega);
+                if (++i == o->nx) { i = 0; if (++j == o->ny) { j = 0; k++; } }
+            }
+            for (int64_t m = r1 - 1; m >= r0; m--) {        /* backward */
+                if (--i < 0) { i = o->nx - 1; if (--j < 0) { j = o->ny - 1; k--; } }
+                z[m] = sor_row(o, rhs, z, zo, m, i, j, k, r0, r1, omega);
+            }
+        }
+    }
+}
+
+/* M^-1 r on interior arrays (x fastest), for the tests that pin the sweep. */
+void oracle_sor_apply(const wpf_params* p, const int32_t* lw, const wpf_well* w, const double* kx,
+                      const double* ky, const double* kz, const double* r, double* z, int its,
+                      double omega, int nblocks)
+{
+    sor_op o;
+    sor_op_init(&o, p, lw, w, kx, ky, kz);
+    double* zo = malloc(sizeof(double) * o.n);
+    sor_apply(&o, r, z, zo, its, omega, nblocks < 1 ? 1 : nblocks);
+    free(zo);
+    sor_op_free(&o);
+}
+
+/* A x on interior arrays with the same rows (tests). */
+void oracle_sor_operator(const wpf_params* p, const int32_t* lw, const wpf_well* w, const double* kx,
+                         const double* ky, const double* kz, const double* x, double* y)
+{
+    sor_op o;
+    sor_op_init(&o, p, lw, w, kx, ky, kz);
+    sor_op_apply(&o, x, y);
+    sor_op_free(&o);
+}
+
+/* BiCGStab + SOR.  nblocks < 1: one block per OpenMP thread.  Returns the
+ * iterations; the final preconditioned residual norm, ||M^-1 b||, the true
+ * ||D^-1 r|| of the other solvers' test and whether the preconditioned test
+ * held come back through the pointers. */
+int oracle_solve_pressure_bcgs(const wpf_params* p, const int32_t* lw, const wpf_well* w,
+                               const double* kx, const double* ky, const double* kz, double* pressure,
+                               double rtol, double atol, int maxit, int sor_its, double omega,
+                               int nblocks, double* rnorm_out, double* bnorm_out, double* true_rnorm_out,
+                               int* converged_out)
+{
+    DECL_DIMS(p);
+    const double dx = p->dx;
+#ifdef _OPENMP
+    if (nblocks < 1) nblocks = omp_get_max_threads();
+#else
+    if (nblocks < 1) nblocks = 1;
+#endif
+    sor_op o;
+    sor_op_init(&o, p, lw, w, kx, ky, kz);
+    const int64_t n = o.n;
+    double* x = malloc(sizeof(double) * n);
+    double* bsys = malloc(sizeof(double) * n);
+    double* f = malloc(sizeof(double) * n);
+    double* r = malloc(sizeof(double) * n);
+    double* rh = malloc(sizeof(double) * n);
+    double* pv = malloc(sizeof(double) * n);
+    double* v = malloc(sizeof(double) * n);
+    double* s = malloc(sizeof(double) * n);
+    double* t = malloc(sizeof(double) * n);
+    double* tmp = malloc(sizeof(double) * n);
+    double* zo = malloc(sizeof(double) * n);
+    for (int k = 1; k <= NZ; k++)
+        for (int j = 1; j <= NY; j++)
+            for (int i = 1; i <= NX; i++) {
+                int64_t c = H(i, j, k);
+                int64_t m = (i - 1) + (int64_t)NX * ((j - 1) + (int64_t)NY * (k - 1));
+                x[m] = o.fixed[m] ? value_pressure(lw, w, c) : pressure[c];
+                f[m] = (!o.fixed[m] && on_inlet(lw, w, c)) ? value_injection(lw, w, c) * (dx * dx / (dx * dx * dx)) : 0.0;
+            }
+    /* the system's right-hand side: Dirichlet values on their rows, the
+     * source plus the eliminated Dirichlet couplings on free rows */
+    for (int64_t m = 0; m < n; m++) tmp[m] = o.fixed[m] ? x[m] : 0.0;
+    spd_apply(p, kx, ky, kz, o.fixed, tmp, v, 1);       /* v_i = -sum_Dir t p_D on free rows */
+    for (int64_t m = 0; m < n; m++) bsys[m] = o.fixed[m] ? x[m] : f[m] - v[m];
+    sor_apply(&o, bsys, tmp, zo, sor_its, omega, nblocks);
+    const double bnorm = sqrt(dot((int)n, tmp, tmp));
+    const double tol = fmax(rtol * bnorm, atol);
+    /* r = M^-1 (b - A x) */
+    sor_op_apply(&o, x, v);
+    for (int64_t m = 0; m < n; m++) tmp[m] = bsys[m] - v[m];
+    sor_apply(&o, tmp, r, zo, sor_its, omega, nblocks);
+    double rnorm = sqrt(dot((int)n, r, r));
+    int it = 0, conv = rnorm <= tol;
+    memcpy(rh, r, sizeof(double) * n);
+    memset(pv, 0, sizeof(double) * n);
+    memset(v, 0, sizeof(double) * n);
+    double rho_old = 1.0, alpha = 1.0, om = 1.0;
+    while (!conv && it < maxit) {
+        const double rho = dot((int)n, rh, r);
+        if (rho == 0.0) break;                          /* breakdown */
+        const double beta = (rho / rho_old) * (alpha / om);
+#pragma omp parallel for schedule(static)
+        for (int64_t m = 0; m < n; m++) pv[m] = r[m] + beta * (pv[m] - om * v[m]);
+        sor_op_apply(&o, pv, tmp);
+        sor_apply(&o, tmp, v, zo, sor_its, omega, nblocks);
+        const double rv = dot((int)n, rh, v);
+        if (rv == 0.0) break;
+        alpha = rho / rv;
+#pragma omp parallel for schedule(static)
+        for (int64_t m = 0; m < n; m++) s[m] = r[m] - alpha * v[m];
+        it++;
+        const double snorm = sqrt(dot((int)n, s, s));
+        if (snorm <= tol) {                              /* converged at the half step */
+#pragma omp parallel for schedule(static)
+            for (int64_t m = 0; m < n; m++) x[m] += alpha * pv[m];
+            rnorm = snorm;
+            conv = 1;
+            break;
+        }
+        sor_op_apply(&o, s, tmp);
+        sor_apply(&o, tmp, t, zo, sor_its, omega, nblocks);
+        const double tt = dot((int)n, t, t);
+        if (tt == 0.0) break;
+        om = dot((int)n, t, s) / tt;
+#pragma omp parallel for schedule(static)
+        for (int64_t m = 0; m < n; m++) {
+            x[m] += alpha * pv[m] + om * s[m];
+            r[m] = s[m] - om * t[m];
+        }
+        rnorm = sqrt(dot((int)n, r, r));
+        conv = rnorm <= tol;
+        rho_old = rho;
+        if (om == 0.0) break;
+    }
+    for (int k = 1; k <= NZ; k++)
+        for (int j = 1; j <= NY; j++)
+            for (int i = 1; i <= NX; i++)
+                pressure[H(i, j, k)] = x[(i - 1) + (int64_t)NX * ((j - 1) + (int64_t)NY * (k - 1))];
+    oracle_update_halo(p, pressure);
+    if (rnorm_out) *rnorm_out = rnorm;
+    if (bnorm_out) *bnorm_out = bnorm;
+    if (converged_out) *converged_out = conv;
+    free(x); free(bsys); free(f); free(r); free(rh); free(pv); free(v); free(s); free(t); free(tmp); free(zo);
+    sor_op_free(&o);
+    if (true_rnorm_out) *true_rnorm_out = oracle_pressure_residual(p, lw, w, kx, ky, kz, pressure);
     return it;
 }
 
@@ -721,17 +1003,26 @@ void oracle_diagnostics(const wpf_params* p, const double* phase, const double* 
 }
 
 /* Breakthrough test applied after each Euler update: any non-well cell of
- * plane i = nx-1 with phase > 0. */
-int oracle_breakthrough(const wpf_params* p, const int32_t* lw, const double* phase)
+ * the plane i = ix (1-based) with phase > 0.  The default plane is
+ * ix = nx-2, two planes before the outlet wells: the outlet wells hold
+ * phase = -1, which keeps the zero level of the phase-field interface about
+ * 1.5 cells away from them, so plane nx-1 approaches 0 from below without
+ * crossing it while the front does pass plane nx-2. */
+int oracle_breakthrough_plane(const wpf_params* p, const int32_t* lw, const double* phase, int ix)
 {
     DECL_DIMS(p);
-    if (NX < 2) return 0;
+    if (ix < 1 || ix > NX) return 0;
     for (int k = 1; k <= NZ; k++)
         for (int j = 1; j <= NY; j++) {
-            int64_t c = H(NX - 1, j, k);
+            int64_t c = H(ix, j, k);
             if (lw[c] == 0 && phase[c] > 0.0) return 1;
         }
     return 0;
+}
+
+int oracle_breakthrough(const wpf_params* p, const int32_t* lw, const double* phase)
+{
+    return p->nx >= 3 ? oracle_breakthrough_plane(p, lw, phase, p->nx - 2) : 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -208,9 +208,15 @@ def test_pressure_solve_long_rows(gpu, solver, flags):
     info = g.solvePressure(solver_opts(solver, rtol=TIGHT, maxit=20000, flags=flags))
     assert info["converged"] == 1
     st.calc_permeability()
+    pg = g.pressure
+    # the oracle's residual of the GPU's answer meets the stopping test
+    assert st.pressure_residual(pg) <= 1.01 * TIGHT * info["bnorm"]
     st.pressure[...] = p0 * 0.5
     st.solve_pressure(rtol=TIGHT, maxit=200000)
-    assert np.abs(g.pressure - st.pressure).max() <= 1e-10 * np.abs(st.pressure).max()
+    # a 3600-cell row makes this system's condition number about 1e8
+    # (nx^2 times the permeability contrast), so two answers that both meet
+    # rtol 1e-13 can differ by far more than 1e-10; measured 1.5e-10
+    assert np.abs(pg - st.pressure).max() <= 1e-9 * np.abs(st.pressure).max()
     g.close()
 
 
@@ -264,36 +270,235 @@ def test_fused_step_matches_procedures(gpu):
 PROJ = _lib.FLAG_PROJECT | _lib.FLAG_EXTRAPOLATE
 
 
-@pytest.mark.parametrize("solver,flags", [("mgcg", 0), ("mixed", 0), ("mixed", CHEB | OP16),
-                                          ("mixed", CHEB | OP16 | PROJ)])
-def test_run_parity_small_workload(gpu, solver, flags):
-    """64x32x32 workload, 60 steps from the loaded state, both sides solving
-    tightly: every field within 1e-10 relative, identical dt history and the
-    same breakthrough step at plane ix_g = 2 (the first plane past the inlet,
-    which the front crosses within the run)."""
-    # 1e-14 sits just above the FP64 floor of the true residual (about 5e-15
-    # relative for this system); solving both sides there keeps the solver
-    # difference well under the 1e-10 bar after 60 coupled steps
-    tight = 1e-14
+# ---------------------------------------------------------------------------
+# the 64x32x32 workload for its full 200 steps, the CPU configuration
+# (CG + Jacobi on the oracle side)
+# ---------------------------------------------------------------------------
+C0_STEPS = 200
+C0_TIGHT = 1e-14   # just above the FP64 floor of the true residual (about
+                   # 5e-15 relative here): solving both sides there keeps the
+                   # solver difference under the 1e-10 bar over the run
+
+
+@pytest.fixture(scope="module")
+def c0_oracle():
+    """The oracle's run of the 64x32x32 workload: dt of every step, the
+    first step with phase > 0 on plane ix_g = 2 (the front crosses it within
+    the run; the outlet side is covered by test_breakthrough_at_default_plane),
+    the final fields and showProgress's reductions."""
     case = io.workload("plain-small", seed=1)
     ref = OracleState(case)
-    g = WPFAW3D(case, opts=solver_opts(solver, rtol=tight, maxit=20000, inner_rtol=1e-4,
+    dts, bt2 = [], 0
+    for _ in range(C0_STEPS):
+        ref.run(1, rtol=C0_TIGHT, maxit=20000)
+        dts.append(ref.dtime)
+        if not bt2 and ref.phase_positive_on_plane(2):
+            bt2 = ref.itime
+    assert bt2 > 0
+    return case, ref, dts, bt2
+
+
+def _diag_dict(d):
+    return {k: getattr(d, k) for k in ("pv", "vcal", "rwhmax", "phase_max", "phase_min",
+                                       "concent_max", "concent_min", "pinj")}
+
+
+@pytest.mark.parametrize("solver,flags", [("mgcg", 0), ("mixed", 0), ("mixed", CHEB | OP16),
+                                          ("mixed", CHEB | OP16 | PROJ)])
+def test_run_parity_full_c0(gpu, c0_oracle, solver, flags):
+    """200 steps from the loaded state, both sides solving tightly: every
+    field within 1e-10 relative, dt of every step within 1e-8, the same
+    step at which the front crosses plane 2, and showProgress's reductions
+    (wPFAW3D.f90:1695, 1756-1810) of the final state: pv and vcal within
+    1e-10 relative, phase/concent extremes and rwhmax within 1e-10 of the
+    field scale, pinj within 1e-10 of the pressure scale."""
+    case, ref, dts, bt2 = c0_oracle
+    g = WPFAW3D(case, opts=solver_opts(solver, rtol=C0_TIGHT, maxit=20000, inner_rtol=1e-4,
                                        flags=flags))
     g.set_breakthrough_plane(2)
-    bt_ref = 0
-    nsteps = 60
-    for s in range(nsteps):
+    for s in range(C0_STEPS):
         g.step()
-        ref.run(1, rtol=tight, maxit=20000)
-        if not bt_ref and (ref.phase[2, 1:-1, 1:-1][ref.lw[2, 1:-1, 1:-1] == 0] > 0).any():
-            bt_ref = ref.itime
         # dt carries taumin, i.e. face velocities, i.e. pressure differences
-        # between neighbours, which hold the solves' 1e-13 tolerance
-        assert abs(g.dtime - ref.dtime) <= 1e-8 * ref.dtime, (s, g.dtime, ref.dtime)
+        # between neighbours, which hold the solves' tolerance
+        assert abs(g.dtime - dts[s]) <= 1e-8 * dts[s], (s, g.dtime, dts[s])
+    err = {}
     for name in ("pressure", "phase", "concent"):
         a, b = _interior(getattr(g, name)), _interior(getattr(ref, name))
-        err = np.abs(a - b).max() / np.abs(b).max()
-        assert err <= 1e-10, (name, err)
+        err[name] = float(np.abs(a - b).max() / np.abs(b).max())
+    print(f"{solver} flags {flags}: relative errors after {C0_STEPS} steps {err}", flush=True)
+    for name, e in err.items():
+        assert e <= 1e-10, (name, e)
+    assert g.breakthrough_step() == bt2
+    dg, dr = _diag_dict(g.showProgress()), _diag_dict(ref.diagnostics())
+    for k in ("pv", "vcal"):
+        assert dg[k] == pytest.approx(dr[k], rel=1e-10), k
+    for k in ("rwhmax", "phase_max", "phase_min", "concent_max", "concent_min"):
+        assert abs(dg[k] - dr[k]) <= 1e-10 * max(abs(dr[k]), 1.0), (k, dg[k], dr[k])
+    assert abs(dg["pinj"] - dr["pinj"]) <= 1e-10 * np.abs(ref.pressure).max()
+    g.close()
+
+
+def test_diagnostics_match_oracle_on_crafted_fields(gpu):
+    """wpf_diagnostics against the oracle on identical uploaded fields: the
+    sums pv and vcal within 1e-12 relative (block partial sums against a
+    sequential sum), the extremes and rwhmax bitwise (max/min of the same
+    values), pinj exactly (a copy of one value).  The fields mix phase > 0
+    cells all over the grid, values outside [-1, 1], and a concentration
+    with both signs."""
+    case = io.workload("plain-small", seed=5, grid=(40, 24, 20))
+    p = case.params
+    rng = np.random.default_rng(11)
+    sh = (p.nx + 2, p.ny + 2, p.nz + 2)
+    phase = np.asfortranarray(rng.uniform(-1.3, 0.9, sh))
+    conc = np.asfortranarray(rng.uniform(-1.1, 0.2, sh))
+    pres = np.asfortranarray(rng.uniform(0.0, 1e5, sh))
+    for a in (phase, conc, pres):
+        io.clamp_halo(a)
+    g = WPFAW3D(case)
+    g.upload("phase", phase)
+    g.upload("concent", conc)
+    g.upload("pressure", pres)
+    st = OracleState(case)
+    st.phase[...], st.concent[...], st.pressure[...] = phase, conc, pres
+    dg, dr = _diag_dict(g.showProgress()), _diag_dict(st.diagnostics())
+    assert dr["rwhmax"] > 0
+    for k in ("pv", "vcal"):
+        assert dg[k] == pytest.approx(dr[k], rel=1e-12), k
+    for k in ("rwhmax", "phase_max", "phase_min", "concent_max", "concent_min", "pinj"):
+        assert dg[k] == dr[k], (k, dg[k], dr[k])
+    g.close()
+
+
+def _overshoot_state(grid=(48, 20, 16), seed=9):
+    """A workload state with the phase above 1 and steep interfaces: phase =
+    1.6 tanh(...) of a rough field plus noise, kept >= -1, so that phix,
+    phiy or phiz exceeds 1/dx at many cells and the signed gates of
+    calcPhaseDeriv (wPFAW3D.f90:1330-1377) open.  (Below -1 the node
+    permeability kwh f + k_rock (1 - f), f = (1 + phase)/2, turns negative
+    within a few percent, and the pressure operator stops being definite.)"""
+    case, st = _state(grid=grid, steps=3, seed=seed)
+    p = case.params
+    rng = np.random.default_rng(seed)
+    x = np.arange(p.nx + 2)[:, None, None]
+    y = np.arange(p.ny + 2)[None, :, None]
+    z = np.arange(p.nz + 2)[None, None, :]
+    u = 0.9 * (x - p.nx / 2) + 0.5 * np.sin(y / 2.3) * 3 + 0.4 * (z - p.nz / 2) + 0.8 * np.cos(x / 3.1 + z / 2.7)
+    ph = np.maximum(1.6 * np.tanh(u / 1.5) + 0.25 * rng.standard_normal(u.shape), -1.0)
+    st.phase[...] = np.asfortranarray(ph)
+    io.clamp_halo(st.phase)
+    # wells keep their own phase, as the reference never updates them
+    wells = st.lw > 0
+    st.phase[wells] = case.phase[wells]
+    return case, st
+
+
+def _gate_count(st, p):
+    ph = st.phase
+    gx = (ph[2:, 1:-1, 1:-1] - ph[:-2, 1:-1, 1:-1]) / (2 * p.dx)
+    gy = (ph[1:-1, 2:, 1:-1] - ph[1:-1, :-2, 1:-1]) / (2 * p.dx)
+    gz = (ph[1:-1, 1:-1, 2:] - ph[1:-1, 1:-1, :-2]) / (2 * p.dx)
+    free = st.lw[1:-1, 1:-1, 1:-1] == 0
+    m = 1.0 / p.dx
+    return int((((gx > m) | (gy > m)) & free).sum()), int((((gx > m) | (gy > m) | (gz > m)) & free).sum())
+
+
+def test_phase_deriv_overshoot_branch(gpu):
+    """calcPhaseDeriv where |phase| > 1 and the gradient exceeds 1/dx, so the
+    gated terms f2 and f4..f6 of wPFAW3D.f90:1330-1414 are live: within
+    1e-12 of max |dphi/dt| of the oracle, taumin within 1e-14, and the same
+    exact zeros."""
+    case, st = _overshoot_state()
+    p = case.params
+    nt, np_ = _gate_count(st, p)
+    assert nt > 100 and np_ > 100, (nt, np_)
+    g = _gpu_from(case, st)
+    g.calcPermeability()
+    g.calcVelocity()
+    tau_gpu = g.calcPhaseDeriv()
+    st.calc_permeability()
+    st.calc_velocity()
+    tau_ref = st.calc_phase_deriv()
+    assert abs(tau_gpu - tau_ref) <= 1e-14 * abs(tau_ref)
+    dph, dph_ref = _interior(g.dphasedt), _interior(st.dphasedt)
+    scale = np.abs(dph_ref).max()
+    err = np.abs(dph - dph_ref)
+    print(f"overshoot branch: {nt} cells with the theta gate open, {np_} with the phi gate; "
+          f"max |dphi/dt| {scale:.3e}, max error {err.max():.3e}", flush=True)
+    assert err.max() <= 1e-12 * scale
+    assert np.array_equal(dph == 0, dph_ref == 0)
+    g.close()
+
+
+def test_fused_step_matches_procedures_in_overshoot(gpu):
+    """The fused step (k_step_fields) on the overshoot state is bitwise the
+    per-procedure path, whose calcPhaseDeriv the test above pins to the
+    oracle."""
+    case, st = _overshoot_state(seed=4)
+    a = _gpu_from(case, st)
+    b = _gpu_from(case, st)
+    for g in (a, b):
+        g.itime, g.time, g.dtime, g.taumin = st.itime, st.time, st.dtime, st.taumin
+    for _ in range(2):
+        a.step()
+        b.step_procedures()
+    assert a.dtime == b.dtime and a.taumin == b.taumin
+    assert np.array_equal(a.phase, b.phase)
+    assert np.array_equal(a.concent, b.concent)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("solver,flags,rtol", [("mgcg", 0, 1e-14),
+                                               ("mixed", CHEB | OP16 | PROJ, 1e-10)])
+def test_breakthrough_at_default_plane(gpu, solver, flags, rtol):
+    """A run to breakthrough at the default plane ix_g = nx-2 (16x8x8: the
+    oracle's front crosses it at step 1926) gives the oracle's step, with
+    the plain FP64 path solving tightly and with the fast mixed path at the
+    workload's rtol 1e-10."""
+    case = io.workload("plain-small", seed=1, grid=(16, 8, 8))
+    ref = OracleState(case)
+    bt_ref = 0
+    while not bt_ref and ref.itime < 4000:
+        bt_ref = ref.run(100, rtol=1e-14, maxit=20000)
     assert bt_ref > 0
-    assert g.breakthrough_step() == bt_ref
+    g = WPFAW3D(case, opts=solver_opts(solver, rtol=rtol, maxit=20000, inner_rtol=1e-4, flags=flags))
+    step = 0
+    while step < bt_ref + 5:
+        rec = g.step()
+        assert rec["converged"] == 1
+        step += 1
+        if step == bt_ref - 1:
+            assert g.breakthrough_step() == 0
+    assert g.breakthrough_step() == bt_ref, (g.breakthrough_step(), bt_ref)
+    g.close()
+
+
+def test_block_count_above_the_old_reduction_limit(gpu):
+    """A grid whose per-procedure launches take 300,000 blocks of 64 x 4
+    cells (2 x 20 x 60000: one x tile, five y tiles), more than the 262,144
+    block minima the reduction buffer once held: calcPhaseDeriv's taumin and
+    showProgress's reductions against the oracle."""
+    p = io.physics_params(2, 20, 60000)
+    kr = np.exp(np.random.default_rng(4).uniform(-33.0, -29.0, (p.nx, p.ny, p.nz)))
+    case = io.build_case(p, io.inlet_outlet_wells(p), kr)
+    rng = np.random.default_rng(2)
+    sh = (p.nx + 2, p.ny + 2, p.nz + 2)
+    pres = np.asfortranarray(rng.uniform(0.0, 1e5, sh))
+    io.clamp_halo(pres)
+    g = WPFAW3D(case)
+    g.upload("pressure", pres)
+    st = OracleState(case)
+    st.pressure[...] = pres
+    g.calcPermeability()
+    g.calcVelocity()
+    tau_gpu = g.calcPhaseDeriv()
+    st.calc_permeability()
+    st.calc_velocity()
+    tau_ref = st.calc_phase_deriv()
+    assert abs(tau_gpu - tau_ref) <= 1e-14 * abs(tau_ref)
+    dg, dr = _diag_dict(g.showProgress()), _diag_dict(st.diagnostics())
+    for k in ("pv", "vcal"):
+        assert dg[k] == pytest.approx(dr[k], rel=1e-12), k
+    for k in ("phase_max", "phase_min", "concent_max", "concent_min", "pinj"):
+        assert dg[k] == dr[k], k
     g.close()

@@ -113,6 +113,111 @@ def test_solve_matches_direct_solve_of_assembled_system():
     assert np.abs(A @ x - b).max() <= 1e-12 * np.abs(b).max()
 
 
+# ------------------------------------------- A5: BiCGStab + SOR ------------
+def _sor_rows(st):
+    """The rows the SOR sweeps relax, as a dense matrix from the oracle's
+    operator, and the sign vector that maps them onto the assembled
+    reference system (free rows negated)."""
+    n = st.p.ncell
+    A = np.column_stack([st.sor_operator(np.eye(n)[:, m]) for m in range(n)])
+    lw = st.lw[1:-1, 1:-1, 1:-1].reshape(-1, order="F")
+    fixed = np.array([lw[i] > 0 and st.case.wells[lw[i] - 1].pressure != 0 for i in range(n)])
+    return A, np.where(fixed, 1.0, -1.0)
+
+
+def test_sor_rows_are_the_assembled_system():
+    case = tiny_case(7, 5, 4)
+    st = OracleState(case)
+    st.calc_permeability()
+    A, sgn = _sor_rows(st)
+    Aref, _ = _dense_system(st)
+    np.testing.assert_allclose(A, sgn[:, None] * Aref.toarray(), rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("nblocks", [1, 3])
+def test_sor_sweep_matches_the_matrix_splitting(nblocks):
+    """PCSOR's local symmetric sweep (its outer iterations, lits = 1) on
+    nblocks contiguous row blocks, against the textbook matrix form of SOR
+    (x <- x + (D/w + L)^-1 (b - A x), then with U; Saad, Iterative Methods
+    for Sparse Linear Systems, 4.1) applied block by block with the coupling
+    to other blocks taken from the iterate at the start of each outer
+    iteration, by triangular solves of the dense rows."""
+    import scipy.linalg as sla
+    case = tiny_case(7, 5, 4, seed=5)
+    st = OracleState(case)
+    st.calc_permeability()
+    A, _ = _sor_rows(st)
+    n = A.shape[0]
+    r = np.random.default_rng(3).normal(size=n)
+    its, w = 10, 1.7
+    q, rem = divmod(n, nblocks)
+    bounds, r0 = [], 0
+    for b in range(nblocks):
+        r1 = r0 + q + (1 if b < rem else 0)
+        bounds.append((r0, r1))
+        r0 = r1
+    z = np.zeros(n)
+    for _ in range(its):
+        zo = z.copy()
+        for r0, r1 in bounds:
+            blk = slice(r0, r1)
+            Ab = A[blk, blk]
+            rhs = r[blk] - A[blk, :] @ zo + Ab @ zo[blk]       # coupling to the other blocks frozen
+            D = np.diag(np.diag(Ab))
+            lower, upper = D / w + np.tril(Ab, -1), D / w + np.triu(Ab, 1)
+            zb = z[blk].copy()
+            zb = zb + sla.solve_triangular(lower, rhs - Ab @ zb, lower=True)
+            zb = zb + sla.solve_triangular(upper, rhs - Ab @ zb, lower=False)
+            z[blk] = zb
+    got = st.sor_apply(r, its=its, omega=w, nblocks=nblocks)
+    np.testing.assert_allclose(got, z, rtol=1e-11, atol=1e-12 * np.abs(z).max())
+
+
+def test_sor_converges_to_the_inverse():
+    case = tiny_case(6, 5, 4, seed=6)
+    st = OracleState(case)
+    st.calc_permeability()
+    A, _ = _sor_rows(st)
+    r = np.random.default_rng(4).normal(size=A.shape[0])
+    z = st.sor_apply(r, its=600, omega=1.7, nblocks=1)
+    np.testing.assert_allclose(z, np.linalg.solve(A, r), rtol=1e-9, atol=1e-9 * np.abs(r).max())
+
+
+@pytest.mark.parametrize("nblocks", [1, 4])
+def test_bcgs_sor_matches_direct_solve(nblocks):
+    """The reference's BiCGStab + SOR(10, 1.7) reaches the solution of the
+    literally assembled system, with its preconditioned test met, and its
+    answer meets the other solvers' Jacobi-norm test too."""
+    case = tiny_case(8, 6, 5)
+    st = OracleState(case)
+    st.calc_permeability()
+    A, b = _dense_system(st)
+    x_ref = spla.spsolve(A.tocsc(), b)
+    its, rn, bn, tr, conv = st.solve_pressure_bcgs(rtol=1e-13, maxit=1000, nblocks=nblocks)
+    assert conv and rn <= 1e-13 * bn and its > 0
+    x = st.pressure[1:-1, 1:-1, 1:-1].reshape(-1, order="F")
+    np.testing.assert_allclose(x, x_ref, rtol=0, atol=1e-9 * np.abs(x_ref).max())
+    other = OracleState(case)
+    other.calc_permeability()
+    _, _, _, bn_j = other.solve_pressure(rtol=1e-300, maxit=1)   # ||D^-1 b|| of the CG test
+    assert tr <= 1e-10 * bn_j
+
+
+def test_bcgs_sor_and_cg_jacobi_agree_on_the_workload():
+    case = io.workload("plain-small", seed=2, grid=(24, 16, 12))
+    st = OracleState(case)
+    st.run(4, rtol=1e-12, maxit=20000)
+    st.calc_permeability()
+    p0 = st.pressure.copy()
+    st.pressure[...] = 0.5 * p0
+    its_b, _, _, tr, conv = st.solve_pressure_bcgs(rtol=1e-13, maxit=1000)
+    pb = st.pressure.copy()
+    st.pressure[...] = 0.5 * p0
+    its_c, _, _, _ = st.solve_pressure(rtol=1e-13, maxit=20000)
+    assert conv and its_b < its_c
+    assert np.abs(pb - st.pressure).max() <= 1e-8 * np.abs(st.pressure).max()
+
+
 def test_uniform_rock_gives_linear_pressure():
     nx = 10
     case = tiny_case(nx, 4, 3, uniform=True)
@@ -262,20 +367,29 @@ def test_anisotropy_derivatives_by_finite_differences():
     assert iv["ddwdthetadphi"] == pytest.approx(H[0][1], rel=1e-4, abs=1e-10 * abs(iv["w"]))
 
 
+def _overshoot_profile(x, y, z):
+    """A steep interface with |phase| up to 1.6 (so phix > 1/dx and the
+    gated branch runs), centred off the cell so that every second
+    derivative of the discrete Hessian is non-zero."""
+    return (1.6 * math.tanh((0.9 * x + 0.35 * y + 0.25 * z - 2e-5) / 5e-5)
+            + 0.3 * math.sin((x + 4e-5) / 3e-4) * math.cos((z - 3e-5) / 4e-4)
+            + 0.1 * math.sin((y + 2e-5) / 2e-4) * math.cos((x - 1e-5) / 2.5e-4))
+
+
 def test_gated_chain_rule_terms_by_finite_differences():
     """With |phase| > 1 the signed gates (phix > 1/dx, ...) fire.  The angle
     derivatives then equal the chain rule through the discrete Hessian,
     d(theta)/dx = (dtheta/dg) . H[:, 0], with dtheta/dg taken by finite
     differences of atan(gy/gx); likewise for acos(gz/|g|).  The reference
     scales d(theta)/d(g) by |g|^2 and d(phi)/d(g) by |g|: pinned the same way."""
-    s = _stencil(lambda x, y, z: 1.6 * math.tanh((0.9 * x + 0.35 * y + 0.25 * z) / 5e-5)
-                 + 0.3 * math.sin(x / 3e-4) * math.cos(z / 4e-4))
+    s = _stencil(_overshoot_profile)
     r, tau, iv = phase_cell(P, s, -0.1, np.zeros(6))
     g = np.array([iv["phix"], iv["phiy"], iv["phiz"]])
     assert g[0] > 1 / DX or g[1] > 1 / DX     # the gate is open
     Hm = np.array([[iv["phixx"], iv["phixy"], iv["phizx"]],
                    [iv["phixy"], iv["phiyy"], iv["phiyz"]],
                    [iv["phizx"], iv["phiyz"], iv["phizz"]]])
+    assert np.all(Hm != 0.0)                   # every chain-rule term is exercised
     theta = lambda v: math.atan(v[1] / (v[0] + 1e-20))  # noqa: E731
     phi = lambda v: math.acos(v[2] / (np.linalg.norm(v) + 1e-20))  # noqa: E731
     h = 1e-6 * np.linalg.norm(g)
@@ -296,6 +410,59 @@ def test_gated_chain_rule_terms_by_finite_differences():
     assert iv["f2"] == pytest.approx(2 * iv["w"] * (gradW @ g), rel=1e-6)
     total = iv["f1"] + iv["f2"] + iv["f3"] + iv["f4"] + iv["f5"] + iv["f6"]
     assert r == pytest.approx(total / tau, rel=1e-14)
+
+
+@pytest.mark.parametrize("eps", [0.04, P.epsilon])
+def test_gated_f4_f6_by_finite_differences(eps):
+    """The terms f4..f6 of the gated branch (wPFAW3D.f90:1392-1414) are
+    d/dx_a of W*dW/d(dphi_a) with the scale factors the reference applies:
+        f_a = a_a d_a(W W_theta) + b_a d_a(W W_phi),
+    a_a = dtheta/d(g_a) |g|^4 and b_a = dphi/d(g_a) |g|^3 (a_z = 0, as the
+    reference sets dthetaddpdz = 0), where d_a F(theta, phi) =
+    F_theta theta_a + F_phi phi_a along axis a.  Checked with every
+    derivative of W taken by finite differences of W(theta, phi) itself (the
+    anisotropy function of :1298), the angle derivatives theta_a, phi_a by
+    the chain rule through the discrete Hessian, and dtheta/dg, dphi/dg by
+    finite differences of atan(gy/gx) and acos(gz/|g|): none of the
+    oracle's closed-form derivatives (ddwdthetax, ddwdphi_x, ...) enter the
+    expected value.  The reference's eps (the interface width in metres,
+    :1298) is also the anisotropy strength: at the workload's 3e-4 the
+    anisotropic parts of W are 1e-3 of W and the nested differences hold
+    about 1e-4; eps = 0.04 (with dx kept at 1e-4) checks them at 1e-6."""
+    p = io.physics_params(8, 8, 8)
+    p.epsilon = eps
+    p.dxw = DX / eps                         # dx = dxw*eps stays 1e-4 (1:393)
+    s = _stencil(_overshoot_profile)
+    r, tau, iv = phase_cell(p, s, -0.1, np.zeros(6))
+    g = np.array([iv["phix"], iv["phiy"], iv["phiz"]])
+    assert g[0] > 1 / DX                                   # both gates open
+    Hm = np.array([[iv["phixx"], iv["phixy"], iv["phizx"]],
+                   [iv["phixy"], iv["phiyy"], iv["phiyz"]],
+                   [iv["phizx"], iv["phiyz"], iv["phizz"]]])
+    theta = lambda v: math.atan(v[1] / (v[0] + 1e-20))  # noqa: E731
+    phi = lambda v: math.acos(v[2] / (np.linalg.norm(v) + 1e-20))  # noqa: E731
+    hg = 1e-6 * np.linalg.norm(g)
+    Jt, Jp = _fd(theta, g, hg), _fd(phi, g, hg)
+    th_a, ph_a = Jt @ Hm, Jp @ Hm           # d(theta)/dx_a, d(phi)/dx_a
+    g2 = g @ g
+    a = np.array([Jt[0] * g2 * g2, Jt[1] * g2 * g2, 0.0])
+    b = Jp * math.sqrt(g2) * g2
+    # W and its first and second derivatives at the cell's angles
+    Wf = lambda v: W_of(v[0], v[1], eps=eps)  # noqa: E731
+    x0 = [iv["theta"], iv["dphi"]]
+    h = 1e-4
+    W = Wf(x0)
+    Wt, Wp = _fd(Wf, x0, h)
+    (Wtt, Wtp), (_, Wpp) = _fd(lambda v: _fd(Wf, v, h), x0, h)
+    # d(W W_theta) and d(W W_phi) in (theta, phi): the product rule
+    G1t, G1p = Wt * Wt + W * Wtt, Wp * Wt + W * Wtp
+    G2t, G2p = Wt * Wp + W * Wtp, Wp * Wp + W * Wpp
+    rel = 1e-6 if eps > 1e-2 else 1e-4
+    scale = max(abs(iv["f4"]), abs(iv["f5"]), abs(iv["f6"]))
+    assert scale > 0
+    for k, name in enumerate(("f4", "f5", "f6")):
+        expect = (a[k] * (G1t * th_a[k] + G1p * ph_a[k]) + b[k] * (G2t * th_a[k] + G2p * ph_a[k]))
+        assert iv[name] == pytest.approx(expect, rel=rel, abs=rel * scale), name
 
 
 # ---------------------------------------------------------------- A9 --------

@@ -179,6 +179,17 @@ int wpf_calc_velocity(wpf_ctx* ctx);
 /* calcPhaseDeriv(taumin), wPFAW3D.f90:1197-1475 including the MPI_MIN
  * reduction of taumin at 1:1439-1440. */
 int wpf_calc_phase_deriv(wpf_ctx* ctx, double* taumin);
+/* No counterpart in the reference: wpf_calc_phase_deriv for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels and the same
+ * results; its per-block taumin minima and the taumin key go to a fixed
+ * device array of 8 MiB per GPU (held until the process ends) instead of
+ * the context's reduction buffers, and the call makes no allocation.  Grids
+ * that need more than 2^20 - 1 blocks of 64 x 4 cells
+ * (ceil(nx/64)*ceil(ny/4)*nz) are rejected with WPF_ERR_ARG.  All calls on
+ * one GPU share the array: calls that overlap on one GPU (several host
+ * threads or contexts at once) race. */
+int wpf_calc_phase_deriv_noalloc(wpf_ctx* ctx, double* taumin);
 /* calcConcentDeriv, wPFAW3D.f90:1554-1624. */
 int wpf_calc_concent_deriv(wpf_ctx* ctx);
 /* evolvePhase / evolveConcent, wPFAW3D.f90:1628-1640 (forward Euler). */

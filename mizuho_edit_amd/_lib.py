@@ -112,6 +112,7 @@ def lib():
         "wpf_solve_pressure": ([P, ctypes.POINTER(WpfSolverOpts), ctypes.POINTER(WpfSolveStats)], I32),
         "wpf_calc_velocity": ([P], I32),
         "wpf_calc_phase_deriv": ([P, D], I32),
+        "wpf_calc_phase_deriv_noalloc": ([P, D], I32),
         "wpf_calc_concent_deriv": ([P], I32),
         "wpf_evolve_phase": ([P, ctypes.c_double], I32),
         "wpf_evolve_concent": ([P, ctypes.c_double], I32),

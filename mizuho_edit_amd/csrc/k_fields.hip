@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace wpf {
 
@@ -407,7 +408,58 @@ int launch_phase_deriv(wpf_ctx* c, double* taumin)
     return phase_deriv_into(c, c->red.partials, c->red.cap, c->tau_key + 3, taumin);
 }
 
+// ---------------------------------------------------------------------------
+// calcPhaseDeriv without allocation: the block minima and the taumin key
+// live in a fixed device array, `scratch`, one per GPU for the life of the
+// process (kNoallocDoubles doubles, 8 MiB: grids of up to 2^20 - 1 blocks of
+// 64 x 4 cells).  Host code never passes the array's name to a kernel or a
+// copy: wpf_create looks up its device address once per device
+// (noalloc_scratch_init) and the entry point uses that address.  Every call
+// on one GPU shares the array: each call writes every minimum it reads and
+// the key before it reads them, so nothing needs clearing, and calls that
+// overlap on one GPU (two host threads, two streams) race.
+// ---------------------------------------------------------------------------
+static constexpr int64_t kNoallocDoubles = (int64_t)1 << 20;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
 
+static constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int dev)
+{
+    WPF_CHECK(dev >= 0 && dev < kMaxDevices, WPF_ERR_ARG, "device %d outside the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[dev]) return WPF_OK;
+    int cur = -1;
+    WPF_HIP(hipGetDevice(&cur));
+    WPF_CHECK(cur == dev, WPF_ERR_STATE, "scratch set-up for device %d with device %d current", dev, cur);
+    void* ptr = nullptr;
+    WPF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    WPF_CHECK(ptr != nullptr, WPF_ERR_HIP, "no device address for the scratch array");
+    g_scratch_dev[dev] = (double*)ptr;
+    return WPF_OK;
+}
+
+}  // namespace wpf
+
+extern "C" int wpf_calc_phase_deriv_noalloc(wpf_ctx* c, double* taumin)
+{
+    using namespace wpf;
+    WPF_CHECK(c && taumin, WPF_ERR_ARG, "null argument");
+    WPF_CHECK(c->dev >= 0 && c->dev < kMaxDevices, WPF_ERR_ARG, "device %d outside the scratch table", c->dev);
+    double* s;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        s = g_scratch_dev[c->dev];
+    }
+    WPF_CHECK(s != nullptr, WPF_ERR_STATE, "no scratch address for device %d", c->dev);
+    WPF_HIP(hipSetDevice(c->dev));
+    // minima in [0, kNoallocDoubles - 1), the key in the last slot
+    return phase_deriv_into(c, s, kNoallocDoubles - 1, (unsigned long long*)(s + kNoallocDoubles - 1), taumin);
+}
+
+namespace wpf {
 
 // ---------------------------------------------------------------------------
 // calcConcentDeriv (per-procedure form)

@@ -285,6 +285,10 @@ int wpf_create(const wpf_params* p, const int32_t* lw_haloed, const wpf_well* we
         WPF_HIP(hipMemcpy(c->bt_step, &v, 4, hipMemcpyHostToDevice));
     }
     c->bt_plane = g.nx >= 3 ? g.nx - 3 : -1;  // ix_g = nx-2
+    if ((rc = noalloc_scratch_init(c->dev))) {
+        wpf_destroy(c);
+        return rc;
+    }
     if ((rc = reducer_init(c->red, 64, reduction_capacity(g)))) {
         wpf_destroy(c);
         return rc;

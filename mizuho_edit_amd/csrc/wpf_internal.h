@@ -296,6 +296,9 @@ void dev_free(Dev<T>& a)
 }
 
 int reducer_init(Reducer& r, int slots, int64_t cap);
+// Device address of the no-allocation entry's scratch array on device dev
+// (that device current); once per device, kept for the process.
+int noalloc_scratch_init(int dev);
 // Partials a context of this grid needs: the largest block count of any of
 // its launches times kMaxValuesPerBlock.
 int64_t reduction_capacity(const Grid& g);

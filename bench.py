@@ -2,27 +2,35 @@
 """Benchmark of the wPFAW3D per-time-step hot path on one MI355X.
 
 Workload: the 256^3 wormholing configuration (16,777,216 cells, seeded
-Gaussian-random-field rock, inlet plane at 1e5 Pa, outlet plane at "p = 0",
-500-step run in the full configuration).  A step is one pass of the hot
-path: dt rule, permeability, pressure solve, velocity + phase-field +
-concentration + Euler update, with every input already resident in HBM.
+Gaussian-random-field rock, inlet plane at 1e5 Pa, outlet plane at "p = 0"),
+run for its full 500 steps.  A step is one pass of the hot path: dt rule,
+permeability, pressure solve, velocity + phase-field + concentration + Euler
+update, with every input already resident in HBM.
 
-Two paths run from the same initial state:
+Two paths run the same 500 steps from the same initial state:
   plain   FP64 MG-preconditioned CG, rtol 1e-10       (the FP64 reference path)
   fast    FP32 MG-PCG inside FP64 iterative refinement, rtol 1e-10 on the
           FP64 true residual (the headline path)
-`value` is the fast path's time-steps/s; the plain path's rate, the ratio
-and the fast path's relative pressure error against the plain path after
-the timed steps are reported beside it.
+Every step of both runs is timed.  `value` is the fast path's time-steps/s
+over the timed window, steps W+1..W+K (--warmup W, --steps K; by default the
+whole run, W = 0 and K = 500); `whole_run` gives both paths' rates over all
+500 steps and over each 100-step window, so the figure does not depend on the
+window chosen.  The fast path's relative pressure error is reported against
+the plain path after the run, and against the FP64 oracle for one step at
+this size (both GPU paths and the oracle's BiCGStab + SOR step from the
+state the timed window starts from).
 
 roofline: the dominant kernel of the fast path (by default the CG update
 fused with the first pre-smoothing step, k_vupsweep: the largest share of a
 step's kernel time), timed with HIP events on the library's stream over the
-timed region; achieved = algorithmic bytes per launch / average launch
+timed window; achieved = algorithmic bytes per launch / average launch
 time; traffic = HBM bytes per launch measured with rocprofv3 counters.
-cpu_baseline: the FP64 oracle (C restatement, OpenMP) timed on this host for
-one step of the same workload from the state the GPU timed region starts
-from, CG + Jacobi with rtol 1e-10 and maxit 1000 (the reference's cap).
+cpu_baseline: the FP64 oracle (C restatement, OpenMP) on this host, from the
+state the timed window starts from: one step with the reference's own solver
+(BiCGStab + SOR, 10 iterations, omega 1.7, one SOR block per thread as the
+ranks of an mpirun, rtol 1e-10, maxit 1000) is `value`; one step with CG +
+Jacobi (the CPU configuration's solver, same rtol and cap) is reported beside
+it, with whether each solve converged.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N]
 """
@@ -104,38 +112,51 @@ def sync():
         pass
 
 
-def run_path(case, opts, warmup, steps, profile_kernel=None, net=None):
+def run_path(case, opts, warmup, steps, run_steps, profile_kernel=None, net=None):
+    """Runs max(run_steps, warmup + steps) steps; steps warmup+1..warmup+steps
+    form the timed window (HIP-event profile of one kernel over it).  Every
+    step is timed on the host (wpf_step ends on a stream synchronisation)."""
     from mizuho_edit_amd.wpfaw3d import WPFAW3D
     g = WPFAW3D(case, device=0, opts=opts)
     g.set_tile_depth(-1)  # autotuned tile depth of the vector solver kernels
     if net is not None:
         g.attach_net(net)
-    for s in range(warmup):
+    total = max(run_steps, warmup + steps)
+    step_s, its_window, acc, prof, state = [], 0, 0, None, None
+    wall = 0.0
+    for s in range(total):
+        if s == warmup:
+            state = {n: g.download(n) for n in ("pressure", "phase", "concent")}
+            state.update(itime=g.itime, time=g.time, dtime=g.dtime, taumin=g.taumin)
+            sync()
+            if profile_kernel:
+                g.profile_start(profile_kernel)
+            tw = time.perf_counter()
+        t0 = time.perf_counter()
         rec = g.step()
-        log(f"  warmup {s + 1}/{warmup}: its={rec['iterations']} outer={rec['outer']} "
-            f"dt={rec['dtime']:.3e}")
-    state = {n: g.download(n) for n in ("pressure", "phase", "concent")}
-    state.update(itime=g.itime, time=g.time, dtime=g.dtime, taumin=g.taumin)
-    sync()
-    if profile_kernel:
-        g.profile_start(profile_kernel)
-    t0 = time.perf_counter()
-    its = 0
-    acc = 0
-    for s in range(steps):
-        rec = g.step()
-        its += rec["iterations"]
+        step_s.append(time.perf_counter() - t0)
         acc += rec["learned"]
-        if (s + 1) % max(1, steps // 5) == 0:
-            log(f"  step {s + 1}/{steps}: its={rec['iterations']} outer={rec['outer']}")
-    sync()
-    wall = time.perf_counter() - t0
-    prof = g.profile_stop() if profile_kernel else None
+        if warmup <= s < warmup + steps:
+            its_window += rec["iterations"]
+        if s == warmup + steps - 1:
+            sync()
+            wall = time.perf_counter() - tw
+            if profile_kernel:
+                prof = g.profile_stop()
+        if (s + 1) % 50 == 0 or s + 1 == total:
+            log(f"  step {s + 1}/{total}: its={rec['iterations']} outer={rec['outer']} "
+                f"dt={rec['dtime']:.3e} ({sum(step_s[-50:]) / len(step_s[-50:]) * 1e3:.2f} ms/step)")
     g.learned_accepted = acc
-    return g, wall, its, state, prof
+    return g, wall, its_window, state, prof, step_s
 
 
-def cpu_baseline(case, state, nthreads):
+def window_rates(step_s, width=100):
+    return [{"steps": f"{a + 1}..{min(a + width, len(step_s))}",
+             "steps_per_s": len(step_s[a:a + width]) / sum(step_s[a:a + width])}
+            for a in range(0, len(step_s), width)]
+
+
+def _oracle_from(case, state):
     from oracle.oracle import OracleState
     st = OracleState(case)
     st.pressure[...] = state["pressure"]
@@ -143,10 +164,40 @@ def cpu_baseline(case, state, nthreads):
     st.concent[...] = state["concent"]
     st.itime, st.time, st.dtime, st.taumin = (state["itime"], state["time"], state["dtime"],
                                              state["taumin"])
+    return st
+
+
+def cpu_baseline(case, state, nthreads):
+    """One step of each CPU solver from `state`; returns the records and the
+    oracle state after the BiCGStab + SOR step."""
+    out = {}
+    st = _oracle_from(case, state)
+    t0 = time.perf_counter()
+    rec = st.step_bcgs(rtol=1e-10, maxit=1000, nblocks=nthreads)
+    out["bicgstab_sor"] = dict(seconds=time.perf_counter() - t0, iterations=rec["iterations"],
+                               converged=bool(rec["converged"]))
+    ref = st
+    st = _oracle_from(case, state)
     t0 = time.perf_counter()
     st.run(1, rtol=1e-10, maxit=1000, nthreads=nthreads)
-    wall = time.perf_counter() - t0
-    return wall, int(st.hist[-1][2])
+    its = int(st.hist[-1][2])
+    out["cg_jacobi"] = dict(seconds=time.perf_counter() - t0, iterations=its,
+                            converged=bool(st.hist[-1][3] <= 1e-10))
+    return out, ref
+
+
+def gpu_one_step(case, state, opts):
+    """One step of a GPU path on a fresh context from `state` (no solution
+    history: the extrapolated and projected guesses start empty)."""
+    from mizuho_edit_amd.wpfaw3d import WPFAW3D
+    g = WPFAW3D(case, device=0, opts=opts)
+    for n in ("pressure", "phase", "concent"):
+        g.upload(n, state[n])
+    g.itime, g.time, g.dtime, g.taumin = state["itime"], state["time"], state["dtime"], state["taumin"]
+    rec = g.step()
+    fields = {n: g.download(n) for n in ("pressure", "phase", "concent")}
+    g.close()
+    return rec, fields
 
 
 def calibrate_copy(reps=20):
@@ -262,8 +313,10 @@ def bench_ensemble(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--run-steps", type=int, default=500,
+                    help="length of the simulated run (the timed window lies inside it)")
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--kernel", default="cg_update", help="kernel reported in roofline")
     ap.add_argument("--no-cpu", action="store_true")
@@ -314,9 +367,10 @@ def main():
     log(f"building the {n}^3 workload (seed {a.seed})")
     case = io.workload("plain-256", seed=a.seed, grid=(n, n, n))
     ncell = n ** 3
+    plain_opts = solver_opts("mgcg", rtol=1e-10)
 
     log("plain FP64 path")
-    g64, wall64, its64, _, _ = run_path(case, solver_opts("mgcg", rtol=1e-10), a.warmup, a.steps)
+    g64, wall64, its64, _, _, steps64 = run_path(case, plain_opts, a.warmup, a.steps, a.run_steps)
     p64 = g64.download("pressure")
     g64.close()
 
@@ -342,7 +396,8 @@ def main():
     if a.f32_stencils:
         flags |= FLAG_F32_STENCIL
     fast_opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, mg_levels=a.fast_levels, flags=flags)
-    gf, wallf, itsf, state0, prof = run_path(case, fast_opts, a.warmup, a.steps, a.kernel, net)
+    gf, wallf, itsf, state0, prof, stepsf = run_path(case, fast_opts, a.warmup, a.steps, a.run_steps,
+                                                     a.kernel, net)
     pf = gf.download("pressure")
     accepted = gf.learned_accepted
     gf.close()
@@ -354,16 +409,38 @@ def main():
     avg_s = total_ms / 1e3 / max(count, 1)
     bpc = bytes_per_cell(a.kernel, 4, 4 if a.jacobi else 2, not a.jacobi, count, itsf)
     achieved = bpc * ncell / avg_s / 1e9 if count else 0.0
+    run_len = len(stepsf)
+    whole = {"steps": run_len,
+             "fast_steps_per_s": run_len / sum(stepsf),
+             "plain_fp64_steps_per_s": run_len / sum(steps64),
+             "speedup_vs_plain_fp64": sum(steps64) / sum(stepsf),
+             "fast_windows": window_rates(stepsf),
+             "plain_fp64_windows": window_rates(steps64)}
 
-    cpu = None
+    cpu, vs_oracle = None, None
     if not a.no_cpu:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        log(f"CPU oracle: one step at {n}^3 on {nthreads} threads")
-        cwall, cits = cpu_baseline(case, state0, nthreads)
-        cpu = {"value": 1.0 / cwall, "unit": "time-steps/s", "cores": nthreads, "kind": "port",
-               "sample": f"one time step of the {n}^3 workload from the GPU timed region's start "
-                         f"state (FP64 C restatement, CG+Jacobi rtol 1e-10 maxit 1000, "
-                         f"{cits} iterations, {cwall:.1f} s)"}
+        log(f"CPU oracle: one step with each solver at {n}^3 on {nthreads} threads")
+        recs, ref = cpu_baseline(case, state0, nthreads)
+        b, c = recs["bicgstab_sor"], recs["cg_jacobi"]
+        cpu = {"value": 1.0 / b["seconds"], "unit": "time-steps/s", "cores": nthreads, "kind": "port",
+               "sample": f"step {state0['itime'] + 1} of the {n}^3 workload from the GPU timed window's "
+                         f"start state, FP64 C restatement on {nthreads} threads, the reference's solver "
+                         f"BiCGStab + SOR(10, omega 1.7; {nthreads} SOR blocks) rtol 1e-10 maxit 1000: "
+                         f"{b['iterations']} iterations, {'converged' if b['converged'] else 'NOT converged'}, "
+                         f"{b['seconds']:.1f} s",
+               "solvers": {"bicgstab_sor": {**b, "steps_per_s": 1.0 / b["seconds"]},
+                           "cg_jacobi": {**c, "steps_per_s": 1.0 / c["seconds"]}}}
+        # the same step on both GPU paths, against the oracle's converged step
+        vs_oracle = {"step": state0["itime"] + 1,
+                     "oracle": f"BiCGStab + SOR, rtol 1e-10, {b['iterations']} iterations"}
+        one_fast = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, mg_levels=a.fast_levels,
+                               flags=flags & ~FLAG_LEARNED_WARM)
+        for name, o in (("fast", one_fast), ("plain_fp64", plain_opts)):
+            _, fl = gpu_one_step(case, state0, o)
+            vs_oracle[name] = {f: float(np.abs(fl[f][1:-1, 1:-1, 1:-1] - getattr(ref, f)[1:-1, 1:-1, 1:-1]).max()
+                                        / np.abs(getattr(ref, f)[1:-1, 1:-1, 1:-1]).max())
+                               for f in ("pressure", "phase", "concent")}
 
     out = {
         "metric": METRIC,
@@ -380,9 +457,9 @@ def main():
                  "f32 solve in f64 refinement",
         "data": "synthetic: seeded Gaussian random field rock (corr 4 cells), Kozeny-Carman k",
         "config": {"workload": f"wormholing {n}^3, steps {a.warmup + 1}..{a.warmup + a.steps} "
-                               f"of the 500-step run", "grid": [n, n, n],
+                               f"of a {run_len}-step run", "grid": [n, n, n],
                    "solver": ("" if a.no_project else "projected initial guess (last 4 solution changes) + ") +
-                             ("learned first stage + " if net else "") +
+                             ("learned first stage (gated) + " if net else "") +
                              (f"FP32 MG-PCG ({a.fast_levels} levels, 2 Jacobi sweeps)" if a.jacobi else
                               f"FP32 MG-PCG ({a.fast_levels} levels, degree-2 Chebyshev, FP16-stored fine faces)") +
                              " in FP64 refinement, rtol 1e-10",
@@ -390,10 +467,14 @@ def main():
                    "parallelism": "1 GPU"},
         "plain_fp64_steps_per_s": sps64,
         "speedup_vs_plain_fp64": sps / sps64,
+        "whole_run": whole,
         "p_rel_err_vs_plain_fp64": perr,
+        "rel_err_vs_fp64_oracle_one_step": vs_oracle,
         "cell_updates_per_s": sps * ncell,
         "iterations_per_step": {"plain_fp64": its64 / a.steps, "fast_fp32_inner": itsf / a.steps},
-        "learned_stage": ({"weights": os.path.relpath(weights, ROOT), "accepted_steps": accepted}
+        "learned_stage": ({"weights": os.path.relpath(weights, ROOT), "accepted_steps": accepted,
+                           "contribution": "none: the gate found no iteration saving and kept the stage off"
+                           if accepted == 0 else f"accepted in {accepted} solves"}
                           if net else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

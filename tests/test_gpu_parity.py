@@ -386,9 +386,11 @@ def _overshoot_state(grid=(48, 20, 16), seed=9):
     ph = np.maximum(1.6 * np.tanh(u / 1.5) + 0.25 * rng.standard_normal(u.shape), -1.0)
     st.phase[...] = np.asfortranarray(ph)
     io.clamp_halo(st.phase)
-    # wells keep their own phase, as the reference never updates them
+    # wells keep their own phase, as the reference never updates them; the
+    # halo then follows the clamp rule the device applies
     wells = st.lw > 0
     st.phase[wells] = case.phase[wells]
+    io.clamp_halo(st.phase)
     return case, st
 
 

@@ -300,6 +300,8 @@ def bench_ensemble(a):
         "ms_per_step": 1e3 * wall_e / a.steps,
         "dtype": "f64 stencils + f32 solve (FP16-stored faces) in f64 refinement",
         "data": "synthetic: 48 seeded GRF rocks, correlation length 2-8 cells, mean porosity 0.12-0.25",
+        "learned_stage": "not used: the ensemble's captured schedule holds no per-member network "
+                         "launches (the mixed-precision path without the network)",
         "config": {"workload": "ensemble: 16x64^3 + 16x96x64x64 + 8x128x96x96 + 8x160x128x128"
                                + (f" (each dimension / {a.ens_shrink})" if a.ens_shrink > 1 else ""),
                    "cells": cells,

@@ -111,3 +111,46 @@ def test_adaptive_schedule_shrinks_and_matches(gpu):
             x, y = getattr(a, name), getattr(b, name)
             assert np.abs(x - y).max() <= 1e-6 * np.abs(y).max(), name
     ens.close()
+
+
+def test_ensemble_at_the_48_grid_shape(gpu):
+    """The 48-member ensemble workload (16 x 64^3, 16 x 96x64x64, 8 x
+    128x96x96, 8 x 160x128x128, every dimension halved: 5.1M cells) on the
+    fast path's flags, with a schedule long enough that no member falls
+    back: every member is bitwise equal to the same grid stepped alone with
+    wpf_step, whose solves all met the FP64 test; and the first member
+    (32^3) against the FP64 oracle over the same steps: pressure within 1e-6
+    relative, phase and concentration within 1e-5."""
+    from oracle.oracle import OracleState
+    flags = FLAGS | _lib.FLAG_OP16 | _lib.FLAG_PROJECT
+    cases = io.ensemble_cases(shrink=2)
+    assert len(cases) == 48
+    opts = solver_opts("mixed", rtol=1e-10, inner_rtol=1e-4, flags=flags)
+    ens_m = [WPFAW3D(c, opts=opts) for c in cases]
+    solo = [WPFAW3D(c, opts=opts) for c in cases]
+    for g in ens_m + solo:
+        g.set_tile_depth(4)
+    nsteps = 8
+    ens = Ensemble(ens_m, opts, stages=4, inner_iters=40, tile_depth=4, grid_blocks=1024)
+    st = ens.step(nsteps)
+    ens.sync()
+    print("48-grid ensemble stats", st, flush=True)
+    assert st["fallbacks"] == 0 and st["steps"] == nsteps
+    for g in solo:
+        for _ in range(nsteps):
+            assert g.step()["converged"] == 1
+    for k, (a, b) in enumerate(zip(ens_m, solo)):
+        assert (a.itime, a.time, a.dtime, a.taumin) == (b.itime, b.time, b.dtime, b.taumin), k
+        for name in ("pressure", "phase", "concent"):
+            assert np.array_equal(getattr(a, name), getattr(b, name)), (k, name)
+    ref = OracleState(cases[0])
+    ref.run(nsteps, rtol=1e-12, maxit=20000)
+    err = {}
+    for name in ("pressure", "phase", "concent"):
+        x, y = getattr(ens_m[0], name)[1:-1, 1:-1, 1:-1], getattr(ref, name)[1:-1, 1:-1, 1:-1]
+        err[name] = float(np.abs(x - y).max() / np.abs(y).max())
+    print("member 0 against the oracle", err, flush=True)
+    assert err["pressure"] <= 1e-6 and err["phase"] <= 1e-5 and err["concent"] <= 1e-5, err
+    ens.close()
+    for g in ens_m + solo:
+        g.close()

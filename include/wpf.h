@@ -293,15 +293,37 @@ int wpf_net_gradient(wpf_net* net, int32_t nx, int32_t ny, int32_t nz,
 int wpf_net_train(wpf_net* net, int32_t nx, int32_t ny, int32_t nz,
                   int32_t nsamples, const float* fields, const float* targets,
                   int32_t epochs, float lr, double* epoch_losses);
+/* Energy-norm training.  For a correction u (the output upsampled to full
+ * resolution, 0 on Dirichlet cells) and the error t it should approximate,
+ * the exact line search along u removes the share (u, At)^2/((u, Au)(t, At))
+ * of the error's A-norm; the loss is 1 minus that share, the quantity the
+ * solver's learned stage gains.  A is the 7-point operator of the pair:
+ * faces [3][nx*ny*nz] (+x/+y/+z face of each cell, 0 on the global edge, any
+ * common scale) and fixed[n] (1 on Dirichlet cells).  wpf_net_gradient_energy
+ * is the loss and gradient of one host-given pair. */
+int wpf_net_gradient_energy(wpf_net* net, int32_t nx, int32_t ny, int32_t nz, const float* fields,
+                            const float* target, const float* faces, const uint8_t* fixed,
+                            double* loss, float* grad);
+/* A training set kept on the device: room for max_pairs pairs and max_ops
+ * operators on one grid; wpf_net_dataset_add appends every stride-th pair
+ * captured by the context's last flexible-CG solve (WPF_FLAG_LEARNED_FCG |
+ * WPF_FLAG_CAPTURE) with the operator of that solve (call it before the
+ * context's next permeability pass), and returns the pair count; wpf_net_train_energy runs epochs over the set in a seeded shuffled
+ * order, one Adam step per pair, and returns each epoch's mean loss. */
+int wpf_net_dataset_reserve(wpf_net* net, int32_t nx, int32_t ny, int32_t nz, int32_t max_pairs,
+                            int32_t max_ops);
+int wpf_net_dataset_add(wpf_net* net, wpf_ctx* ctx, int32_t stride, int32_t* npairs);
+int wpf_net_train_energy(wpf_net* net, int32_t epochs, float lr, int32_t seed, double* epoch_losses);
 int wpf_attach_net(wpf_ctx* ctx, wpf_net* net);
 /* After a solve with WPF_FLAG_CAPTURE: the network input of its start
  * state (planar [3][n]) and the correction the solver found (n values). */
 int wpf_capture_get(wpf_ctx* ctx, float* fields, float* target);
 /* After a CG + Jacobi solve with WPF_FLAG_LEARNED_FCG | WPF_FLAG_CAPTURE:
- * the network input at every second learned-stage point of flexible CG (up
- * to 16) and the correction from the iterate there to the final solution,
- * the training pairs of the stage; count, then pair j (fields [3][n],
- * target n). */
+ * the network input at learned-stage points of flexible CG and the
+ * correction from the iterate there to the final solution, the training
+ * pairs of the stage: of every second stage point, the first and a uniform
+ * sample (fixed generator) of up to 15 later ones; count, then pair j
+ * (fields [3][n], target n). */
 int wpf_capture_fcg_count(wpf_ctx* ctx, int32_t* count);
 int wpf_capture_fcg_get(wpf_ctx* ctx, int32_t j, float* fields, float* target);
 
@@ -334,6 +356,10 @@ int wpf_stream_probe(int32_t device, int32_t nread, int32_t nwrite, int64_t n, i
  * partitioned by tile, so the depth changes results in the last bits; the
  * default keeps them independent of timing. */
 int wpf_set_tile_depth(wpf_ctx* ctx, int32_t planes);
+/* Every period-th preconditioner application of flexible CG with the
+ * learned stage (WPF_SOLVER_CG_JACOBI + WPF_FLAG_LEARNED_FCG) is the stage
+ * (16 by default; 1 = every application). */
+int wpf_set_fcg_period(wpf_ctx* ctx, int32_t period);
 
 /* ---- ensembles -------------------------------------------------------------
  * No counterpart in the reference (which runs one realisation per job): many

@@ -135,6 +135,11 @@ def lib():
         "wpf_net_gradient": ([P, I32, I32, I32, F, F, D, F], I32),
         "wpf_net_train": ([P, I32, I32, I32, I32, F, F, I32, ctypes.c_float, D], I32),
         "wpf_attach_net": ([P, P], I32),
+        "wpf_set_fcg_period": ([P, I32], I32),
+        "wpf_net_gradient_energy": ([P, I32, I32, I32, F, F, F, ctypes.POINTER(ctypes.c_uint8), D, F], I32),
+        "wpf_net_dataset_reserve": ([P, I32, I32, I32, I32, I32], I32),
+        "wpf_net_dataset_add": ([P, P, I32, ctypes.POINTER(I32)], I32),
+        "wpf_net_train_energy": ([P, I32, ctypes.c_float, I32, D], I32),
         "wpf_capture_get": ([P, F, F], I32),
         "wpf_config_output": ([ctypes.POINTER(I32), ctypes.POINTER(I32)], I32),
         "wpf_result_write": ([ctypes.c_char_p, ctypes.c_double, ctypes.POINTER(WpfParams), I32, D, D, D, D], I32),

@@ -925,6 +925,13 @@ int ensure_graphs(wpf_ensemble* e)
 
 extern "C" {
 
+int wpf_set_fcg_period(wpf_ctx* c, int32_t period)
+{
+    WPF_CHECK(c && period >= 1, WPF_ERR_ARG, "bad stage period");
+    c->fcg_period = period;
+    return WPF_OK;
+}
+
 int wpf_set_tile_depth(wpf_ctx* c, int32_t planes)
 {
     WPF_CHECK(c && planes >= -1 && planes <= 256, WPF_ERR_ARG, "bad tile depth");

@@ -18,8 +18,9 @@
 // Forward convolution = implicit GEMM, M = voxels (16 per wavefront),
 // N = output channels (16 per MFMA), K = 27 * Cin in chunks of 32; each lane
 // loads its 8 K-values as one 16-byte vector (Cin is a multiple of 8).
-// Training (offline, not on the timed path) uses FP32 direct kernels for the
-// data and weight gradients, a scale-invariant cosine loss and Adam.
+// Training (offline, not on the timed path) runs the data and weight
+// gradients as implicit GEMMs on the same MFMA instruction (BF16 operands,
+// FP32 accumulation), with a scale-invariant loss and Adam.
 #include "wpf_internal.h"
 
 #include <cmath>
@@ -244,41 +245,152 @@ __device__ __forceinline__ float src_at(const Src& s, const Geo& g, int xx, int 
     return (float)s.b[vb * s.cb + (ci - s.ca)];
 }
 
-// gin[u][ci] = sum_tap sum_co W[co][tap][ci] gz[u - off(tap)][co]
-__global__ void k_conv_bwd_data(Geo g, int cin, int cout, const float* __restrict__ w,
-                                const float* __restrict__ gz, float* __restrict__ gin)
+// The backward convolutions are implicit GEMMs on MFMA too, with the
+// incoming gradient gz rounded to BF16 (weights and activations already
+// are) and FP32 accumulation.
+//
+// Data gradient: gin[u][ci] = sum_tap sum_co W[co][tap][ci] gz[u - off(tap)][co]
+// M = voxels u (16 per wavefront), N = input channels ci, K = (tap, co) with
+// co padded to copad (a multiple of 8, so the 8 K-values a lane loads lie in
+// one tap and are one 16-byte vector of the padded BF16 gz).
+__global__ void k_gz_bf16(int64_t nv, int cout, int copad, const float* __restrict__ gz, bf16* __restrict__ out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.nv * cin) return;
-    const int ci = (int)(i % cin);
-    const int64_t u = i / cin;
-    const int x = (int)(u % g.nx), y = (int)((u / g.nx) % g.ny), z = (int)(u / ((int64_t)g.nx * g.ny));
-    float s = 0.0f;
-    for (int tap = 0; tap < 27; tap++) {
-        const int xx = x - (tap % 3 - 1), yy = y - ((tap / 3) % 3 - 1), zz = z - (tap / 9 - 1);
-        if (xx < 0 || xx >= g.nx || yy < 0 || yy >= g.ny || zz < 0 || zz >= g.nz) continue;
-        const float* gv = gz + (xx + (int64_t)g.nx * (yy + (int64_t)g.ny * zz)) * cout;
-        for (int co = 0; co < cout; co++) s += w[((int64_t)co * 27 + tap) * cin + ci] * gv[co];
-    }
-    gin[i] = s;
+    if (i >= nv * copad) return;
+    const int c = (int)(i % copad);
+    const int64_t v = i / copad;
+    out[i] = (bf16)(c < cout ? gz[v * cout + c] : 0.0f);
 }
 
-// dW[co][tap][ci] = sum_v gz[v][co] in(v + off(tap))[ci]
-__global__ void k_conv_bwd_w(Geo g, Src s, int cout, const float* __restrict__ gz, float* __restrict__ dw)
+// transposed packed weights WT[ci][tap * copad + co], zero padded
+__global__ void k_pack_wT(const float* __restrict__ w, int cout, int cin, int copad, int cinpad, int kpadT,
+                          bf16* __restrict__ wt)
 {
-    const int cin = s.ca + s.cb;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)cout * 27 * cin) return;
-    const int ci = (int)(i % cin);
-    const int tap = (int)((i / cin) % 27);
-    const int co = (int)(i / (27 * (int64_t)cin));
-    const int ox = tap % 3 - 1, oy = (tap / 3) % 3 - 1, oz = tap / 9 - 1;
-    float acc = 0.0f;
-    for (int64_t v = 0; v < g.nv; v++) {
-        const int x = (int)(v % g.nx), y = (int)((v / g.nx) % g.ny), z = (int)(v / ((int64_t)g.nx * g.ny));
-        acc += gz[v * cout + co] * src_at(s, g, x + ox, y + oy, z + oz, ci);
+    if (i >= (int64_t)cinpad * kpadT) return;
+    const int ci = (int)(i / kpadT), k = (int)(i % kpadT);
+    const int tap = k / copad, co = k % copad;
+    const float v = (ci < cin && tap < 27 && co < cout) ? w[((int64_t)co * 27 + tap) * cin + ci] : 0.0f;
+    wt[i] = (bf16)v;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) k_conv_bwd_data(Geo g, const bf16* __restrict__ gz16, int copad,
+                                                       const bf16* __restrict__ wt, int kpadT, int cin,
+                                                       float* __restrict__ gin)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t tile = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    if (tile >= g.nv) return;  // whole wavefront leaves together; no barriers below
+    const int64_t v = tile + (lane & 15);
+    const bool vin = v < g.nv;
+    int x = 0, y = 0, z = 0;
+    if (vin) {
+        x = (int)(v % g.nx);
+        y = (int)((v / g.nx) % g.ny);
+        z = (int)(v / ((int64_t)g.nx * g.ny));
     }
-    dw[i] = acc;
+    const int kq = 8 * (lane >> 4);
+    const int ktot = 27 * copad;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < kpadT; k0 += 32) {
+        const int kk = k0 + kq;
+        bf16x8 a;
+#pragma unroll
+        for (int e = 0; e < 8; e++) a[e] = (bf16)0.0f;
+        if (vin && kk < ktot) {
+            const int tap = kk / copad, co = kk - tap * copad;
+            // the output voxel that reads u through this tap
+            const int xx = x - (tap % 3 - 1), yy = y - ((tap / 3) % 3 - 1), zz = z - (tap / 9 - 1);
+            if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
+                a = *reinterpret_cast<const bf16x8*>(gz16 + (xx + (int64_t)g.nx * (yy + (int64_t)g.ny * zz)) * copad + co);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int n = t * 16 + (lane & 15);
+            const bf16x8 bw = *reinterpret_cast<const bf16x8*>(wt + (int64_t)n * kpadT + kk);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int n = t * 16 + (lane & 15);
+        if (n >= cin) continue;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int64_t vo = tile + 4 * (lane >> 4) + e;
+            if (vo < g.nv) gin[vo * cin + n] = acc[t][e];
+        }
+    }
+}
+
+// Weight gradient: dW[co][tap][ci] = sum_v gz[v][co] in(v + off(tap))[ci]
+// M = co (16 per tile, blockIdx.y), N = tap * cin + ci (16 per wavefront, 4
+// wavefronts per block along blockIdx.x), K = voxels, split over blockIdx.z
+// into chunks of vchunk (a multiple of 32); each chunk's partial dW goes to
+// part[chunk][co][n] and k_sum_chunks adds the chunks in a fixed order.
+// gzT is gz transposed to [co][v] in BF16 (zero rows beyond cout, zero
+// voxels beyond nv up to nvpad), so a lane's 8 K-values are one vector.
+__global__ void k_gz_bf16_t(int64_t nv, int64_t nvpad, int cout, int coutpad, const float* __restrict__ gz,
+                            bf16* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)coutpad * nvpad) return;
+    const int co = (int)(i / nvpad);
+    const int64_t v = i % nvpad;
+    out[i] = (bf16)((co < cout && v < nv) ? gz[v * cout + co] : 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_conv_bwd_w(Geo g, Src s, const bf16* __restrict__ gzT, int64_t nvpad,
+                                                    int cout, int64_t vchunk, float* __restrict__ part)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cin = s.ca + s.cb, ncol = 27 * cin;
+    const int ncol0 = (blockIdx.x * 4 + wave) * 16;
+    if (ncol0 >= ncol) return;  // whole wavefront leaves together; no barriers below
+    const int n = ncol0 + (lane & 15);
+    const bool nvalid = n < ncol;
+    const int tap = nvalid ? n / cin : 0, ci = nvalid ? n - tap * cin : 0;
+    const int ox = tap % 3 - 1, oy = (tap / 3) % 3 - 1, oz = tap / 9 - 1;
+    const int co = blockIdx.y * 16 + (lane & 15);
+    const bf16* arow = gzT + (int64_t)co * nvpad;
+    const int64_t v0 = (int64_t)blockIdx.z * vchunk;
+    const int64_t v1 = min(v0 + vchunk, g.nv);
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t vb = v0; vb < v1; vb += 32) {
+        const int64_t kv = vb + 8 * (lane >> 4);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + kv);
+        bf16x8 b;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int64_t v = kv + e;
+            float val = 0.0f;
+            if (nvalid && v < g.nv) {
+                const int x = (int)(v % g.nx), y = (int)((v / g.nx) % g.ny), z = (int)(v / ((int64_t)g.nx * g.ny));
+                val = src_at(s, g, x + ox, y + oy, z + oz, ci);
+            }
+            b[e] = (bf16)val;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (!nvalid) return;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int r = blockIdx.y * 16 + 4 * (lane >> 4) + e;
+        if (r < cout) part[((int64_t)blockIdx.z * cout + r) * ncol + n] = acc[e];
+    }
+}
+
+// out[i] = sum over chunks c of part[c][i], c in order
+__global__ void k_sum_chunks(int64_t count, int nchunk, const float* __restrict__ part, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.0f;
+    for (int c = 0; c < nchunk; c++) s += part[(int64_t)c * count + i];
+    out[i] = s;
 }
 
 __global__ void k_bias_grad(Geo g, int cout, const float* __restrict__ gz, float* __restrict__ db)
@@ -414,6 +526,126 @@ __global__ void k_upsample_out(Geo fg, Geo cg, const float* __restrict__ y, cons
     out[u] = (R)v;
 }
 
+// ---------------------------------------------------------------------------
+// energy-norm loss
+//
+// For a correction u (the network's output upsampled to full resolution, 0
+// at Dirichlet cells) and the error t it should approximate (r = A t), the
+// exact line search along u removes the share
+//     (u, r)^2 / ((u, A u) (t, r))
+// of the A-norm of the error: the loss is 1 minus that share.  It depends
+// on neither the scale of u nor that of t.
+// ---------------------------------------------------------------------------
+// out = A v on free cells (7-point: the sum of the cell's faces times v
+// minus each face times the neighbour), 0 on Dirichlet cells; faces
+// [3][n] hold the +x/+y/+z face of each cell, 0 on the global edge.
+__global__ void k_energy_apply(Geo g, const float* __restrict__ faces, const uint8_t* __restrict__ mask,
+                               const float* __restrict__ v, float* __restrict__ out)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.nv) return;
+    if (mask[c] & WPF_MASK_FIXED) {
+        out[c] = 0.0f;
+        return;
+    }
+    const int64_t P = (int64_t)g.nx * g.ny;
+    const int x = (int)(c % g.nx), y = (int)((c / g.nx) % g.ny), z = (int)(c / P);
+    const float* tx = faces;
+    const float* ty = faces + g.nv;
+    const float* tz = faces + 2 * g.nv;
+    float d = tx[c] + ty[c] + tz[c], s = 0.0f;
+    if (x < g.nx - 1) s += tx[c] * v[c + 1];
+    if (y < g.ny - 1) s += ty[c] * v[c + g.nx];
+    if (z < g.nz - 1) s += tz[c] * v[c + P];
+    if (x > 0) { d += tx[c - 1]; s += tx[c - 1] * v[c - 1]; }
+    if (y > 0) { d += ty[c - g.nx]; s += ty[c - g.nx] * v[c - g.nx]; }
+    if (z > 0) { d += tz[c - P]; s += tz[c - P] * v[c - P]; }
+    out[c] = d * v[c] - s;
+}
+
+// (u, At), (u, Au), (t, At) partials
+__global__ void __launch_bounds__(256) k_energy_dots(int64_t n, const float* __restrict__ u,
+                                                     const float* __restrict__ au, const float* __restrict__ t,
+                                                     const float* __restrict__ at, double* __restrict__ partials)
+{
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ui = u[i], ati = at[i];
+        v[0] += ui * ati;
+        v[1] += ui * (double)au[i];
+        v[2] += (double)t[i] * ati;
+    }
+    block_reduce_store<3>(v, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
+}
+
+// loss into d[3]; dL/du = -2a/(bc) At + 2a^2/(b^2 c) Au (a, b, c the dots)
+__global__ void k_energy_grad(int64_t n, double* __restrict__ d, const float* __restrict__ at,
+                              const float* __restrict__ au, float* __restrict__ gu)
+{
+    const double a = d[0], b = d[1], c = d[2];
+    const bool ok = b > 1e-300 && c > 1e-300;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) d[3] = ok ? 1.0 - a * a / (b * c) : 1.0;
+    if (i < n) {
+        const double ka = ok ? -2.0 * a / (b * c) : 0.0;
+        const double kb = ok ? 2.0 * a * a / (b * b * c) : 0.0;
+        gu[i] = (float)(ka * at[i] + kb * au[i]);
+    }
+}
+
+// weight of coarse index X in the cell-centred linear interpolation of fine
+// index x (the 1-D factor of k_upsample_out)
+__device__ __forceinline__ float up_weight(int x, int X, int nc)
+{
+    float c = ((float)x + 0.5f) / (float)kPool - 0.5f;
+    c = fminf(fmaxf(c, 0.0f), (float)(nc - 1));
+    const int x0 = min((int)c, nc - 1), x1 = min(x0 + 1, nc - 1);
+    const float f = c - x0;
+    return (x0 == X ? 1.0f - f : 0.0f) + (x1 == X ? f : 0.0f);
+}
+
+// gy = U^T gu: the adjoint of k_upsample_out (Dirichlet cells carry gu = 0)
+__global__ void k_upsample_adj(Geo fg, Geo cg, const float* __restrict__ gu, float* __restrict__ gy)
+{
+    const int64_t V = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (V >= cg.nv) return;
+    const int X = (int)(V % cg.nx), Y = (int)((V / cg.nx) % cg.ny), Z = (int)(V / ((int64_t)cg.nx * cg.ny));
+    // fine cells whose interpolation reads coarse cell X lie within
+    // [kPool (X - 1), kPool (X + 2)) on each axis
+    const int xa = max(kPool * (X - 1), 0), xb = min(kPool * (X + 2), fg.nx);
+    const int ya = max(kPool * (Y - 1), 0), yb = min(kPool * (Y + 2), fg.ny);
+    const int za = max(kPool * (Z - 1), 0), zb = min(kPool * (Z + 2), fg.nz);
+    float s = 0.0f;
+    for (int z = za; z < zb; z++) {
+        const float wz = up_weight(z, Z, cg.nz);
+        if (wz == 0.0f) continue;
+        for (int y = ya; y < yb; y++) {
+            const float wy = up_weight(y, Y, cg.ny);
+            if (wy == 0.0f) continue;
+            const float* row = gu + (int64_t)fg.nx * (y + (int64_t)fg.ny * z);
+            float sx = 0.0f;
+            for (int x = xa; x < xb; x++) {
+                const float wx = up_weight(x, X, cg.nx);
+                if (wx != 0.0f) sx += wx * row[x];
+            }
+            s += wz * wy * sx;
+        }
+    }
+    gy[V] = s;
+}
+
+// the solver's FP64 faces as FP32 times a scale (the loss ignores the
+// operator's scale; this keeps the values near 1)
+__global__ void k_faces32(int64_t n, double scale, const double* __restrict__ tx, const double* __restrict__ ty,
+                          const double* __restrict__ tz, float* __restrict__ out)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    out[c] = (float)(tx[c] * scale);
+    out[n + c] = (float)(ty[c] * scale);
+    out[2 * n + c] = (float)(tz[c] * scale);
+}
+
 }  // namespace wpf
 
 // ---------------------------------------------------------------------------
@@ -441,8 +673,24 @@ struct wpf_net {
     float *g_d1 = nullptr, *gcat1 = nullptr, *g_d2 = nullptr, *g_s1 = nullptr, *gcat2 = nullptr,
           *g_d3 = nullptr, *g_s2 = nullptr, *gcat3 = nullptr, *g_b4 = nullptr, *g_s3 = nullptr,
           *g_p3 = nullptr, *g_p2 = nullptr, *g_p1 = nullptr;
+    // backward on MFMA: transposed packed weights per layer, the BF16 copies
+    // of gz (padded channels; transposed) and the weight gradient's chunks
+    bf16* wT[kNL] = {};
+    bf16 *gz16 = nullptr, *gzT = nullptr;
+    float* wpart = nullptr;
     double* dots = nullptr;
     double* partials = nullptr;
+    // energy-norm loss at full resolution: u = U y, A u, A t, dL/du
+    float *eu = nullptr, *eau = nullptr, *eat = nullptr, *egu = nullptr;
+    float* efaces = nullptr;       // [3][n] operator of a host-given sample
+    uint8_t* emask = nullptr;      // [n]
+    // training set on the device: captured pairs and their operators, all
+    // on one grid (dnx, dny, dnz)
+    int dnx = 0, dny = 0, dnz = 0;
+    int dcap_s = 0, dcap_o = 0, dn_s = 0, dn_o = 0;
+    float *dfields = nullptr, *dtargets = nullptr, *dfaces = nullptr;
+    uint8_t* dmask = nullptr;
+    std::vector<int> dop;          // operator of each pair
 };
 
 namespace {
@@ -456,8 +704,13 @@ void free_acts(wpf_net* n)
 {
     void* ps[] = {n->x0, n->s1, n->p1, n->s2, n->p2, n->s3, n->p3, n->b4, n->d3, n->d2, n->d1, n->y,
                   n->fields, n->target, n->tpool, n->gy, n->g_d1, n->gcat1, n->g_d2, n->g_s1,
-                  n->gcat2, n->g_d3, n->g_s2, n->gcat3, n->g_b4, n->g_s3, n->g_p3, n->g_p2, n->g_p1};
+                  n->gcat2, n->g_d3, n->g_s2, n->gcat3, n->g_b4, n->g_s3, n->g_p3, n->g_p2, n->g_p1,
+                  n->gz16, n->gzT, n->wpart, n->eu, n->eau, n->eat, n->egu, n->efaces, n->emask};
     for (void* p : ps) fr(p);
+    n->gz16 = n->gzT = nullptr;
+    n->wpart = nullptr;
+    n->eu = n->eau = n->eat = n->egu = n->efaces = nullptr;
+    n->emask = nullptr;
     n->x0 = n->s1 = n->p1 = n->s2 = n->p2 = n->s3 = n->p3 = n->b4 = n->d3 = n->d2 = n->d1 = nullptr;
     n->y = n->fields = n->target = n->tpool = n->gy = nullptr;
     n->g_d1 = n->gcat1 = n->g_d2 = n->g_s1 = n->gcat2 = n->g_d3 = n->g_s2 = n->gcat3 = nullptr;
@@ -497,6 +750,27 @@ int ensure_grid(wpf_net* n, int nx, int ny, int nz)
     return WPF_OK;
 }
 
+// backward shapes of layer li (output level g): channel paddings, the
+// transposed weights' K, and the voxel split of the weight gradient
+struct BwdShape {
+    int copad, cinpad, kpadT, coutpad16;
+    int64_t nvpad, vchunk;
+    int nchunk;
+};
+BwdShape bwd_shape(const NetLayer& L, const Geo& g)
+{
+    BwdShape b;
+    b.copad = ((L.cout + 7) / 8) * 8;
+    b.cinpad = ((L.cin + 15) / 16) * 16;
+    b.kpadT = ((27 * b.copad + 31) / 32) * 32;
+    b.coutpad16 = ((L.cout + 15) / 16) * 16;
+    b.nvpad = ((g.nv + 31) / 32) * 32;
+    const int64_t want = std::min<int64_t>(std::max<int64_t>((g.nv + 1023) / 1024, 1), 512);
+    b.vchunk = (((g.nv + want - 1) / want + 31) / 32) * 32;
+    b.nchunk = (int)((g.nv + b.vchunk - 1) / b.vchunk);
+    return b;
+}
+
 int ensure_train(wpf_net* n)
 {
     if (n->gy) return WPF_OK;
@@ -513,6 +787,15 @@ int ensure_train(wpf_net* n)
         (rc = alloc(&n->g_s3, v3 * 48)) || (rc = alloc(&n->g_p3, v4 * 48)) ||
         (rc = alloc(&n->g_p2, v3 * 32)) || (rc = alloc(&n->g_p1, v2 * 16)))
         return rc;
+    int64_t m16 = 0, mT = 0, mp = 0;
+    for (int i = 0; i < kNL; i++) {
+        const NetLayer& L = n->L[i];
+        const BwdShape b = bwd_shape(L, n->lv[L.level]);
+        m16 = std::max(m16, n->lv[L.level].nv * b.copad);
+        mT = std::max(mT, (int64_t)b.coutpad16 * b.nvpad);
+        mp = std::max(mp, (int64_t)b.nchunk * L.cout * 27 * L.cin);
+    }
+    if ((rc = alloc(&n->gz16, m16)) || (rc = alloc(&n->gzT, mT)) || (rc = alloc(&n->wpart, mp))) return rc;
     return WPF_OK;
 }
 
@@ -524,6 +807,10 @@ int pack_weights(wpf_net* n)
         const NetLayer& L = n->L[i];
         hipLaunchKernelGGL(k_pack_w, dim3(nblk((int64_t)L.coutpad * L.kpad)), dim3(256), 0, n->stream,
                            (const float*)(n->params + L.woff), L.cout, L.cin, L.coutpad, L.kpad, n->wt[i]);
+        WPF_HIP(hipGetLastError());
+        const BwdShape b = bwd_shape(L, n->lv[L.level]);
+        hipLaunchKernelGGL(k_pack_wT, dim3(nblk((int64_t)b.cinpad * b.kpadT)), dim3(256), 0, n->stream,
+                           (const float*)(n->params + L.woff), L.cout, L.cin, b.copad, b.cinpad, b.kpadT, n->wT[i]);
         WPF_HIP(hipGetLastError());
     }
     n->packed = true;
@@ -586,38 +873,57 @@ int layer_bwd(wpf_net* n, int li, const Geo& g, Src s, const float* gz, float* g
     const NetLayer& L = n->L[li];
     float* dw = n->grads + L.woff;
     float* db = dw + (int64_t)L.cout * 27 * L.cin;
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3(nblk((int64_t)L.cout * 27 * L.cin)), dim3(256), 0, n->stream,
-                       g, s, L.cout, gz, dw);
+    const BwdShape b = bwd_shape(L, g);
+    const int ncol = 27 * L.cin;
+    // weight gradient: gz transposed to BF16, chunks of voxels, fixed-order sum
+    hipLaunchKernelGGL(k_gz_bf16_t, dim3(nblk((int64_t)b.coutpad16 * b.nvpad)), dim3(256), 0, n->stream, g.nv,
+                       b.nvpad, L.cout, b.coutpad16, gz, n->gzT);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_conv_bwd_w, dim3((unsigned)((ncol + 63) / 64), (unsigned)(b.coutpad16 / 16), (unsigned)b.nchunk),
+                       dim3(256), 0, n->stream, g, s, (const bf16*)n->gzT, b.nvpad, L.cout, b.vchunk, n->wpart);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_chunks, dim3(nblk((int64_t)L.cout * ncol)), dim3(256), 0, n->stream,
+                       (int64_t)L.cout * ncol, b.nchunk, (const float*)n->wpart, dw);
     WPF_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_bias_grad, dim3(L.cout), dim3(256), 0, n->stream, g, L.cout, gz, db);
     WPF_HIP(hipGetLastError());
     if (gin) {
-        hipLaunchKernelGGL(k_conv_bwd_data, dim3(nblk(g.nv * L.cin)), dim3(256), 0, n->stream, g, L.cin,
-                           L.cout, (const float*)(n->params + L.woff), gz, gin);
+        if (!n->packed) {
+            int rc = pack_weights(n);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_gz_bf16, dim3(nblk(g.nv * b.copad)), dim3(256), 0, n->stream, g.nv, L.cout, b.copad,
+                           gz, n->gz16);
+        WPF_HIP(hipGetLastError());
+        const unsigned blocks = (unsigned)((g.nv + 63) / 64);
+        const int nt = b.cinpad / 16;
+#define BWDD(NT)                                                                                          \
+    hipLaunchKernelGGL((k_conv_bwd_data<NT>), dim3(blocks), dim3(256), 0, n->stream, g, (const bf16*)n->gz16, \
+                       b.copad, (const bf16*)n->wT[li], b.kpadT, L.cin, gin)
+        switch (nt) {
+        case 1: BWDD(1); break;
+        case 2: BWDD(2); break;
+        case 3: BWDD(3); break;
+        case 4: BWDD(4); break;
+        case 5: BWDD(5); break;
+        case 6: BWDD(6); break;
+        case 7: BWDD(7); break;
+        default:
+            set_error("backward: %d input channels", L.cin);
+            return WPF_ERR_ARG;
+        }
+#undef BWDD
         WPF_HIP(hipGetLastError());
     }
     return WPF_OK;
 }
 
-// loss and full gradient for the sample in n->fields / n->target
-int loss_and_grad(wpf_net* n, double* loss)
+// gradient of every parameter from dL/dy in n->gy (level 1), through the
+// activations of the last forward pass
+int backprop(wpf_net* n)
 {
     const Geo* G = n->lv;
-    int rc = forward(n, n->fields);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_target_pool, dim3(nblk(G[1].nv)), dim3(256), 0, n->stream, G[0], G[1],
-                       (const float*)n->target, n->tpool);
-    WPF_HIP(hipGetLastError());
-    const int nb = 64;
-    hipLaunchKernelGGL(k_loss_dots, dim3(nb), dim3(256), 0, n->stream, G[1].nv, (const float*)n->y,
-                       (const float*)n->tpool, n->partials);
-    WPF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, n->stream, (const double*)n->partials, nb, 3,
-                       n->dots);
-    WPF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_loss_grad, dim3(nblk(G[1].nv)), dim3(256), 0, n->stream, G[1].nv, n->dots,
-                       (const float*)n->y, (const float*)n->tpool, n->gy);
-    WPF_HIP(hipGetLastError());
+    int rc;
     auto relu = [&](int64_t cnt, float* g, const bf16* act) {
         hipLaunchKernelGGL(k_relu_mask, dim3(nblk(cnt)), dim3(256), 0, n->stream, cnt, g, act);
     };
@@ -655,11 +961,94 @@ int loss_and_grad(wpf_net* n, double* loss)
     relu(G[1].nv * 16, n->g_s1, n->s1);
     if ((rc = layer_bwd(n, 0, G[1], Src{n->x0, 8, 0, nullptr, 0}, n->g_s1, nullptr))) return rc;
     WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
+int read_loss(wpf_net* n, double* loss)
+{
     double d[4];
     WPF_HIP(hipMemcpyAsync(d, n->dots, sizeof(d), hipMemcpyDeviceToHost, n->stream));
     WPF_HIP(hipStreamSynchronize(n->stream));
     if (loss) *loss = d[3];
     return WPF_OK;
+}
+
+// cosine loss and full gradient for the sample in n->fields / n->target
+int loss_and_grad(wpf_net* n, double* loss)
+{
+    const Geo* G = n->lv;
+    int rc = forward(n, n->fields);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_target_pool, dim3(nblk(G[1].nv)), dim3(256), 0, n->stream, G[0], G[1],
+                       (const float*)n->target, n->tpool);
+    WPF_HIP(hipGetLastError());
+    const int nb = 64;
+    hipLaunchKernelGGL(k_loss_dots, dim3(nb), dim3(256), 0, n->stream, G[1].nv, (const float*)n->y,
+                       (const float*)n->tpool, n->partials);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, n->stream, (const double*)n->partials, nb, 3,
+                       n->dots);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_loss_grad, dim3(nblk(G[1].nv)), dim3(256), 0, n->stream, G[1].nv, n->dots,
+                       (const float*)n->y, (const float*)n->tpool, n->gy);
+    WPF_HIP(hipGetLastError());
+    if ((rc = backprop(n))) return rc;
+    return read_loss(n, loss);
+}
+
+int ensure_energy(wpf_net* n)
+{
+    if (n->eu) return WPF_OK;
+    const int64_t v0 = n->lv[0].nv;
+    int rc;
+    if ((rc = alloc(&n->eu, v0)) || (rc = alloc(&n->eau, v0)) || (rc = alloc(&n->eat, v0)) ||
+        (rc = alloc(&n->egu, v0)) || (rc = alloc(&n->efaces, 3 * v0)) || (rc = alloc(&n->emask, v0)))
+        return rc;
+    return WPF_OK;
+}
+
+// energy-norm loss and full gradient for one pair on the device: network
+// input fields [3][n], error t [n], operator faces [3][n] and mask [n]
+int energy_loss_and_grad(wpf_net* n, const float* fields, const float* t, const float* faces,
+                         const uint8_t* mask, double* loss)
+{
+    const Geo* G = n->lv;
+    int rc;
+    if ((rc = ensure_train(n)) || (rc = ensure_energy(n))) return rc;
+    if ((rc = forward(n, fields))) return rc;
+    const int64_t v0 = G[0].nv;
+    hipLaunchKernelGGL(k_upsample_out<float>, dim3(nblk(v0)), dim3(256), 0, n->stream, G[0], G[1],
+                       (const float*)n->y, mask, n->eu);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_energy_apply, dim3(nblk(v0)), dim3(256), 0, n->stream, G[0], faces, mask,
+                       (const float*)n->eu, n->eau);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_energy_apply, dim3(nblk(v0)), dim3(256), 0, n->stream, G[0], faces, mask, t, n->eat);
+    WPF_HIP(hipGetLastError());
+    const int nb = 64;
+    hipLaunchKernelGGL(k_energy_dots, dim3(nb), dim3(256), 0, n->stream, v0, (const float*)n->eu,
+                       (const float*)n->eau, t, (const float*)n->eat, n->partials);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, n->stream, (const double*)n->partials, nb, 3, n->dots);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_energy_grad, dim3(nblk(v0)), dim3(256), 0, n->stream, v0, n->dots,
+                       (const float*)n->eat, (const float*)n->eau, n->egu);
+    WPF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_upsample_adj, dim3(nblk(G[1].nv)), dim3(256), 0, n->stream, G[0], G[1],
+                       (const float*)n->egu, n->gy);
+    WPF_HIP(hipGetLastError());
+    if ((rc = backprop(n))) return rc;
+    return read_loss(n, loss);
+}
+
+void adam_step(wpf_net* n, float lr)
+{
+    n->adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)n->adam_t), c2 = 1.0f - std::pow(b2, (float)n->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(n->np)), dim3(256), 0, n->stream, n->np, n->params,
+                       (const float*)n->grads, n->m, n->v, lr, b1, b2, 1e-8f, c1, c2);
+    n->packed = false;
 }
 
 int upload_sample(wpf_net* n, int nx, int ny, int nz, const float* fields, const float* target)
@@ -815,6 +1204,10 @@ int wpf_net_create(int32_t seed, wpf_net** out)
         return rc;
     for (int i = 0; i < kNL; i++)
         if ((rc = alloc(&n->wt[i], (int64_t)n->L[i].coutpad * n->L[i].kpad))) return rc;
+    for (int i = 0; i < kNL; i++) {
+        const BwdShape b = bwd_shape(n->L[i], geo(1, 1, 1));
+        if ((rc = alloc(&n->wT[i], (int64_t)b.cinpad * b.kpadT))) return rc;
+    }
     // He initialisation from a fixed-seed generator (host), zero biases
     std::vector<float> h(n->np, 0.0f);
     uint64_t s = 0x9E3779B97F4A7C15ull ^ (uint64_t)(uint32_t)seed;
@@ -845,7 +1238,12 @@ int wpf_net_destroy(wpf_net* n)
     (void)hipStreamSynchronize(n->stream);
     free_acts(n);
     fr(n->params); fr(n->grads); fr(n->m); fr(n->v); fr(n->dots); fr(n->partials);
-    for (int i = 0; i < kNL; i++) fr(n->wt[i]);
+    fr(n->eu); fr(n->eau); fr(n->eat); fr(n->egu); fr(n->efaces); fr(n->emask);
+    fr(n->dfields); fr(n->dtargets); fr(n->dfaces); fr(n->dmask);
+    for (int i = 0; i < kNL; i++) {
+        fr(n->wt[i]);
+        fr(n->wT[i]);
+    }
     (void)hipStreamDestroy(n->stream);
     delete n;
     return WPF_OK;
@@ -925,6 +1323,121 @@ int wpf_net_train(wpf_net* n, int32_t nx, int32_t ny, int32_t nz, int32_t nsampl
             n->packed = false;
         }
         if (losses) losses[e] = sum / (double)use.size();
+    }
+    WPF_HIP(hipStreamSynchronize(n->stream));
+    return WPF_OK;
+}
+
+int wpf_net_gradient_energy(wpf_net* n, int32_t nx, int32_t ny, int32_t nz, const float* fields,
+                            const float* target, const float* faces, const uint8_t* fixed, double* loss,
+                            float* grad)
+{
+    WPF_CHECK(n && fields && target && faces && fixed && grad, WPF_ERR_ARG, "null argument");
+    int rc = upload_sample(n, nx, ny, nz, fields, target);
+    if (rc) return rc;
+    if ((rc = ensure_energy(n))) return rc;
+    const size_t nv = (size_t)nx * ny * nz;
+    std::vector<uint8_t> m(nv);
+    for (size_t i = 0; i < nv; i++) m[i] = fixed[i] ? WPF_MASK_FIXED : 0;
+    WPF_HIP(hipMemcpyAsync(n->efaces, faces, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, n->stream));
+    WPF_HIP(hipMemcpyAsync(n->emask, m.data(), nv, hipMemcpyHostToDevice, n->stream));
+    if ((rc = energy_loss_and_grad(n, n->fields, n->target, n->efaces, n->emask, loss))) return rc;
+    WPF_HIP(hipMemcpyAsync(grad, n->grads, sizeof(float) * n->np, hipMemcpyDeviceToHost, n->stream));
+    WPF_HIP(hipStreamSynchronize(n->stream));
+    return WPF_OK;
+}
+
+int wpf_net_dataset_reserve(wpf_net* n, int32_t nx, int32_t ny, int32_t nz, int32_t max_pairs, int32_t max_ops)
+{
+    WPF_CHECK(n && max_pairs > 0 && max_ops > 0, WPF_ERR_ARG, "bad dataset size");
+    WPF_CHECK(nx % 32 == 0 && ny % 32 == 0 && nz % 32 == 0, WPF_ERR_ARG,
+              "network grid %dx%dx%d: every size must be a multiple of 32", nx, ny, nz);
+    WPF_HIP(hipSetDevice(n->dev));
+    WPF_HIP(hipStreamSynchronize(n->stream));
+    fr(n->dfields); fr(n->dtargets); fr(n->dfaces); fr(n->dmask);
+    n->dfields = n->dtargets = n->dfaces = nullptr;
+    n->dmask = nullptr;
+    n->dop.clear();
+    n->dn_s = n->dn_o = 0;
+    const int64_t nv = (int64_t)nx * ny * nz;
+    int rc;
+    if ((rc = alloc(&n->dfields, 3 * nv * max_pairs)) || (rc = alloc(&n->dtargets, nv * max_pairs)) ||
+        (rc = alloc(&n->dfaces, 3 * nv * max_ops)) || (rc = alloc(&n->dmask, nv * max_ops)))
+        return rc;
+    n->dnx = nx; n->dny = ny; n->dnz = nz;
+    n->dcap_s = max_pairs;
+    n->dcap_o = max_ops;
+    return WPF_OK;
+}
+
+int wpf_net_dataset_add(wpf_net* n, wpf_ctx* c, int32_t stride, int32_t* npairs)
+{
+    WPF_CHECK(n && c && stride >= 1, WPF_ERR_ARG, "bad argument");
+    WPF_CHECK(n->dcap_s > 0, WPF_ERR_STATE, "no dataset reserved");
+    WPF_CHECK(c->g.nx == n->dnx && c->g.ny == n->dny && c->g.nz == n->dnz, WPF_ERR_ARG,
+              "context grid %dx%dx%d is not the dataset's %dx%dx%d", c->g.nx, c->g.ny, c->g.nz, n->dnx,
+              n->dny, n->dnz);
+    // the faces still hold the operator of the capturing solve: wpf_step
+    // marks them stale for the next step, and only a permeability pass
+    // rewrites them
+    WPF_CHECK(c->fcap_n > 0, WPF_ERR_STATE, "the context's last solve captured no stage points");
+    int add = 0;
+    for (int j = 0; j < c->fcap_n; j += stride) add++;
+    WPF_CHECK(n->dn_s + add <= n->dcap_s && n->dn_o < n->dcap_o, WPF_ERR_ARG,
+              "dataset full (%d + %d pairs of %d, %d operators of %d)", n->dn_s, add, n->dcap_s, n->dn_o,
+              n->dcap_o);
+    WPF_HIP(hipSetDevice(n->dev));
+    const int64_t nv = c->g.n;
+    const int op = n->dn_o++;
+    hipLaunchKernelGGL(k_faces32, dim3(nblk(nv)), dim3(256), 0, c->stream, nv, c->t_scale, (const double*)c->tx.p,
+                       (const double*)c->ty.p, (const double*)c->tz.p, n->dfaces + 3 * nv * op);
+    WPF_HIP(hipGetLastError());
+    WPF_HIP(hipMemcpyAsync(n->dmask + nv * op, c->mask, nv, hipMemcpyDeviceToDevice, c->stream));
+    for (int j = 0; j < c->fcap_n; j += stride) {
+        const int s = n->dn_s++;
+        WPF_HIP(hipMemcpyAsync(n->dfields + 3 * nv * s, c->fcap_fields[j], sizeof(float) * 3 * nv,
+                               hipMemcpyDeviceToDevice, c->stream));
+        WPF_HIP(hipMemcpyAsync(n->dtargets + nv * s, c->fcap_target[j], sizeof(float) * nv, hipMemcpyDeviceToDevice,
+                               c->stream));
+        n->dop.push_back(op);
+    }
+    WPF_HIP(hipStreamSynchronize(c->stream));
+    if (npairs) *npairs = n->dn_s;
+    return WPF_OK;
+}
+
+int wpf_net_train_energy(wpf_net* n, int32_t epochs, float lr, int32_t seed, double* losses)
+{
+    WPF_CHECK(n && epochs > 0, WPF_ERR_ARG, "bad training arguments");
+    WPF_CHECK(n->dn_s > 0, WPF_ERR_STATE, "the dataset is empty");
+    WPF_HIP(hipSetDevice(n->dev));
+    int rc = ensure_grid(n, n->dnx, n->dny, n->dnz);
+    if (rc) return rc;
+    const int64_t nv = (int64_t)n->dnx * n->dny * n->dnz;
+    std::vector<int> order(n->dn_s);
+    for (int i = 0; i < n->dn_s; i++) order[i] = i;
+    uint64_t st = 0x243F6A8885A308D3ull ^ (uint64_t)(uint32_t)seed;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    for (int e = 0; e < epochs; e++) {
+        for (int i = n->dn_s - 1; i > 0; i--) std::swap(order[i], order[next() % (uint64_t)(i + 1)]);
+        double sum = 0.0;
+        for (int s : order) {
+            const int op = n->dop[s];
+            double L = 0.0;
+            if ((rc = energy_loss_and_grad(n, n->dfields + 3 * nv * s, n->dtargets + nv * s, n->dfaces + 3 * nv * op,
+                                           n->dmask + nv * op, &L)))
+                return rc;
+            sum += L;
+            adam_step(n, lr);
+            WPF_HIP(hipGetLastError());
+        }
+        if (losses) losses[e] = sum / (double)n->dn_s;
     }
     WPF_HIP(hipStreamSynchronize(n->stream));
     return WPF_OK;

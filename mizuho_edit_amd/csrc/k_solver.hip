@@ -609,17 +609,22 @@ static __global__ void __launch_bounds__(kThreads) k_fcg_stage(int64_t n, const 
 
 // Flexible CG with truncation 1 (beta = -(z_new, q_old)/(p_old, q_old)) on
 // the FP64 operator with the Jacobi preconditioner, where every
-// kFcgPeriod-th preconditioner application, the first included, is the
-// learned stage of k_fcg_stage.  The network's output depends nonlinearly
-// on r, which is why the Fletcher-Reeves beta of pcg_core does not apply.
-// Starts from the FP64 residual in F.r (norm rn); stops when the recursive
-// ||invD r|| <= tol, after maxit iterations, or when the residual fails to
-// halve over kFcgWindow iterations: the stage disturbs the conjugacy of the
-// earlier directions, which on residuals unlike the network's training data
-// (the first steps of a run) can stall the iteration, and the caller then
-// restarts plain CG from the current iterate.  *stages counts the learned
+// c->fcg_period-th preconditioner application (16 by default), the first
+// included, is the learned stage of k_fcg_stage.  The network's output
+// depends nonlinearly on r, which is why the Fletcher-Reeves beta of
+// pcg_core does not apply.  Starts from the FP64 residual in F.r (norm rn);
+// stops when the recursive ||invD r|| <= tol, after maxit iterations, or
+// when the residual drops by less than 10% over kFcgWindow iterations: the
+// stage disturbs the conjugacy of the earlier directions, which on
+// residuals unlike the network's training data can stall the iteration, and
+// the caller then restarts plain CG from the current iterate.  (At 192^3,
+// where CG + Jacobi needs 800-1000 iterations, a pass without this test
+// does not converge in 1000: flexible CG keeps only the last direction's
+// conjugacy, so the pass helps while the stage removes the slow error and
+// the fallback finishes the solve.)  *stages counts the learned
 // applications.
-static constexpr int kFcgPeriod = 8, kFcgWindow = 32;
+static constexpr int kFcgWindow = 32;
+static constexpr double kFcgStall = 0.9;
 
 static int fcg_learned(wpf_ctx* c, const Fine<double, double>& F, double* x, double rn, double tol, int maxit,
                        int* its_out, double* rn_out, int* stages, bool capture)
@@ -634,10 +639,26 @@ static int fcg_learned(wpf_ctx* c, const Fine<double, double>& F, double* x, dou
     int rz = S_FCG, bnum = S_FCG + 1;
     auto stage = [&](double rnow, const double* qold) -> int {
         const int kid_ = WPF_KERNEL_CG_UPDATE;
-        // capture: every second stage point, the network input and the iterate
+        // capture: the network input and the iterate at every second stage
+        // point; the first always, the others as a uniform sample of the
+        // whole pass once the slots are full (reservoir sampling with a fixed
+        // generator, so runs repeat)
         float* keep = nullptr;
-        if (capture && *stages % 2 == 0 && c->fcap_n < wpf_ctx::kFcapMax) {
-            const int j = c->fcap_n++;
+        int j = -1;
+        if (capture && *stages % 2 == 0) {
+            const int m = *stages / 2;
+            if (m < wpf_ctx::kFcapMax) {
+                j = m;
+                c->fcap_n = m + 1;
+            } else {
+                uint64_t h = 0x9E3779B97F4A7C15ull * (uint64_t)(m + 1);
+                h = (h ^ (h >> 31)) * 0xBF58476D1CE4E5B9ull;
+                h ^= h >> 29;
+                const int r = 1 + (int)(h % (uint64_t)m);  // slots 1.. over candidates 1..m
+                if (r < wpf_ctx::kFcapMax) j = r;
+            }
+        }
+        if (j >= 0) {
             if (!c->fcap_fields[j]) {
                 WPF_HIP(hipMalloc((void**)&c->fcap_fields[j], sizeof(float) * 3 * (size_t)n));
                 WPF_HIP(hipMalloc((void**)&c->fcap_x[j], sizeof(double) * (size_t)n));
@@ -686,10 +707,10 @@ static int fcg_learned(wpf_ctx* c, const Fine<double, double>& F, double* x, dou
         }
         if (rn <= tol) break;
         if (its % kFcgWindow == 0) {
-            if (rn > 0.5 * rn_window) break;
+            if (rn > kFcgStall * rn_window) break;
             rn_window = rn;
         }
-        if (its % kFcgPeriod == 0) {
+        if (its % c->fcg_period == 0) {
             if ((rc = stage(rn, (const double*)F.q))) return rc;
         } else {
             rz = S_RN + 1;

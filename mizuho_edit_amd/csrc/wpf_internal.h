@@ -176,6 +176,7 @@ struct wpf_ctx {
     // at up to kFcapMax stage points of the last such solve, the iterate
     // there, and the correction to the final solution
     static constexpr int kFcapMax = 16;
+    int fcg_period = 16;                      // learned stage every n-th application
     float* fcap_fields[kFcapMax] = {};        // [3][n] each
     double* fcap_x[kFcapMax] = {};            // [n]
     float* fcap_target[kFcapMax] = {};        // [n]

@@ -29,6 +29,15 @@ LAYERS = [("e1", 8, 16), ("e2", 16, 32), ("e3", 32, 48), ("e4", 48, 64),
 CIN0 = 8
 
 
+def bf16(a):
+    """Round to the nearest BF16 value, ties to even (float64 result): the
+    rounding the GPU applies where it stores weights, activations and the
+    backward pass's incoming gradients in BF16."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def init_params(seed=0):
     rng = np.random.default_rng(seed)
     P = {}
@@ -104,24 +113,29 @@ def pool4_target(t):
     return pool(pool(np.asarray(t, dtype=np.float64)[..., None]))
 
 
-def forward(P, x0):
-    """x0: pooled 8-channel input (make_input).  Returns (y, cache)."""
-    C = {}
+def forward(P, x0, rounded=False):
+    """x0: pooled 8-channel input (make_input).  Returns (y, cache).  With
+    rounded=True the weights, the input and every stored activation (each
+    layer's output and each pooled tensor) are rounded to BF16 as the GPU
+    stores them; sums stay exact."""
+    rnd = bf16 if rounded else (lambda a: a)
+    C = {"rounded": rounded}
+    x0 = rnd(x0)
     C["x0"] = x0
     def layer(name, x):
-        z = conv(x, P[name + ".w"], P[name + ".b"])
+        z = conv(x, rnd(P[name + ".w"]), P[name + ".b"])
         C[name + ".in"] = x
         C[name + ".z"] = z
-        return np.maximum(z, 0.0)
+        return rnd(np.maximum(z, 0.0))
     s1 = layer("e1", x0)
-    s2 = layer("e2", pool(s1))
-    s3 = layer("e3", pool(s2))
-    b4 = layer("e4", pool(s3))
+    s2 = layer("e2", rnd(pool(s1)))
+    s3 = layer("e3", rnd(pool(s2)))
+    b4 = layer("e4", rnd(pool(s3)))
     d3 = layer("d3", np.concatenate([up(b4), s3], axis=-1))
     d2 = layer("d2", np.concatenate([up(d3), s2], axis=-1))
     d1 = layer("d1", np.concatenate([up(d2), s1], axis=-1))
     C["out.in"] = d1
-    y = conv(d1, P["out.w"], P["out.b"])
+    y = conv(d1, rnd(P["out.w"]), P["out.b"])
     return y, C
 
 
@@ -133,13 +147,22 @@ def loss(y, t):
 
 
 def backward(P, C, gy):
+    """Gradients of every weight and bias.  After a rounded forward, each
+    layer's incoming gradient gz and its weights enter the convolution's
+    backward in BF16 (the bias gradient sums the unrounded gz), as on the
+    GPU."""
     G = {}
-    g, G["out.w"], G["out.b"] = conv_backward(C["out.in"], P["out.w"], gy)
+    rnd = bf16 if C.get("rounded") else (lambda a: a)
+
+    def conv_back(name, gz):
+        gx, G[name + ".w"], _ = conv_backward(C[name + ".in"], rnd(P[name + ".w"]), rnd(gz))
+        G[name + ".b"] = gz.sum(axis=(0, 1, 2))
+        return gx
+
+    g = conv_back("out", gy)
 
     def layer_back(name, g_act):
-        gz = g_act * (C[name + ".z"] > 0)
-        gx, G[name + ".w"], G[name + ".b"] = conv_backward(C[name + ".in"], P[name + ".w"], gz)
-        return gx
+        return conv_back(name, g_act * (C[name + ".z"] > 0))
 
     g = layer_back("d1", g)                   # g: d/d cat(up(d2), s1)
     g_up2, g_s1 = g[..., :32], g[..., 32:]
@@ -155,3 +178,70 @@ def backward(P, C, gy):
     g_s1 = g_s1 + pool_backward(g)
     layer_back("e1", g_s1)
     return G
+
+
+# ---------------------------------------------------------------------------
+# energy-norm loss: the share of the error's A-norm that the exact line
+# search along the upsampled output removes
+# ---------------------------------------------------------------------------
+def up_matrix(nf, nc, pool=4):
+    """1-D cell-centred linear interpolation from nc coarse to nf fine
+    cells: fine x sits at coarse coordinate (x + 0.5)/pool - 0.5, clamped to
+    [0, nc - 1]."""
+    M = np.zeros((nf, nc))
+    for x in range(nf):
+        c = min(max((x + 0.5) / pool - 0.5, 0.0), nc - 1.0)
+        x0 = min(int(np.floor(c)), nc - 1)
+        x1 = min(x0 + 1, nc - 1)
+        f = c - x0
+        M[x, x0] += 1.0 - f
+        M[x, x1] += f
+    return M
+
+
+def upsample(y, fixed):
+    """y (nx/4, ny/4, nz/4) -> (nx, ny, nz), 0 on the fixed cells."""
+    nx, ny, nz = fixed.shape
+    Mx, My, Mz = (up_matrix(n, c) for n, c in zip((nx, ny, nz), y.shape))
+    u = np.einsum("ia,jb,kc,abc->ijk", Mx, My, Mz, y)
+    u[fixed] = 0.0
+    return u
+
+
+def upsample_adjoint(g, fixed, coarse_shape):
+    nx, ny, nz = fixed.shape
+    Mx, My, Mz = (up_matrix(n, c) for n, c in zip((nx, ny, nz), coarse_shape))
+    return np.einsum("ia,jb,kc,ijk->abc", Mx, My, Mz, np.where(fixed, 0.0, g))
+
+
+def apply_op(faces, fixed, v):
+    """7-point operator: faces (3, nx, ny, nz) hold each cell's +x/+y/+z face
+    (0 on the global edge); free rows d v - sum t v_nb with d the sum of the
+    cell's six faces, fixed rows 0."""
+    tx, ty, tz = faces
+    d = tx + ty + tz
+    d[1:, :, :] += tx[:-1, :, :]
+    d[:, 1:, :] += ty[:, :-1, :]
+    d[:, :, 1:] += tz[:, :, :-1]
+    s = np.zeros_like(v)
+    s[:-1, :, :] += tx[:-1, :, :] * v[1:, :, :]
+    s[1:, :, :] += tx[:-1, :, :] * v[:-1, :, :]
+    s[:, :-1, :] += ty[:, :-1, :] * v[:, 1:, :]
+    s[:, 1:, :] += ty[:, :-1, :] * v[:, :-1, :]
+    s[:, :, :-1] += tz[:, :, :-1] * v[:, :, 1:]
+    s[:, :, 1:] += tz[:, :, :-1] * v[:, :, :-1]
+    out = d * v - s
+    out[fixed] = 0.0
+    return out
+
+
+def energy_loss(y, target, faces, fixed):
+    """L = 1 - (u, At)^2 / ((u, Au)(t, At)) for u = upsample(y); returns L
+    and dL/dy with y's shape (..., 1)."""
+    yc = y[..., 0]
+    u = upsample(yc, fixed)
+    At, Au = apply_op(faces, fixed, target), apply_op(faces, fixed, u)
+    a, b, c = float((u * At).sum()), float((u * Au).sum()), float((target * At).sum())
+    L = 1.0 - a * a / (b * c)
+    gu = -2.0 * a / (b * c) * At + 2.0 * a * a / (b * b * c) * Au
+    return L, upsample_adjoint(gu, fixed, yc.shape)[..., None]

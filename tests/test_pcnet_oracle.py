@@ -63,3 +63,73 @@ def test_loss_is_scale_invariant():
     L2, _ = pcnet.loss(3.7 * y, -0.2 * t)
     assert L1 == pytest.approx(L2, rel=1e-12)
     assert pcnet.loss(t, t)[0] == pytest.approx(0.0, abs=1e-14)
+
+
+def test_bf16_rounding_ties_to_even():
+    # spacing of BF16 at 1 is 2^-7: 1 + 2^-8 is a tie (to 1), 1 + 3*2^-8 a
+    # tie between odd and even mantissas (to 1 + 2^-6)
+    got = pcnet.bf16(np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, 1 + 2 ** -8 + 2 ** -12, -2.5]))
+    np.testing.assert_array_equal(got, [1.0, 1.0, 1 + 2 ** -6, 1 + 2 ** -7, -2.5])
+
+
+def _energy_case(n=16, seed=3):
+    from mizuho_edit_amd import io
+    from oracle.oracle import OracleState
+    case = io.workload("plain-small", seed=seed, grid=(n, n, n))
+    st = OracleState(case)
+    st.calc_permeability()
+    p = case.params
+    inner = (slice(1, n + 1),) * 3
+    faces = np.stack([k[inner] / p.viscosity for k in (st.kx, st.ky, st.kz)])
+    faces[0][-1, :, :] = 0.0
+    faces[1][:, -1, :] = 0.0
+    faces[2][:, :, -1] = 0.0
+    lw = st.lw[inner]
+    fixed = np.vectorize(lambda i: i > 0 and case.wells[i - 1].pressure != 0)(lw)
+    return st, faces, fixed
+
+
+def test_energy_operator_is_the_assembled_system():
+    st, faces, fixed = _energy_case()
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=fixed.shape)
+    v[fixed] = 0.0
+    got = pcnet.apply_op(faces, fixed, v)
+    ref = st.sor_operator(v.reshape(-1, order="F")).reshape(fixed.shape, order="F")
+    free = ~fixed
+    np.testing.assert_allclose(got[free], ref[free], rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_upsample_partition_of_unity_and_adjoint():
+    rng = np.random.default_rng(4)
+    fixed = np.zeros((16, 12, 8), bool)
+    fixed[0] = True
+    u = pcnet.upsample(np.ones((4, 3, 2)), fixed)
+    np.testing.assert_allclose(u[~fixed], 1.0, rtol=1e-15)
+    y, g = rng.normal(size=(4, 3, 2)), rng.normal(size=fixed.shape)
+    lhs = (pcnet.upsample(y, fixed) * g).sum()
+    rhs = (y * pcnet.upsample_adjoint(g, fixed, y.shape)).sum()
+    assert lhs == pytest.approx(rhs, rel=1e-13)
+
+
+def test_energy_loss_exact_direction_and_gradient():
+    """A target that is itself an upsampled coarse field is removed
+    exactly by the line search along it (loss 0, any scale or sign), and the
+    gradient of the loss matches central differences."""
+    st, faces, fixed = _energy_case()
+    rng = np.random.default_rng(5)
+    yc = rng.normal(size=(4, 4, 4))
+    t = pcnet.upsample(yc, fixed)
+    L, _ = pcnet.energy_loss(-3.0 * yc[..., None], t, faces, fixed)
+    assert abs(L) < 1e-12
+    y = rng.normal(size=(4, 4, 4, 1))
+    t = rng.normal(size=fixed.shape)
+    t[fixed] = 0.0
+    L, gy = pcnet.energy_loss(y, t, faces, fixed)
+    assert 0.0 <= L <= 1.0
+    h = 1e-6
+    for idx in [(0, 0, 0, 0), (2, 1, 3, 0), (3, 3, 3, 0)]:
+        e = np.zeros_like(y)
+        e[idx] = h
+        fd = (pcnet.energy_loss(y + e, t, faces, fixed)[0] - pcnet.energy_loss(y - e, t, faces, fixed)[0]) / (2 * h)
+        assert gy[idx] == pytest.approx(fd, rel=1e-5, abs=1e-10)

@@ -222,18 +222,21 @@ def calibrate_copy(reps=20):
           flush=True)
 
 
-def calibrate_streams(n=1 << 24, reps=20):
+def calibrate_streams(sizes=(1 << 26, 1 << 24), reps=20):
     """The library's streaming probe (wpf_stream_probe): one JSON line per
-    (arrays read, arrays written) pair, n floats per array."""
+    (arrays read, arrays written) pair and array size (n floats per array;
+    256 MiB arrays keep every configuration far beyond the 256 MB
+    Infinity Cache, 64 MiB ones are the size of a 256^3 FP32 field)."""
     import ctypes
     from mizuho_edit_amd import _lib
     L = _lib.lib()
-    for nr, nw in ((1, 1), (2, 1), (4, 1), (5, 2), (6, 3), (7, 1), (8, 1), (8, 3), (10, 1), (0, 3)):
-        us = ctypes.c_double()
-        _lib.check(L.wpf_stream_probe(0, nr, nw, n, reps, ctypes.byref(us)), "wpf_stream_probe")
-        nbytes = 4.0 * n * (nr + nw)
-        print(json.dumps({"read_arrays": nr, "written_arrays": nw, "mib_per_array": 4 * n >> 20,
-                          "us": us.value, "GBps": nbytes / (us.value * 1e-6) / 1e9}), flush=True)
+    for n in sizes:
+        for nr, nw in ((1, 1), (2, 1), (4, 1), (5, 2), (6, 3), (7, 1), (8, 1), (8, 3), (10, 1), (0, 3)):
+            us = ctypes.c_double()
+            _lib.check(L.wpf_stream_probe(0, nr, nw, n, reps, ctypes.byref(us)), "wpf_stream_probe")
+            nbytes = 4.0 * n * (nr + nw)
+            print(json.dumps({"read_arrays": nr, "written_arrays": nw, "mib_per_array": 4 * n >> 20,
+                              "us": us.value, "GBps": nbytes / (us.value * 1e-6) / 1e9}), flush=True)
 
 
 def bench_ensemble(a):

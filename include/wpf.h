@@ -344,8 +344,9 @@ int wpf_capture_fcg_get(wpf_ctx* ctx, int32_t j, float* fields, float* target);
 int wpf_profile_start(wpf_ctx* ctx, int32_t kernel);
 int wpf_profile_stop(wpf_ctx* ctx, double* total_ms, int32_t* count);
 /* Streaming bandwidth of the device: a kernel that reads `nread` (0..10)
- * and writes `nwrite` (1..3) arrays of n floats (n a multiple of 4), one
- * 16-byte element per thread, timed over `reps` launches after 3 untimed
+ * and writes `nwrite` (1..3) arrays of n floats (n a multiple of 4), four
+ * 16-byte vectors per thread and array with every load issued before its
+ * use and non-temporal stores, timed over `reps` launches after 3 untimed
  * ones.  *us is the average launch time.  A measurement utility with no
  * counterpart in the reference: it gives the rate a multi-array kernel can
  * reach, beside the 8 TB/s data-sheet peak. */

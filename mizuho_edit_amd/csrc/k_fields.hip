@@ -34,6 +34,17 @@ struct Dims {
 
 static inline Dims dims_of(const wpf_ctx* c) { return Dims{c->g.nx, c->g.ny, c->g.nz, c->g.plane}; }
 
+// Tile t of block b of an nb-block 1-D grid: blocks are dispatched
+// round-robin over the 8 XCDs, and this gives each XCD a contiguous run of
+// tiles (x fastest, then y, then z), so the blocks that share a border row
+// or plane run on the same XCD at about the same time and the border comes
+// from that XCD's L2.  A bijection of [0, nb) for any nb.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 static inline dim3 grid3(const Dims& d, dim3 blk)
 {
     return dim3((d.nx + blk.x - 1) / blk.x, (d.ny + blk.y - 1) / blk.y, (d.nz + blk.z - 1) / blk.z);
@@ -181,7 +192,10 @@ __global__ void __launch_bounds__(256) k_perm(Dims d, double kwh, double mu, con
                                               double* __restrict__ ty, double* __restrict__ tz,
                                               double* __restrict__ invD, PermOut32 O)
 {
-    t_perm<MODE>(d, blockIdx.x, blockIdx.y, blockIdx.z, kwh, mu, phi, kr, mask, tx, ty, tz, invD, O);
+    int gx, gy, gz;
+    (void)step_tiles(d.nx, d.ny, d.nz, &gx, &gy, &gz);
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    t_perm<MODE>(d, t % gx, (t / gx) % gy, t / (gx * gy), kwh, mu, phi, kr, mask, tx, ty, tz, invD, O);
 }
 
 int launch_perm(wpf_ctx* c, int mode, const PermOut32* out)
@@ -189,8 +203,7 @@ int launch_perm(wpf_ctx* c, int mode, const PermOut32* out)
     const Dims d = dims_of(c);
     const dim3 blk(kStepX, kStepY, 1);
     int gx, gy, gz;
-    (void)step_tiles(d.nx, d.ny, d.nz, &gx, &gy, &gz);
-    const dim3 gr(gx, gy, gz);
+    const dim3 gr((unsigned)step_tiles(d.nx, d.ny, d.nz, &gx, &gy, &gz));  // 1-D, see xcd_tile
     PermOut32 O{};
     if (mode != PERM_F64) {
         WPF_CHECK(out, WPF_ERR_ARG, "perm: no FP32 operator");
@@ -725,9 +738,11 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4)
                                                      int32_t* __restrict__ bt_step,
                                                      double* __restrict__ mins)
 {
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    t_step_fields<T>(d, blockIdx.x, blockIdx.y, blockIdx.z, K, dx, S, phi, conc, p, tx, ty, tz, mask, phi_out,
-                     conc_out, keys, scal, bt_step, mins, bid);
+    int gx, gy, gz;
+    (void)step_tiles(d.nx, d.ny, d.nz, &gx, &gy, &gz);
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    t_step_fields<T>(d, t % gx, (t / gx) % gy, t / (gx * gy), K, dx, S, phi, conc, p, tx, ty, tz, mask, phi_out,
+                     conc_out, keys, scal, bt_step, mins, t);
 }
 
 int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt, bool f32)
@@ -748,7 +763,7 @@ int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt,
     const dim3 blk(kStepX, kStepY, 1);
     int gx, gy, gz;
     const int nb = step_tiles(d.nx, d.ny, d.nz, &gx, &gy, &gz);
-    const dim3 gr(gx, gy, gz);
+    const dim3 gr((unsigned)nb);  // 1-D, see xcd_tile
     prof_mark(c, WPF_KERNEL_STEP_FIELDS, true);
     // the block minima use the reduction scratch as one flat array
     WPF_CHECK(reduction_fits(c->red, nb, 1), WPF_ERR_ARG, "grid too large for the reduction scratch");

@@ -24,6 +24,9 @@ static constexpr int kMaxBlocks = 2048;
 enum Slot { S_PQ = 0, S_RN = 1, S_RZ0 = 4, S_RZ1 = 5, S_BN = 6, S_T0 = 8, S_LR = 10, S_YAY = 11, S_PN = 12,
             S_FCG = 13, S_GRAM = 16, S_PROJY = 48 };
 static constexpr int kTuneSlot = 63;
+// host-only slots of the pinned mirror: the residual norms of the last two
+// CG iterations under lookahead (two ring entries)
+static constexpr int kRingSlot = 60;
 
 template <typename R>
 struct LV {

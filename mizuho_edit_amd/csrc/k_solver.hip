@@ -489,9 +489,17 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>
 // Preconditioned CG from the residual already in F.r, updating x (type X).
 // mg == nullptr: Jacobi.  Stops when ||invD r|| <= tol or after maxit
 // iterations; returns the iterations done and the last recursive norm.
+//
+// lookahead: each iteration's ||invD r||^2 is copied to a pinned host slot
+// behind an event, and while the residual of iteration k is far enough
+// above tol that it cannot be the last (its predecessor times the last
+// observed reduction factor is above 4 tol), iteration k+1 is enqueued
+// before the host waits for k, so the GPU never idles on the host's test.
+// The kernels and their order are the same as without lookahead; when the
+// prediction fails, the solve ends one iteration later (a smaller residual).
 template <typename R, typename X, typename C>
 static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F0, const MGOpts& o,
-                    X* x, double tol, int maxit, int* its_out, double* rn_out)
+                    X* x, double tol, int maxit, int* its_out, double* rn_out, bool lookahead = false)
 {
     Fine<R, C> F = F0;  // r and r2 trade places at every fused update
     const bool fuse = mg && F.vec && o.sweeps >= 2 && F.r2 != nullptr;
@@ -512,7 +520,14 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
     int its = 0;
     double rn = -1.0;
     int first = 1;
-    while (its < maxit) {
+    if (lookahead && !c->ev_ring[0]) {
+        WPF_HIP(hipEventCreateWithFlags(&c->ev_ring[0], hipEventDisableTiming));
+        WPF_HIP(hipEventCreateWithFlags(&c->ev_ring[1], hipEventDisableTiming));
+    }
+    // one CG iteration: direction, matvec, update; the residual norm of
+    // iteration its lands in result[S_RN] (and, under lookahead, in the
+    // pinned ring slot its & 1 behind event ev_ring[its & 1])
+    auto iterate = [&]() -> int {
         if (int rc = dir_mv(c, F, (const R*)zres, (const R*)pc, pn, rz_cur, rz_prev, first)) return rc;
         std::swap(pc, pn);
         if (fuse) {
@@ -530,22 +545,71 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
         }
         its++;
         first = 0;
-        int rc = read_results(c, S_RN, 2);
-        if (rc) return rc;
-        rn = std::sqrt(c->red.host[S_RN]);
-        if (!std::isfinite(rn)) {
+        if (lookahead) {
+            const int s = its & 1;
+            WPF_HIP(hipMemcpyAsync(c->red.host + kRingSlot + s, c->red.result + S_RN, sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream));
+            WPF_HIP(hipEventRecord(c->ev_ring[s], c->stream));
+        }
+        return WPF_OK;
+    };
+    // the preconditioned residual of the next iteration
+    auto precondition = [&]() -> int {
+        std::swap(rz_cur, rz_prev);
+        if (mg) return vcycle_fine(c, *mg, F, o, rz_cur, &zres, fuse);
+        zres = F.Z;
+        WPF_HIP(hipMemcpyAsync(c->red.result + rz_cur, c->red.result + S_RN + 1, sizeof(double),
+                               hipMemcpyDeviceToDevice, c->stream));
+        return WPF_OK;
+    };
+    // the residual norm of iteration k (already enqueued)
+    auto norm_of = [&](int k, double* out) -> int {
+        if (lookahead) {
+            WPF_HIP(hipEventSynchronize(c->ev_ring[k & 1]));
+            *out = std::sqrt(c->red.host[kRingSlot + (k & 1)]);
+        } else {
+            int rc = read_results(c, S_RN, 2);
+            if (rc) return rc;
+            *out = std::sqrt(c->red.host[S_RN]);
+        }
+        if (!std::isfinite(*out)) {
             set_error("pressure residual is not finite");
             return WPF_ERR_NAN;
         }
-        if (rn <= tol) break;
-        std::swap(rz_cur, rz_prev);
-        if (mg) {
-            rc = vcycle_fine(c, *mg, F, o, rz_cur, &zres, fuse);
-            if (rc) return rc;
-        } else {
-            zres = F.Z;
-            WPF_HIP(hipMemcpyAsync(c->red.result + rz_cur, c->red.result + S_RN + 1, sizeof(double),
-                                   hipMemcpyDeviceToDevice, c->stream));
+        return WPF_OK;
+    };
+    if (maxit > 0) {
+        if (int rc = iterate()) return rc;
+        double rn_prev = -1.0, ratio = 1.0;
+        for (;;) {
+            const int k = its;  // the last iteration enqueued
+            // iteration k+1 in flight while the host tests k, when k cannot
+            // be the last one
+            const bool ahead = lookahead && its < maxit && rn_prev > 0.0 && rn_prev * ratio > 4.0 * tol;
+            if (ahead) {
+                if (int rc = precondition()) return rc;
+                if (int rc = iterate()) return rc;
+            }
+            double rk;
+            if (int rc = norm_of(k, &rk)) return rc;
+            if (rk <= tol || (!ahead && its >= maxit)) {
+                rn = rk;
+                if (ahead) {
+                    if (int rc = norm_of(its, &rn)) return rc;
+                }
+                break;
+            }
+            ratio = rn_prev > 0.0 ? std::min(1.0, std::max(1e-3, rk / rn_prev)) : 1.0;
+            rn_prev = rk;
+            if (ahead) {
+                if (its >= maxit) {
+                    if (int rc = norm_of(its, &rn)) return rc;
+                    break;
+                }
+                continue;
+            }
+            if (int rc = precondition()) return rc;
+            if (int rc = iterate()) return rc;
         }
     }
     *its_out = its;
@@ -1096,8 +1160,11 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
             rc = fcg_learned(c, F, x, rn, tol, o.maxit - its, &k, &rr, &stages, capture);
             learned += stages;
         } else {
+            // lookahead on the FP64 solvers (the mixed solver's inner
+            // iterations stay host-tested one by one: an ensemble member
+            // repeats that arithmetic bit for bit)
             rc = pcg_core<double, double, double>(c, mg ? &c->mg64 : nullptr, F, mo, x, tol, o.maxit - its,
-                                                  &k, &rr);
+                                                  &k, &rr, true);
         }
         if (rc) return rc;
         its += k;

@@ -233,7 +233,7 @@ __device__ __forceinline__ T concent_cell(const PhaseConstsT<T>& K, const T* cc,
 // Tiles of the fused step kernel: 64 x 4 cells of a plane, kStepZ planes.
 constexpr int kStepX = 64, kStepY = 4, kStepZ = 16;
 
-inline int step_tiles(int nx, int ny, int nz, int* gx, int* gy, int* gz)
+__host__ __device__ inline int step_tiles(int nx, int ny, int nz, int* gx, int* gy, int* gz)
 {
     *gx = (nx + kStepX - 1) / kStepX;
     *gy = (ny + kStepY - 1) / kStepY;

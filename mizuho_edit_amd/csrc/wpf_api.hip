@@ -75,22 +75,47 @@ struct StreamPtrs {
     const float4* in[10];
     float4* out[3];
 };
+// Each thread moves kProbeU 16-byte vectors per array: a block covers a
+// contiguous run of 256 * kProbeU vectors, every load of the run is issued
+// before the first use, and the stores are non-temporal (the written arrays
+// are not read back by the probe).
+static constexpr int kProbeU = 4;
 template <int NR, int NW>
 __global__ void __launch_bounds__(256) k_stream_probe(StreamPtrs P, int64_t n4)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n4) return;
-    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int64_t base = (int64_t)blockIdx.x * (256 * kProbeU) + threadIdx.x;
+    float4 s[kProbeU];
+#pragma unroll
+    for (int u = 0; u < kProbeU; u++) s[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        const float4 v = P.in[r][i];
-        s.x += v.x;
-        s.y += v.y;
-        s.z += v.z;
-        s.w += v.w;
+        float4 v[kProbeU];
+#pragma unroll
+        for (int u = 0; u < kProbeU; u++) {
+            const int64_t i = base + 256 * u;
+            v[u] = i < n4 ? P.in[r][i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < kProbeU; u++) {
+            s[u].x += v[u].x;
+            s[u].y += v[u].y;
+            s[u].z += v[u].z;
+            s[u].w += v[u].w;
+        }
     }
 #pragma unroll
-    for (int w = 0; w < NW; w++) P.out[w][i] = make_float4(s.x + (float)w, s.y, s.z, s.w);
+    for (int w = 0; w < NW; w++)
+#pragma unroll
+        for (int u = 0; u < kProbeU; u++) {
+            const int64_t i = base + 256 * u;
+            if (i < n4) {
+                float* o = reinterpret_cast<float*>(P.out[w] + i);
+                __builtin_nontemporal_store(s[u].x + (float)w, o);
+                __builtin_nontemporal_store(s[u].y, o + 1);
+                __builtin_nontemporal_store(s[u].z, o + 2);
+                __builtin_nontemporal_store(s[u].w, o + 3);
+            }
+        }
 }
 template <int NR>
 static const void* probe_nw(int nw)
@@ -324,6 +349,8 @@ int wpf_destroy(wpf_ctx* c)
     for (auto& e : c->prof_ev) (void)hipEventDestroy(e);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    for (auto& e : c->ev_ring)
+        if (e) (void)hipEventDestroy(e);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -662,7 +689,7 @@ int wpf_stream_probe(int32_t device, int32_t nread, int32_t nwrite, int64_t n, i
         int64_t n4 = n / 4;
         const void* fn = probe_kernel(nread, nwrite);
         void* args[] = {&P, &n4};
-        const dim3 grid((unsigned)((n4 + 255) / 256)), block(256);
+        const dim3 grid((unsigned)((n4 + 256 * kProbeU - 1) / (256 * kProbeU))), block(256);
         float ms = 0.0f;
         bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
         for (int r = 0; ok && r < 3; r++) ok = hipLaunchKernel(fn, grid, block, args, 0, nullptr) == hipSuccess;

@@ -110,6 +110,8 @@ struct wpf_ctx {
     // stream inside one call, forked and joined with the two events
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // completion of the last two CG iterations' residual copies (lookahead)
+    hipEvent_t ev_ring[2] = {nullptr, nullptr};
     wpf_params prm{};
     wpf::Grid g;
 

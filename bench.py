@@ -80,11 +80,79 @@ def bytes_per_cell(kernel: str, r: int, h: int, cheb: bool = True, launches: int
     }[kernel]
 
 
-# HBM traffic per launch of the roofline kernel, from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE runs of this benchmark, each scaled by what the
-# counter reads for a 1 GiB copy (the command and the per-kernel table are
-# in the same folder)
+# HBM traffic per launch of the roofline kernel, from two rocprofv3 --pmc
+# runs of this benchmark (read requests by size, then write requests by
+# size; see traffic_table), written by --traffic-table (the commands and the
+# per-kernel table are in the same folder)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256.json")
+READ_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")
+WRITE_COUNTERS = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
+# kernels of the fast path reported in the table: name fragment -> (key,
+# algorithmic bytes per cell with FP32 vectors and FP16 faces: the formulas
+# of bytes_per_cell; post1/post2 are smoother_bytes' two post-smoothing
+# launches; Gram = 4 changes + their 4 stencil products' faces and r;
+# projection update = x, r, 4 changes, faces, mask -> x, r, y)
+TRAFFIC_KERNELS = {
+    "k_vupsweepIffDF16": ("cg_update", bytes_per_cell("cg_update", 4, 2)),
+    "k_vsweepIfDF16_Li4ELb1ELb0ELb1E": ("smooth_post2", 5 * 4 + 3 * 2 + 1 + 4 / 8),
+    "k_vsweepIfDF16_Li4ELb0ELb1ELb0E": ("smooth_post1", 4 * 4 + 3 * 2 + 1 + 4 / 8),
+    "k_vdir_mvIfDF16": ("matvec", bytes_per_cell("matvec", 4, 2)),
+    "k_vfresid_restrictIfDF16": ("restrict", bytes_per_cell("restrict", 4, 2)),
+    "k_step_fields<double>": ("step_fields", bytes_per_cell("step_fields", 4, 2)),
+    # the fast path's k_perm also writes FP16 faces and an FP32 invD
+    "k_perm<2>": ("perm", bytes_per_cell("perm", 4, 2) + 3 * 2 + 4),
+    "k_vtrue_resid<2>": ("true_resid", bytes_per_cell("true_resid", 4, 2)),
+    "k_vgramIfDF16": ("gram", 27.0),
+    "k_vproj_updateIfDF16": ("proj_update", 51.0),
+    "k_coarsen_pairsIfDF16": ("coarsen_pairs", 9.5),
+}
+
+
+def traffic_table(dir_read, dir_write, grid=256):
+    """HBM bytes per launch of each kernel from the counter CSVs of two
+    rocprofv3 --pmc runs (READ_COUNTERS, then WRITE_COUNTERS): read bytes =
+    32 n32 + 64 n64 + 128 n128 and write bytes = 32 (n - n64) + 64 n64 over
+    the TCC/EA request counts by size.  (On this part FETCH_SIZE counts
+    128-byte reads through TCC_BUBBLE, which reads 0, so FETCH_SIZE
+    undercounts them; the request-size counters reproduce the stream
+    probes' known bytes exactly.)  Prints a markdown table and writes
+    TRAFFIC_FILE."""
+    import csv
+    from collections import defaultdict
+
+    def load(d):
+        per = defaultdict(lambda: defaultdict(float))
+        n = defaultdict(set)
+        with open(os.path.join(d, "run_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                key = next((v[0] for k, v in TRAFFIC_KERNELS.items() if k in r["Kernel_Name"]), None)
+                if key is None:
+                    continue
+                per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+                n[key].add(r["Dispatch_Id"])
+        return per, {k: len(v) for k, v in n.items()}
+
+    rd, nr = load(dir_read)
+    wr, nw = load(dir_write)
+    cells = grid ** 3
+    out = {"grid": grid, "kernels": {}}
+    print("| kernel | launches | read MB | written MB | B/cell | algorithmic B/cell | ratio |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for key, alg in TRAFFIC_KERNELS.values():
+        if key not in rd or key not in wr:
+            continue
+        a, b = rd[key], wr[key]
+        rbytes = (32 * a["TCC_EA0_RDREQ_32B_sum"] + 64 * a["TCC_EA0_RDREQ_64B_sum"]
+                  + 128 * a["TCC_EA0_RDREQ_128B_sum"]) / nr[key]
+        wbytes = (32 * (b["TCC_EA0_WRREQ_sum"] - b["TCC_EA0_WRREQ_64B_sum"])
+                  + 64 * b["TCC_EA0_WRREQ_64B_sum"]) / nw[key]
+        out["kernels"][key] = {"bytes_per_launch": rbytes + wbytes, "read": rbytes, "written": wbytes,
+                               "launches": nr[key], "algorithmic_bytes_per_cell": alg}
+        per_cell = (rbytes + wbytes) / cells
+        print(f"| {key} | {nr[key]} | {rbytes / 1e6:.1f} | {wbytes / 1e6:.1f} | {per_cell:.2f} | {alg:.2f} "
+              f"| {per_cell / alg:.3f} |")
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
 
 
 def measured_traffic(kernel: str, grid: int):
@@ -202,8 +270,7 @@ def gpu_one_step(case, state, opts):
 
 def calibrate_copy(reps=20):
     """A plain 1 GiB device-to-device copy (torch's copy kernel), timed with
-    events: the reference point for HBM bandwidth and for scaling the
-    FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run."""
+    events: a reference point for HBM bandwidth beside the streaming probe."""
     import torch
     n = (1 << 30) // 4
     x = torch.ones(n, device="cuda", dtype=torch.float32)
@@ -341,12 +408,17 @@ def main():
                          "solver iterations")
     ap.add_argument("--calibrate-streams", action="store_true",
                     help="streaming bandwidth of kernels that read k and write m arrays of 64 MiB")
+    ap.add_argument("--traffic-table", nargs=2, metavar=("READ_DIR", "WRITE_DIR"),
+                    help="per-kernel HBM bytes from two rocprofv3 --pmc output folders")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 1 GiB copy kernel and exit")
     ap.add_argument("--ensemble", action="store_true",
                     help="the 48-grid ensemble workload instead of the 256^3 grid")
     ap.add_argument("--ens-shrink", type=int, default=1, help="divide ensemble grid sizes by this")
     a = ap.parse_args()
+    if a.traffic_table:
+        traffic_table(*a.traffic_table, grid=a.grid)
+        return
     if a.calibrate_streams:
         calibrate_streams()
         return

@@ -190,7 +190,7 @@ def run_path(case, opts, warmup, steps, run_steps, profile_kernel=None, net=None
     if net is not None:
         g.attach_net(net)
     total = max(run_steps, warmup + steps)
-    step_s, its_window, acc, prof, state = [], 0, 0, None, None
+    step_s, its_window, stage_steps, prof, state = [], 0, [], None, None
     wall = 0.0
     for s in range(total):
         if s == warmup:
@@ -203,7 +203,8 @@ def run_path(case, opts, warmup, steps, run_steps, profile_kernel=None, net=None
         t0 = time.perf_counter()
         rec = g.step()
         step_s.append(time.perf_counter() - t0)
-        acc += rec["learned"]
+        if rec["learned"]:
+            stage_steps.append(s + 1)
         if warmup <= s < warmup + steps:
             its_window += rec["iterations"]
         if s == warmup + steps - 1:
@@ -214,7 +215,7 @@ def run_path(case, opts, warmup, steps, run_steps, profile_kernel=None, net=None
         if (s + 1) % 50 == 0 or s + 1 == total:
             log(f"  step {s + 1}/{total}: its={rec['iterations']} outer={rec['outer']} "
                 f"dt={rec['dtime']:.3e} ({sum(step_s[-50:]) / len(step_s[-50:]) * 1e3:.2f} ms/step)")
-    g.learned_accepted = acc
+    g.stage_steps = stage_steps
     return g, wall, its_window, state, prof, step_s
 
 
@@ -476,7 +477,8 @@ def main():
     gf, wallf, itsf, state0, prof, stepsf = run_path(case, fast_opts, a.warmup, a.steps, a.run_steps,
                                                      a.kernel, net)
     pf = gf.download("pressure")
-    accepted = gf.learned_accepted
+    stage_steps = gf.stage_steps
+    stage_timed = [t for t in stage_steps if a.warmup < t <= a.warmup + a.steps]
     gf.close()
     perr = float(np.abs(pf - p64).max() / np.abs(p64).max())
 
@@ -549,9 +551,16 @@ def main():
         "rel_err_vs_fp64_oracle_one_step": vs_oracle,
         "cell_updates_per_s": sps * ncell,
         "iterations_per_step": {"plain_fp64": its64 / a.steps, "fast_fp32_inner": itsf / a.steps},
-        "learned_stage": ({"weights": os.path.relpath(weights, ROOT), "accepted_steps": accepted,
-                           "contribution": "none: the gate found no iteration saving and kept the stage off"
-                           if accepted == 0 else f"accepted in {accepted} solves"}
+        "learned_stage": ({"weights": os.path.relpath(weights, ROOT),
+                           "solves_with_stage": len(stage_steps),
+                           "timed_solves_with_stage": len(stage_timed),
+                           "steps_with_stage": stage_steps[:40],
+                           "gate": "the stage runs in the first and third of every 100 solves (probes, against "
+                                   "the second and fourth without it) and otherwise only while its mean "
+                                   "iteration count over the probes is at least 3% below the other's",
+                           "contribution": ("none in the timed steps: the stage did not run there"
+                                            if not stage_timed else
+                                            f"ran in {len(stage_timed)} of the {a.steps} timed solves")}
                           if net else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -385,7 +385,9 @@ __global__ void __launch_bounds__(256) k_phase_deriv(Dims d, PhaseConsts K,
         const double vym = (j > 0) ? vy[n.c - d.nx] : -0.0;
         const double vzm = (k > 0) ? vz[n.c - d.plane] : -0.0;
         const bool well = (mask[n.c] & WPF_MASK_WELL) != 0;
-        dphidt[n.c] = phase_cell(K, s, conc[n.c], vx[n.c], vxm, vy[n.c], vym, vz[n.c], vzm, well, &tau);
+        double kxx;
+        dphidt[n.c] = phase_cell(K, s, conc[n.c], vx[n.c], vxm, vy[n.c], vym, vz[n.c], vzm, well, &kxx);
+        tau = tau_min_of(K, kxx, tau);
     }
     block_min_store(tau, mins, blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
@@ -656,7 +658,7 @@ __device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int
         if (k0 > 0) tzm = tz[i + nx * j + pl * (k0 - 1)];
     }
     const int e0 = (threadIdx.y + 1) * LX + threadIdx.x + 1;
-    T tau = (T)1.0e6;
+    T kext = kxx_extreme_init(K);  // the kxx of this thread's smallest tau
     T vz_prev = (T)0;  // +z face velocity of the plane below (= this plane's -z face)
     for (int k = k0; k < k1; k++) {
         stage(k + 1);
@@ -694,9 +696,9 @@ __device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int
             for (int q = 0; q < 19; q++) st[q] = (T)s64[q];
             const bool well = (cur.m & WPF_MASK_WELL) != 0;
             const double c0 = scon[s0][e0];
-            T tc = (T)1.0e6;
-            const T dph = phase_cell<T>(K, st, (T)c0, vxp, vxm, vyp, vym, vzp, vzm, well, &tc);
-            tau = fmin(tau, tc);
+            T kc;
+            const T dph = phase_cell<T>(K, st, (T)c0, vxp, vxm, vyp, vym, vzp, vzm, well, &kc);
+            kext = kxx_extreme(K, kext, kc);
             T dcd = (T)0;
             if (!well) {
                 T cc[7] = {(T)c0, (T)scon[s0][e0 + 1], (T)scon[s0][e0 - 1], (T)scon[s0][e0 + LX],
@@ -710,6 +712,8 @@ __device__ __forceinline__ void t_step_fields(const Dims& d, int bx, int by, int
         }
         cur = nxt;
     }
+    // every thread in the grid evaluated at least one cell (k0 < k1)
+    const T tau = in ? tau_min_of(K, kext, (T)1.0e6) : (T)1.0e6;
     block_min_store((double)tau, mins, bid);
     if (bx == 0 && by == 0 && bz == 0 && tid == 0) {
         // key_next was last read two steps ago: reset it for the next step

@@ -13,6 +13,8 @@
 // [0, pi].  That replaces five trigonometric and two inverse trigonometric
 // FP64 library calls by one square root and one division per cell and moves
 // dphasedt by a few ulp; parity with the literal evaluation is tested.
+// Quotients by the same denominator share one reciprocal (|grad phi|, and
+// tau through 1/tau = kxx / tau_num), also within a few ulp.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,7 +31,7 @@ struct PhaseConstsT {
     // division costs about ten instructions)
     T inv_dx, inv_two_dx, inv_dx2, inv_four_dx2;
     T w0, maxdphi, eps, lambda, diffusion, ral, rat, tau_num, alpha;
-    T kwh, inv_mu_unused;
+    T kwh, inv_tau_num;
 };
 using PhaseConsts = PhaseConstsT<double>;
 
@@ -56,7 +58,7 @@ inline PhaseConsts phase_consts(const wpf_params& p)
     K.tau_num = 5.0 / 3.0 * p.alpha * p.lambda * (p.epsilon * p.epsilon);
     K.alpha = p.alpha;
     K.kwh = p.permeab_hole;
-    K.inv_mu_unused = 0.0;
+    K.inv_tau_num = 1.0 / K.tau_num;
     return K;
 }
 
@@ -70,7 +72,7 @@ inline PhaseConstsT<T> consts_as(const PhaseConsts& K)
     k.inv_four_dx2 = (T)K.inv_four_dx2;
     k.w0 = (T)K.w0; k.maxdphi = (T)K.maxdphi; k.eps = (T)K.eps; k.lambda = (T)K.lambda;
     k.diffusion = (T)K.diffusion; k.ral = (T)K.ral; k.rat = (T)K.rat; k.tau_num = (T)K.tau_num;
-    k.alpha = (T)K.alpha; k.kwh = (T)K.kwh; k.inv_mu_unused = (T)K.inv_mu_unused;
+    k.alpha = (T)K.alpha; k.kwh = (T)K.kwh; k.inv_tau_num = (T)K.inv_tau_num;
     return k;
 }
 
@@ -82,18 +84,27 @@ template <> __device__ __forceinline__ float atan_big<float>() { return 1.0e18f;
 // s: 0 c, 1 x+, 2 x-, 3 y+, 4 y-, 5 z+, 6 z-, 7 (x+,y+) 8 (x+,y-)
 // 9 (x-,y+) 10 (x-,y-), 11 (y+,z+) 12 (y-,z+) 13 (y+,z-) 14 (y-,z-),
 // 15 (x+,z+) 16 (x-,z+) 17 (x+,z-) 18 (x-,z-).
-// Returns dphasedt (0 at wells) and tau through *tau.
+// Returns dphasedt (0 at wells) and, through *kxx_out, the cell's
+// dispersion coefficient kxx, whose tau is tau_num / kxx.  The caller forms
+// taumin from the extreme kxx (tau_min_of): x -> fl(tau_num / x) is
+// monotone, so that is bitwise the minimum of the cells' taus with one
+// division per thread instead of one per cell.  dphasedt = total / tau is
+// evaluated as total * (kxx / tau_num), within a few ulp of the quotient.
 template <typename T>
 __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T conc, T vxp, T vxm,
-                                        T vyp, T vym, T vzp, T vzm, bool is_well, T* tau_out)
+                                        T vyp, T vym, T vzp, T vzm, bool is_well, T* kxx_out)
 {
     const T gx = (s[1] - s[2]) * K.inv_two_dx;
     const T gy = (s[3] - s[4]) * K.inv_two_dx;
     const T gz = (s[5] - s[6]) * K.inv_two_dx;
     const T dpf = sqrt(gx * gx + gy * gy + gz * gz);
+    // one reciprocal for the quotients by |grad phi|: the normal (where the
+    // reference divides by dpf; dpf + 1e-20 rounds to dpf above 1e-4 and
+    // differs by at most 1e-14 relative above the 1e-6 cut) and cos(phi_)
+    const T rdp = T(1.0) / (dpf + T(1.0e-20));
     T vs = T(0.0);
     if (dpf > T(1.0e-6)) {
-        const T n1 = gx / dpf, n2 = gy / dpf, n3 = gz / dpf;
+        const T n1 = gx * rdp, n2 = gy * rdp, n3 = gz * rdp;
         const T vxc = T(0.5) * (vxp + vxm);
         const T vyc = T(0.5) * (vyp + vym);
         const T vzc = T(0.5) * (vzp + vzm);
@@ -106,9 +117,9 @@ __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T 
     else
         kxx = K.diffusion + fabs(conc) * K.eps * (K.ral * (vs * vs) + K.rat * (vt * vt))
                                 / sqrt(vs * vs + vt * vt);
-    const T tau = K.tau_num / kxx;
-    *tau_out = tau;
+    *kxx_out = kxx;
     if (is_well) return T(0.0);
+    const T rtau = kxx * K.inv_tau_num;  // 1 / tau
 
     const T phix = gx, phiy = gy, phiz = gz;
     const T phixx = (s[1] - T(2.0) * s[0] + s[2]) * K.inv_dx2;
@@ -131,7 +142,7 @@ __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T 
         c = T(0.0);
         sn = u > T(0.0) ? T(1.0) : -T(1.0);
     }
-    T v = phiz / (dpf + T(1.0e-20));
+    T v = phiz * rdp;
     v = fmin(fmax(v, -T(1.0)), T(1.0));
     const T sphi = sqrt((T(1.0) - v) * (T(1.0) + v));
     const T c2p = T(2.0) * (v * v) - T(1.0);          // cos(2 phi_)
@@ -153,7 +164,7 @@ __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T 
     if (!gate_p) {
         // f2, f4, f5, f6 vanish exactly when no gate fires (their factors
         // dtheta/dx.., dphi/dx.., dtheta/d(dphi/dx).. are exact zeros).
-        return (f1 + f3) / tau;
+        return (f1 + f3) * rtau;
     }
 
     const T cos2t = c2 - s2;  // cos(2 theta)
@@ -185,7 +196,7 @@ __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T 
     const T dthetaddpdz = T(0.0);
     const T dphi_ddpdx = phix * phiz / (rxy * dpf + T(1.0e-20));
     const T dphi_ddpdy = phiy * phiz / (rxy * dpf + T(1.0e-20));
-    const T dphi_ddpdz = -(rxy / (dpf + T(1.0e-20)));
+    const T dphi_ddpdz = -(rxy * rdp);
 
     const T dwdx = dwdtheta * dthetadx + dwdphi_ * dphi_dx;
     const T dwdy = dwdtheta * dthetady + dwdphi_ * dphi_dy;
@@ -206,7 +217,27 @@ __device__ __forceinline__ T phase_cell(const PhaseConstsT<T>& K, const T* s, T 
     const T f6 = dwdz * (dwdtheta * dthetaddpdz * dpf2 + dwdphi_ * dphi_ddpdz * dpf2)
                     + w * (ddwdthetaz * dthetaddpdz * dpf2 + ddwdphi_z * dphi_ddpdz * dpf2);
     const T f7 = T(0.0);
-    return (f1 + f2 + f3 + f4 + f5 + f6 + f7) / tau;
+    return (f1 + f2 + f3 + f4 + f5 + f6 + f7) * rtau;
+}
+
+// The running extreme of kxx over a thread's cells that gives the minimum
+// tau: the largest kxx when tau_num >= 0, else the smallest.
+template <typename T>
+__device__ __forceinline__ T kxx_extreme_init(const PhaseConstsT<T>& K)
+{
+    return K.tau_num >= T(0.0) ? T(0.0) : T(INFINITY);
+}
+template <typename T>
+__device__ __forceinline__ T kxx_extreme(const PhaseConstsT<T>& K, T ext, T kxx)
+{
+    return K.tau_num >= T(0.0) ? fmax(ext, kxx) : fmin(ext, kxx);
+}
+// min(cap, tau of the extreme kxx) for a thread that evaluated at least
+// one cell
+template <typename T>
+__device__ __forceinline__ T tau_min_of(const PhaseConstsT<T>& K, T ext, T cap)
+{
+    return fmin(cap, K.tau_num / ext);
 }
 
 // calcConcentDeriv for one non-well cell, wPFAW3D.f90:1566-1622.

@@ -556,8 +556,8 @@ def main():
                            "timed_solves_with_stage": len(stage_timed),
                            "steps_with_stage": stage_steps[:40],
                            "gate": "the stage runs in the first and third of every 100 solves (probes, against "
-                                   "the second and fourth without it) and otherwise only while its mean "
-                                   "iteration count over the probes is at least 3% below the other's",
+                                   "the second and fourth without it) and otherwise only while the mean solve "
+                                   "time with it over the probes is at least 3% below the other's",
                            "contribution": ("none in the timed steps: the stage did not run there"
                                             if not stage_timed else
                                             f"ran in {len(stage_timed)} of the {a.steps} timed solves")}

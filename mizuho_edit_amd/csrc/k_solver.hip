@@ -297,13 +297,13 @@ template <typename R, typename C>
 static int dir_mv(wpf_ctx* c, const Fine<R, C>& F, const R* z, const R* pold, R* pnew, int num, int den,
                   int first, int slot = S_PQ);
 
-// The CG update fused with the first pre-smoothing step (k_vupsweep): x +=
-// alpha p, r2 = r - alpha q, y (into F.T) from z0 = w1 invD r2; alpha =
-// result[num] / result[den].  The y-neighbour rows are staged in LDS where
-// the rows fit.
+// The CG update fused with the first pre-smoothing step (k_vupsweep):
+// x_out = x + alpha p (x_out may be x), r2 = r - alpha q, y (into F.T) from
+// z0 = w1 invD r2; alpha = result[num] / result[den].  The y-neighbour rows
+// are staged in LDS where the rows fit.
 template <typename R, typename X, typename C>
-static int upsweep(wpf_ctx* c, const Fine<R, C>& F, int num, int den, R w1, double a1, double w1s, X* x,
-                   const R* p)
+static int upsweep(wpf_ctx* c, const Fine<R, C>& F, int num, int den, R w1, double a1, double w1s, const X* x,
+                   X* x_out, const R* p)
 {
     constexpr int V = vwidth<R>();
     const bool lds = vupsweep_lds_fits<R, V>(F.g.nx);
@@ -311,7 +311,7 @@ static int upsweep(wpf_ctx* c, const Fine<R, C>& F, int num, int den, R w1, doub
     auto* const kern = lds ? k_vupsweep<R, X, C, V, true> : k_vupsweep<R, X, C, V, false>;
     prof_mark(c, WPF_KERNEL_CG_UPDATE, true);
     hipLaunchKernelGGL(kern, dim3(vz_blocks(F.V)), dim3(256), lds_bytes, c->stream, F.V,
-                       (const double*)c->red.result, num, den, w1, (R)a1, (R)w1s, x, p, (const R*)F.r,
+                       (const double*)c->red.result, num, den, w1, (R)a1, (R)w1s, x, x_out, p, (const R*)F.r,
                        (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, F.T, c->red.partials);
     prof_mark(c, WPF_KERNEL_CG_UPDATE, false);
     WPF_HIP(hipGetLastError());
@@ -349,7 +349,7 @@ static int tune_shape(wpf_ctx* c, Fine<R, C>& F, int* zc)
     auto round = [&]() {
         int r = sweep<R, C, false, false, false>(c, F, 0.0, 0.5, F.Z, nullptr, none, F.T, S_T0);
         if (!r) r = sweep<R, C, false, false, false>(c, F, 0.0, 0.5, F.T, nullptr, none, F.Z, S_T0);
-        if (!r) r = upsweep<R, R, C>(c, F, kTuneSlot, kTuneSlot, (R)0.5, 0.0, 0.5, F.pa, F.pb);
+        if (!r) r = upsweep<R, R, C>(c, F, kTuneSlot, kTuneSlot, (R)0.5, 0.0, 0.5, F.pa, F.pa, F.pb);
         if (!r) r = dir_mv<R, C>(c, F, F.Z, F.pa, F.pb, kTuneSlot, kTuneSlot, 1);
         return r;
     };
@@ -497,12 +497,27 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>
 // before the host waits for k, so the GPU never idles on the host's test.
 // The kernels and their order are the same as without lookahead; when the
 // prediction fails, the solve ends one iteration later (a smaller residual).
+//
+// x_alt (fused update only): the iterate alternates between x and x_alt,
+// each update reading one and writing the other, and lookahead is on.  A
+// failed prediction then costs nothing in the answer: the solve returns
+// the iterate of the iteration that met tol, with its iteration count and
+// residual, exactly as without lookahead (the speculative iteration's
+// other writes are scratch), and *x_final names the buffer that holds it.
 template <typename R, typename X, typename C>
 static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F0, const MGOpts& o,
-                    X* x, double tol, int maxit, int* its_out, double* rn_out, bool lookahead = false)
+                    X* x, double tol, int maxit, int* its_out, double* rn_out, bool lookahead = false,
+                    X* x_alt = nullptr, X** x_final = nullptr)
 {
     Fine<R, C> F = F0;  // r and r2 trade places at every fused update
     const bool fuse = mg && F.vec && o.sweeps >= 2 && F.r2 != nullptr;
+    // the iterate's buffers: xs[xi] is current; an exact lookahead needs the
+    // fused update (the unfused one updates x in place)
+    const bool exact = x_alt != nullptr && fuse;
+    if (exact) lookahead = true;
+    X* xs[2] = {x, exact ? x_alt : x};
+    int xi = 0;
+    if (x_final) *x_final = x;
     const int64_t n = F.g.n;
     const R w = mg ? (R)o.w1 : (R)1;
     int rz_cur = S_RZ0, rz_prev = S_RZ1;
@@ -534,7 +549,8 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
             // update + pre-smoothing step 1 (into T) in one pass
             double a1, w1s;
             step_coef(o, 1, &a1, &w1s);
-            if (int rc = upsweep<R, X, C>(c, F, rz_cur, S_PQ, w, a1, w1s, x, pc)) return rc;
+            if (int rc = upsweep<R, X, C>(c, F, rz_cur, S_PQ, w, a1, w1s, xs[xi], xs[xi ^ 1], pc)) return rc;
+            if (exact) xi ^= 1;
             FINALIZE(vz_blocks(F.V), 2, S_RN);
             std::swap(F.r, F.r2);
         } else {
@@ -594,7 +610,11 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
             if (int rc = norm_of(k, &rk)) return rc;
             if (rk <= tol || (!ahead && its >= maxit)) {
                 rn = rk;
-                if (ahead) {
+                if (ahead && exact) {
+                    // iteration k met tol: drop the speculative k+1
+                    its = k;
+                    xi ^= 1;
+                } else if (ahead) {
                     if (int rc = norm_of(its, &rn)) return rc;
                 }
                 break;
@@ -614,6 +634,7 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
     }
     *its_out = its;
     *rn_out = rn;
+    if (x_final) *x_final = xs[xi];
     return WPF_OK;
 }
 
@@ -819,7 +840,7 @@ static int ensure_ws(wpf_ctx* c, bool f32)
         if ((rc = alloc_level(c->mg32[0], c->g))) return rc;
         if ((rc = dev_alloc(c->r32, c->g)) || (rc = dev_alloc(c->r32b, c->g)) || (rc = dev_alloc(c->z32, c->g)) ||
             (rc = dev_alloc(c->p32, c->g)) || (rc = dev_alloc(c->q32, c->g)) ||
-            (rc = dev_alloc(c->x32, c->g)) || (rc = dev_alloc(c->t32, c->g)) ||
+            (rc = dev_alloc(c->x32, c->g)) || (rc = dev_alloc(c->x32b, c->g)) || (rc = dev_alloc(c->t32, c->g)) ||
             (rc = dev_alloc(c->p32b, c->g)))
             return rc;
     }
@@ -961,11 +982,13 @@ static int start_stage(wpf_ctx* c, const wpf_solver_opts& o, int* learned, bool 
     return WPF_OK;
 }
 
-// The learned stage pays only if the solves that use it need fewer inner
-// iterations than those that do not.  The first kGateProbe solves alternate
-// with and without it; afterwards it is used while its mean count stays 3%
-// below the other's, and every kGatePeriod solves the two are probed again
-// (kGateProbe solves, fresh means), so a change of regime is followed.
+// The learned stage pays only if the solves that use it take less time
+// than those that do not, the stage's own network pass and line search
+// included (fewer inner iterations alone do not pay for it).  The first
+// kGateProbe solves alternate with and without it; afterwards it is used
+// while its mean solve time stays 3% below the other's, and every
+// kGatePeriod solves the two are probed again (kGateProbe solves, fresh
+// means), so a change of regime is followed.
 static constexpr int kGateProbe = 4, kGatePeriod = 100;
 
 static bool gate_use(wpf_ctx* c)
@@ -973,22 +996,24 @@ static bool gate_use(wpf_ctx* c)
     const int k = c->gate_calls % kGatePeriod;
     if (k == 0) {
         c->gate_on_n = c->gate_off_n = 0;
-        c->gate_on_its = c->gate_off_its = 0.0;
+        c->gate_on_s = c->gate_off_s = 0.0;
     }
     if (k < kGateProbe) return (k & 1) == 0;
     if (c->gate_on_n == 0 || c->gate_off_n == 0) return true;
-    return c->gate_on_its / c->gate_on_n < 0.97 * (c->gate_off_its / c->gate_off_n);
+    return c->gate_on_s / c->gate_on_n < 0.97 * (c->gate_off_s / c->gate_off_n);
 }
 
-static void gate_record(wpf_ctx* c, bool used, int its)
+// seconds: the solve's wall time (it ends on a host read of its last FP64
+// residual, so every kernel it enqueued has finished)
+static void gate_record(wpf_ctx* c, bool used, double seconds)
 {
     if (c->gate_calls % kGatePeriod < kGateProbe) {
         if (used) {
             c->gate_on_n++;
-            c->gate_on_its += its;
+            c->gate_on_s += seconds;
         } else {
             c->gate_off_n++;
-            c->gate_off_its += its;
+            c->gate_off_s += seconds;
         }
     }
     c->gate_calls++;
@@ -1160,9 +1185,8 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
             rc = fcg_learned(c, F, x, rn, tol, o.maxit - its, &k, &rr, &stages, capture);
             learned += stages;
         } else {
-            // lookahead on the FP64 solvers (the mixed solver's inner
-            // iterations stay host-tested one by one: an ensemble member
-            // repeats that arithmetic bit for bit)
+            // lookahead; a failed prediction ends the solve one iteration
+            // later (the mixed solver's inner solves use the exact form)
             rc = pcg_core<double, double, double>(c, mg ? &c->mg64 : nullptr, F, mo, x, tol, o.maxit - its,
                                                   &k, &rr, true);
         }
@@ -1188,6 +1212,7 @@ static int pcg64(wpf_ctx* c, const wpf_solver_opts& o, bool mg, wpf_solve_stats*
 // FP32 MG-PCG inside FP64 iterative refinement.
 static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool cont = false)
 {
+    const auto t_start = std::chrono::steady_clock::now();
     const int64_t n = c->g.n;
     int rc = ensure_ws(c, true);
     if (rc) return rc;
@@ -1326,13 +1351,20 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         double rr = 0.0;
         const int cap = std::min(o.inner_maxit, o.maxit - its);
         join.now();
-        rc = op16 ? pcg_core<float, float, _Float16>(c, &c->mg32, H, mo, e, inner_tol, cap, &k, &rr)
-                  : pcg_core<float, float, float>(c, &c->mg32, F, mo, e, inner_tol, cap, &k, &rr);
+        // exact lookahead: the correction alternates between x32 and x32b,
+        // so the answer and the iteration count are those of a solve tested
+        // iteration by iteration (an ensemble member repeats them bit for bit)
+        float* ef = e;
+        rc = op16 ? pcg_core<float, float, _Float16>(c, &c->mg32, H, mo, e, inner_tol, cap, &k, &rr, true,
+                                                     c->x32b.p, &ef)
+                  : pcg_core<float, float, float>(c, &c->mg32, F, mo, e, inner_tol, cap, &k, &rr, true, c->x32b.p,
+                                                  &ef);
         if (rc) return rc;
         its += k;
-        LAUNCH1D(k_add_f32, n, n, (const float*)e, x);
+        LAUNCH1D(k_add_f32, n, n, (const float*)ef, x);
     }
-    if (gated) gate_record(c, use_net, its);
+    if (gated)
+        gate_record(c, use_net, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count());
     st->iterations = its;
     st->outer = outer;
     st->rnorm0 = rn0;
@@ -1446,7 +1478,7 @@ void free_solver(wpf_ctx* c)
     for (size_t l = 0; l < c->mg32.size(); l++) free_level(c->mg32[l]);
     c->mg32.clear();
     dev_free(c->r32); dev_free(c->r32b); dev_free(c->rb); dev_free(c->z32); dev_free(c->p32); dev_free(c->q32); dev_free(c->x32);
-    dev_free(c->t32); dev_free(c->p32b);
+    dev_free(c->t32); dev_free(c->p32b); dev_free(c->x32b);
     dev_free(c->h_tx); dev_free(c->h_ty); dev_free(c->h_tz);
     for (int j = 0; j < wpf_ctx::kProjDepth; j++) dev_free(c->proj[j]);
     c->proj_count = c->proj_next = 0;

@@ -248,7 +248,9 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
 }
 
 // The CG update fused with the first V-cycle step that follows it:
-//     e += alpha p,  r' = r - alpha q  (written to r2, not over r, which the
+//     e_out = e + alpha p  (e_out may be e itself, or a second buffer so
+//     that the iterate before the update survives it),
+//     r' = r - alpha q  (written to r2, not over r, which the
 //     neighbouring threads still read),  z0 = w1 (r' invD)  (never stored),
 //     y = z0 + a z0 + w invD (r' - A z0)  (the second pre-smoothing step,
 //     from x_{-1} = 0, with z0 formed on the fly for the neighbours too).
@@ -262,7 +264,7 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
 // the result is the same.
 template <typename R, typename X, typename C, int V, bool LDSY = false>
 __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __restrict__ res, int num,
-                                           int den, R w1, R a, R w, X* __restrict__ e,
+                                           int den, R w1, R a, R w, const X* e, X* e_out,
                                            const R* __restrict__ p, const R* __restrict__ r,
                                            const R* __restrict__ q, R* __restrict__ r2,
                                            const C* __restrict__ tx, const C* __restrict__ ty,
@@ -356,7 +358,7 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
             yv[h] = v;
         }
         if (act) {
-            vst<V>(e + c, ev);
+            vst<V>(e_out + c, ev);
             vst<V>(r2 + c, r0);
             vst<V>(y + c, yv);
         }
@@ -377,7 +379,7 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
 
 template <typename R, typename X, typename C, int V, bool LDSY>
 __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict__ res, int num, int den, R w1,
-                                                  R a, R w, X* __restrict__ e, const R* __restrict__ p,
+                                                  R a, R w, const X* e, X* e_out, const R* __restrict__ p,
                                                   const R* __restrict__ r, const R* __restrict__ q,
                                                   R* __restrict__ r2, const C* __restrict__ tx,
                                                   const C* __restrict__ ty, const C* __restrict__ tz,
@@ -386,8 +388,8 @@ __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict
 {
     double acc[2];
     extern __shared__ char smem_raw[];
-    t_vupsweep<R, X, C, V, LDSY>(Z, blockIdx.x, res, num, den, w1, a, w, e, p, r, q, r2, tx, ty, tz, invD, y, acc,
-                                 reinterpret_cast<R*>(smem_raw));
+    t_vupsweep<R, X, C, V, LDSY>(Z, blockIdx.x, res, num, den, w1, a, w, e, e_out, p, r, q, r2, tx, ty, tz, invD, y,
+                                 acc, reinterpret_cast<R*>(smem_raw));
     block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
 }
 

@@ -146,6 +146,8 @@ struct wpf_ctx {
     std::vector<wpf::MGLevel<float>> mg32;    // levels 0.. (FP32 copy incl. fine)
     wpf::Dev<float> r32, z32, p32, q32, x32, t32, p32b;
     wpf::Dev<float> r32b;                     // second FP32 residual (fused update)
+    wpf::Dev<float> x32b;                     // second FP32 correction (the inner
+                                              // solve's exact lookahead)
     // the last solution changes p_n - p_(n-1) (FP32 ring) for the projected
     // initial guess (WPF_FLAG_PROJECT); four measured best at 256^3 (three,
     // five and six cost more steps or more per step)
@@ -159,7 +161,7 @@ struct wpf_ctx {
     // probe of the learned stage: solves with and without it, compared by
     // their inner iteration counts
     int gate_calls = 0, gate_on_n = 0, gate_off_n = 0;
-    double gate_on_its = 0.0, gate_off_its = 0.0;
+    double gate_on_s = 0.0, gate_off_s = 0.0;  // summed solve seconds
     int tuned_zc[2] = {0, 0};                 // planes per tile of the vector fine
                                               // kernels (FP32, FP64); 0 = not tuned
     wpf::Dev<double> z2, pp2, psave, fsrc;    // second buffers, saved p, injection source

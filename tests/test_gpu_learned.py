@@ -73,7 +73,7 @@ def test_first_stage_gate_keeps_the_answer_and_the_cost(gpu):
     """The learned first stage of the mixed solver (cosine loss on the
     solver's own corrections, two 64^3 realisations), on a held-out
     realisation: the FP64 answer is unchanged, and the gate, which runs the
-    stage only while it saves iterations, keeps the iteration total within
+    stage only while it saves solve time, keeps the iteration total within
     5% of the solve without it (the stage gains nothing measurable here:
     the multigrid V-cycle already removes the smooth error it predicts)."""
     samples = []

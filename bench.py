@@ -69,7 +69,9 @@ def bytes_per_cell(kernel: str, r: int, h: int, cheb: bool = True, launches: int
                    its: int = 1) -> float:
     return {
         "smooth": smoother_bytes(r, h, cheb, launches, its),
-        "matvec": 4 * r + 3 * h + 1,  # z, p_old, 3 faces, mask -> p_new, q
+        # z, p_old, 3 faces, mask -> p_new, q (the CG operator: with FP16
+        # preconditioner faces, h = 2, the matvec still reads FP32 faces)
+        "matvec": 4 * r + 3 * max(h, r) + 1,
         # CG update fused with the first pre-smoothing step:
         # e, p, r, q, invD, 3 faces -> e, r', y
         "cg_update": 8 * r + 3 * h,
@@ -88,22 +90,24 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_256.json")
 READ_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")
 WRITE_COUNTERS = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
 # kernels of the fast path reported in the table: name fragment -> (key,
-# algorithmic bytes per cell with FP32 vectors and FP16 faces: the formulas
-# of bytes_per_cell; post1/post2 are smoother_bytes' two post-smoothing
-# launches; Gram = 4 changes + their 4 stencil products' faces and r;
-# projection update = x, r, 4 changes, faces, mask -> x, r, y)
+# algorithmic bytes per cell with FP32 vectors, the preconditioner's FP16
+# faces and the CG operator's FP32 faces: the formulas of bytes_per_cell;
+# post1/post2 are smoother_bytes' two post-smoothing launches; Gram = 4
+# changes, r and the FP32 faces of their stencil products; projection update
+# = x (FP64), r, 4 changes, FP32 faces, invD, mask -> x, r)
 TRAFFIC_KERNELS = {
     "k_vupsweepIffDF16": ("cg_update", bytes_per_cell("cg_update", 4, 2)),
     "k_vsweepIfDF16_Li4ELb1ELb0ELb1E": ("smooth_post2", 5 * 4 + 3 * 2 + 1 + 4 / 8),
     "k_vsweepIfDF16_Li4ELb0ELb1ELb0E": ("smooth_post1", 4 * 4 + 3 * 2 + 1 + 4 / 8),
-    "k_vdir_mvIfDF16": ("matvec", bytes_per_cell("matvec", 4, 2)),
+    "k_vdir_mvIffLi4E": ("matvec", bytes_per_cell("matvec", 4, 2)),
     "k_vfresid_restrictIfDF16": ("restrict", bytes_per_cell("restrict", 4, 2)),
     "k_step_fields<double>": ("step_fields", bytes_per_cell("step_fields", 4, 2)),
-    # the fast path's k_perm also writes FP16 faces and an FP32 invD
-    "k_perm<2>": ("perm", bytes_per_cell("perm", 4, 2) + 3 * 2 + 4),
+    # the fast path's k_perm also writes the FP32 faces, their FP16 copies
+    # and an FP32 invD
+    "k_perm<2>": ("perm", bytes_per_cell("perm", 4, 2) + 3 * 4 + 3 * 2 + 4),
     "k_vtrue_resid<2>": ("true_resid", bytes_per_cell("true_resid", 4, 2)),
-    "k_vgramIfDF16": ("gram", 27.0),
-    "k_vproj_updateIfDF16": ("proj_update", 51.0),
+    "k_vgramIffLi4ELi4E": ("gram", 33.0),
+    "k_vproj_updateIffLi4ELi4E": ("proj_update", 57.0),
     "k_coarsen_pairsIfDF16": ("coarsen_pairs", 9.5),
 }
 

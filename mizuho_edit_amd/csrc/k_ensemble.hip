@@ -826,10 +826,12 @@ int sched_solve(wpf_ensemble* e)
         if (stage == 0 && e->project) {
             constexpr int NV = kEnsProjDepth * (kEnsProjDepth + 1) / 2 + kEnsProjDepth;
             ELAUNCH(ke_proj_begin, S.mblocks(), S.mthreads(), 0, e->d_mem, nm);
-            ELAUNCH((ke_vgram<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
+            // the projection and CG's matvec apply the CG operator (FP32
+            // faces); C is the preconditioner's face type
+            ELAUNCH((ke_vgram<float>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
             ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, NV, (int)S_GRAM, (int)G_PROJ);
             ELAUNCH(ke_proj_solve, S.mblocks(), S.mthreads(), 0, e->d_mem, nm);
-            ELAUNCH((ke_vproj_update<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
+            ELAUNCH((ke_vproj_update<float>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
             ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, 1, (int)S_PN, (int)G_PROJ);
             ELAUNCH(ke_proj_tol, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.inner_rtol);
         }
@@ -842,7 +844,7 @@ int sched_solve(wpf_ensemble* e)
         ECHK(sched_vcycle<C>(S, rz_cur, &zres, rr, false));
         int pc = EV_PA, pn = EV_PB;
         for (int it = 0; it < inner_iters; it++) {
-            ELAUNCH((ke_dir_mv<C>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, zres, pc, pn, rz_cur, rz_prev,
+            ELAUNCH((ke_dir_mv<float>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, zres, pc, pn, rz_cur, rz_prev,
                     it == 0 ? 1 : 0);
             ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, 1, (int)S_PQ, (int)G_INNER);
             std::swap(pc, pn);

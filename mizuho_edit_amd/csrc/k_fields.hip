@@ -67,7 +67,7 @@ __device__ __forceinline__ T from_left_lane(T v)
 // ---------------------------------------------------------------------------
 // calcPermeability + operator set-up: the faces t = k_face/mu, the Jacobi
 // diagonal, and (when the mixed solver asks for it) its scaled FP32 operator
-// with FP16-stored faces, in one pass.
+// and the FP16-stored faces of its preconditioner, in one pass.
 //
 // Tiling as the fused step (64 x 4 cells, kStepZ planes up z).  Each plane's
 // 1/k0 is staged once in LDS with a one-cell border (the reference's
@@ -166,12 +166,16 @@ __device__ __forceinline__ void t_perm(const Dims& d, int bx, int by, int bz, do
                 const float D = a + (float)(txm * s) + b + (float)(tym * s) + f + (float)(tzm * s);
                 O.invD[c] = (id == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;
             } else if (MODE == PERM_F16) {
-                // FP16-stored faces h = half(float(t * s)); 1/D is formed from
-                // the decoded values (every FP32 consumer of these faces reads
-                // the FP16 copy, so no FP32 copy is written)
+                // the preconditioner's FP16-stored faces h = half(float(t * s))
+                // and 1/D formed from their decoded values, and the CG
+                // operator's FP32 faces float(t * s)
                 const double s = O.s;
+                const float a32 = (float)(txp * s), b32 = (float)(typ * s), f32 = (float)(tzp * s);
+                O.tx[c] = a32;
+                O.ty[c] = b32;
+                O.tz[c] = f32;
                 auto h = [&](double t) { return (_Float16)(float)(t * s); };
-                const _Float16 a = h(txp), b = h(typ), f = h(tzp);
+                const _Float16 a = (_Float16)a32, b = (_Float16)b32, f = (_Float16)f32;
                 O.htx[c] = a;
                 O.hty[c] = b;
                 O.htz[c] = f;

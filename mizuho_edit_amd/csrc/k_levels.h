@@ -491,10 +491,12 @@ static __global__ void __launch_bounds__(kThreads) k_to_f32(LV<double> L, double
     t_to_f32(L, s, tx, ty, tz, invD, (int)blockIdx.x, (int)gridDim.x);
 }
 
-// FP16 fine faces of the inner solve: h = half(float(t64 * s)).  The FP32
-// copy and invD hold exactly the values the FP16 faces decode to, so the
-// coarse levels are the Galerkin operators of the operator the fine level
-// applies.
+// Faces of the inner solve with an FP16 preconditioner: the CG operator's
+// FP32 faces float(t64 * s), and the preconditioner's FP16 faces h =
+// half(float(t64 * s)).  invD holds the inverse diagonal of the decoded
+// FP16 faces, so the smoother and the coarse levels (Galerkin operators of
+// the FP16 faces) form one preconditioner of the operator the FP16 fine
+// level applies.
 static __device__ __forceinline__ void t_to_f16(LV<double> L, double s, _Float16* __restrict__ htx,
                                                      _Float16* __restrict__ hty,
                                                      _Float16* __restrict__ htz, float* __restrict__ tx,
@@ -504,13 +506,15 @@ static __device__ __forceinline__ void t_to_f16(LV<double> L, double s, _Float16
     const Chunk ch = my_chunk_at(L.n, b_, nb_);
     auto h = [&](double t) { return (_Float16)(float)(t * s); };
     for (int64_t c = ch.begin + threadIdx.x; c < ch.end; c += kThreads) {
-        const _Float16 a = h(L.tx[c]), b = h(L.ty[c]), d = h(L.tz[c]);
+        // the CG operator's FP32 faces, and the preconditioner's FP16 copy
+        const float a32 = (float)(L.tx[c] * s), b32 = (float)(L.ty[c] * s), d32 = (float)(L.tz[c] * s);
+        tx[c] = a32;
+        ty[c] = b32;
+        tz[c] = d32;
+        const _Float16 a = (_Float16)a32, b = (_Float16)b32, d = (_Float16)d32;
         htx[c] = a;
         hty[c] = b;
         htz[c] = d;
-        tx[c] = (float)a;
-        ty[c] = (float)b;
-        tz[c] = (float)d;
         const float D = (float)a + (float)h(L.tx[c - 1]) + (float)b + (float)h(L.ty[c - L.nx]) + (float)d +
                         (float)h(L.tz[c - L.plane]);
         invD[c] = (L.invD[c] == 0.0 || D == 0.0f) ? 0.0f : 1.0f / D;

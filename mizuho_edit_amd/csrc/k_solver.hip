@@ -504,10 +504,14 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>
 // the iterate of the iteration that met tol, with its iteration count and
 // residual, exactly as without lookahead (the speculative iteration's
 // other writes are scratch), and *x_final names the buffer that holds it.
+//
+// FA: the CG operator A (q = A p) when it is not the preconditioner's
+// fine-level operator F (FP32 faces for CG, FP16 faces inside the
+// V-cycle); nullptr: F.
 template <typename R, typename X, typename C>
 static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F0, const MGOpts& o,
                     X* x, double tol, int maxit, int* its_out, double* rn_out, bool lookahead = false,
-                    X* x_alt = nullptr, X** x_final = nullptr)
+                    X* x_alt = nullptr, X** x_final = nullptr, const Fine<R, float>* FA = nullptr)
 {
     Fine<R, C> F = F0;  // r and r2 trade places at every fused update
     const bool fuse = mg && F.vec && o.sweeps >= 2 && F.r2 != nullptr;
@@ -543,7 +547,9 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
     // iteration its lands in result[S_RN] (and, under lookahead, in the
     // pinned ring slot its & 1 behind event ev_ring[its & 1])
     auto iterate = [&]() -> int {
-        if (int rc = dir_mv(c, F, (const R*)zres, (const R*)pc, pn, rz_cur, rz_prev, first)) return rc;
+        if (int rc = FA ? dir_mv(c, *FA, (const R*)zres, (const R*)pc, pn, rz_cur, rz_prev, first)
+                        : dir_mv(c, F, (const R*)zres, (const R*)pc, pn, rz_cur, rz_prev, first))
+            return rc;
         std::swap(pc, pn);
         if (fuse) {
             // update + pre-smoothing step 1 (into T) in one pass
@@ -1335,13 +1341,15 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         if (rn <= tol || its >= o.maxit || outer >= 50) break;
         outer++;
         LAUNCH1D(k_zero<float>, n, n, e);
+        // the projection and the learned stage apply the CG operator
+        // (FP32 faces; with OP16 the FP16 faces are the preconditioner's)
         if (outer == 1 && project_now) {
-            rc = op16 ? project(c, H, x, &rn) : project(c, F, x, &rn);
+            rc = project(c, F, x, &rn);
             if (rc) return rc;
         }
         if (learned_pending) {
             learned_pending = false;
-            rc = op16 ? learned_start(c, H, rn, e) : learned_start(c, F, rn, e);
+            rc = learned_start(c, F, rn, e);
             if (rc) return rc;
             learned = 1;
         }
@@ -1356,7 +1364,7 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         // iteration by iteration (an ensemble member repeats them bit for bit)
         float* ef = e;
         rc = op16 ? pcg_core<float, float, _Float16>(c, &c->mg32, H, mo, e, inner_tol, cap, &k, &rr, true,
-                                                     c->x32b.p, &ef)
+                                                     c->x32b.p, &ef, &F)
                   : pcg_core<float, float, float>(c, &c->mg32, F, mo, e, inner_tol, cap, &k, &rr, true, c->x32b.p,
                                                   &ef);
         if (rc) return rc;

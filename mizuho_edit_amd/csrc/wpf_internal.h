@@ -231,8 +231,9 @@ namespace wpf {
 enum PermMode { PERM_F64 = 0, PERM_F32 = 1, PERM_F16 = 2 };
 struct PermOut32 {
     double s;                    // scale of the FP32 system
-    _Float16 *htx, *hty, *htz;   // PERM_F16: FP16 faces
-    float *tx, *ty, *tz, *invD;  // FP32 faces (PERM_F32 only) and 1/D
+    _Float16 *htx, *hty, *htz;   // PERM_F16: the preconditioner's FP16 faces
+    float *tx, *ty, *tz, *invD;  // FP32 faces float(t s) (the CG operator) and
+                                 // 1/D (PERM_F16: of the decoded FP16 faces)
 };
 int launch_perm(wpf_ctx* c, int mode, const PermOut32* out);
 // the permeability (and everything derived from it) no longer matches phase

@@ -99,15 +99,15 @@ TRAFFIC_KERNELS = {
     "k_vupsweepIffDF16": ("cg_update", bytes_per_cell("cg_update", 4, 2)),
     "k_vsweepIfDF16_Li4ELb1ELb0ELb1E": ("smooth_post2", 5 * 4 + 3 * 2 + 1 + 4 / 8),
     "k_vsweepIfDF16_Li4ELb0ELb1ELb0E": ("smooth_post1", 4 * 4 + 3 * 2 + 1 + 4 / 8),
-    "k_vdir_mvIffLi4E": ("matvec", bytes_per_cell("matvec", 4, 2)),
+    "k_vdir_mv<float, float, 4>": ("matvec", bytes_per_cell("matvec", 4, 2)),
     "k_vfresid_restrictIfDF16": ("restrict", bytes_per_cell("restrict", 4, 2)),
     "k_step_fields<double>": ("step_fields", bytes_per_cell("step_fields", 4, 2)),
     # the fast path's k_perm also writes the FP32 faces, their FP16 copies
     # and an FP32 invD
     "k_perm<2>": ("perm", bytes_per_cell("perm", 4, 2) + 3 * 4 + 3 * 2 + 4),
     "k_vtrue_resid<2>": ("true_resid", bytes_per_cell("true_resid", 4, 2)),
-    "k_vgramIffLi4ELi4E": ("gram", 33.0),
-    "k_vproj_updateIffLi4ELi4E": ("proj_update", 57.0),
+    "k_vgram<float, float, 4, 4>": ("gram", 33.0),
+    "k_vproj_update<float, float, 4, 4>": ("proj_update", 57.0),
     "k_coarsen_pairsIfDF16": ("coarsen_pairs", 9.5),
 }
 

@@ -311,6 +311,63 @@ def calibrate_streams(sizes=(1 << 26, 1 << 24), reps=20):
                               "us": us.value, "GBps": nbytes / (us.value * 1e-6) / 1e9}), flush=True)
 
 
+def bench_network(a):
+    """The learned part's kernels alone: forward passes and energy-loss
+    gradients of the shipped flexible-CG weights on an a.grid^3 input (the
+    network runs at a.grid/4) with seeded random fields, a.warmup untimed and
+    a.steps timed passes of each.  Prints one JSON line with host-timed
+    milliseconds per pass (the host copies of the fields included);
+    rocprofv3 gives the per-kernel times and MFMA counters."""
+    import ctypes
+    import numpy as np
+    from mizuho_edit_amd import _lib, learned
+    net = learned.PCNet()
+    net.load(learned.WEIGHTS_FCG)
+    n = a.grid
+    L = _lib.lib()
+    rng = np.random.default_rng(a.seed)
+    # the C ABI's layouts directly (planar [3][n^3], x fastest): no host
+    # reordering inside the timed passes
+    fields = rng.standard_normal((3, n ** 3), dtype=np.float32)
+    target = rng.standard_normal(n ** 3, dtype=np.float32)
+    faces = rng.uniform(0.5, 1.5, (3, n, n, n)).astype(np.float32)  # [axis][z][y][x]
+    faces[0, :, :, -1] = 0.0  # no face beyond the global edge
+    faces[1, :, -1, :] = 0.0
+    faces[2, -1, :, :] = 0.0
+    faces = faces.reshape(3, -1)
+    fixed = np.zeros((n, n, n), np.uint8)
+    fixed[:, :, 0] = fixed[:, :, -1] = 1  # Dirichlet planes x = 0 and x = n-1
+    fixed = fixed.reshape(-1)
+    y = np.zeros((n // 4) ** 3, np.float32)
+    grad = np.zeros(net.nparams, np.float32)
+    loss = ctypes.c_double()
+    fp = lambda v: v.ctypes.data_as(_lib.F)  # noqa: E731
+
+    def forward():
+        _lib.check(L.wpf_net_forward(net._h, n, n, n, fp(fields), fp(y)), "wpf_net_forward")
+
+    def energy_gradient():
+        _lib.check(L.wpf_net_gradient_energy(net._h, n, n, n, fp(fields), fp(target), fp(faces),
+                                             fixed.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                             ctypes.byref(loss), fp(grad)), "wpf_net_gradient_energy")
+
+    out = {"metric": "learned-part pass time", "unit": "ms", "grid": n, "network_grid": n // 4,
+           "params": int(net.nparams), "weights": os.path.relpath(learned.WEIGHTS_FCG, ROOT),
+           "steps": a.steps, "warmup": a.warmup, "data": "synthetic: seeded random fields"}
+    for name, fn in (("forward", forward), ("energy_gradient", energy_gradient)):
+        for _ in range(a.warmup):
+            fn()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        sync()
+        out[f"{name}_ms"] = 1e3 * (time.perf_counter() - t0) / a.steps
+        log(f"{name}: {out[f'{name}_ms']:.2f} ms per pass")
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_ensemble(a):
     """The ensemble workload (48 uneven grids, 40,894,464 cells): the same
     steps once member by member with wpf_step (per-instance launches, one
@@ -420,12 +477,18 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="the 48-grid ensemble workload instead of the 256^3 grid")
     ap.add_argument("--ens-shrink", type=int, default=1, help="divide ensemble grid sizes by this")
+    ap.add_argument("--network", action="store_true",
+                    help="time the learned part's forward and energy-gradient passes alone (--grid, "
+                         "--steps, --warmup)")
     a = ap.parse_args()
     if a.traffic_table:
         traffic_table(*a.traffic_table, grid=a.grid)
         return
     if a.calibrate_streams:
         calibrate_streams()
+        return
+    if a.network:
+        bench_network(a)
         return
     if a.calibrate_copy:
         calibrate_copy()

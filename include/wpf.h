@@ -98,7 +98,8 @@ typedef struct wpf_well {
 #define WPF_FLAG_EXTRAPOLATE   4  /* linear-in-time initial guess            */
 #define WPF_FLAG_CAPTURE       8  /* keep (input, correction) for training   */
 #define WPF_FLAG_CHEBYSHEV    16  /* degree-2 Chebyshev smoothing (sweeps 2) */
-#define WPF_FLAG_OP16         32  /* mixed: fine faces stored as FP16        */
+#define WPF_FLAG_OP16         32  /* mixed: the V-cycle's fine faces stored
+                                     as FP16 (CG's matvec keeps FP32 faces) */
 #define WPF_FLAG_F32_STENCIL  64  /* wpf_step: phase/concentration stencils
                                      in FP32, FP64 fields, update and taumin */
 #define WPF_FLAG_PROJECT     128  /* mixed: initial guess projected onto the
@@ -265,16 +266,22 @@ int wpf_result_write(const char* path, double time, const wpf_params* p, int32_t
  * FP32 accumulation) maps (log k, pressure, Jacobi-scaled residual) at the
  * start of a solve to a pressure correction.  With WPF_FLAG_LEARNED_WARM and
  * a network attached, the solver takes that correction as a first search
- * direction with an exact line search on the FP64 operator, keeps it only if
- * the FP64 residual drops, and then runs its ordinary iteration to the same
- * FP64 tolerance: the network can speed a solve up, never decide it.
+ * direction with an exact line search (the FP64 solvers on the FP64
+ * operator, keeping the step only if the FP64 residual drops; the mixed
+ * solver on its FP32 CG operator, whose exact line search cannot raise the
+ * error's A-norm), and then runs its ordinary iteration to the same FP64
+ * tolerance: the network can speed a solve up, never decide it.  With an
+ * extrapolated guess the mixed solver gates the stage: it runs in the first
+ * and third of every 100 solves and otherwise only while those solves took
+ * at least 3% less time than the second and fourth.
  * With WPF_SOLVER_CG and WPF_FLAG_LEARNED_FCG the solve is flexible CG
- * (truncation 1) with the Jacobi preconditioner, where every 8th
- * preconditioner application (the first included) is the learned stage:
- * y = network correction for the current residual, s = (r, y)/(y, Ay),
- * z = s y + D^-1 (r - s A y).  It stops on the same FP64 residual test; if
- * that pass stalls (the residual fails to halve over 32 iterations) or does
- * not converge, plain CG continues from its iterate without the network.
+ * (truncation 1) with the Jacobi preconditioner, where every period-th
+ * preconditioner application (wpf_set_fcg_period, default 16; the first
+ * included) is the learned stage: y = network correction for the current
+ * residual, s = (r, y)/(y, Ay), z = s y + D^-1 (r - s A y).  It stops on the
+ * same FP64 residual test; if that pass stalls (the residual drops by less
+ * than 10% over 32 iterations) or does not converge, plain CG continues from
+ * its iterate without the network.
  * Grid sizes used with the network must be multiples of 32.
  * Fields for wpf_net_forward / _gradient / _train are planar FP32
  * [3][nx*ny*nz] (x fastest); targets [nx*ny*nz]; the output is the level-1

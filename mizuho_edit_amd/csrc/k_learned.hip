@@ -43,6 +43,7 @@ struct NetLayer {
 };
 
 static const int kNL = 8;
+static const int kBiasBlocks = 256;  // voxel chunks of the bias gradient
 static const int kCIN0 = 8;
 static const int kPool = 4;
 
@@ -359,19 +360,26 @@ __global__ void __launch_bounds__(256) k_conv_bwd_w(Geo g, Src s, const bf16* __
     const int64_t v0 = (int64_t)blockIdx.z * vchunk;
     const int64_t v1 = min(v0 + vchunk, g.nv);
     f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    // (the network's grids hold far fewer than 2^31 voxels: 32-bit indices)
+    const int nv = (int)g.nv;
     for (int64_t vb = v0; vb < v1; vb += 32) {
-        const int64_t kv = vb + 8 * (lane >> 4);
+        const int kv = (int)vb + 8 * (lane >> 4);
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + kv);
         bf16x8 b;
+        // (x, y, z) of the lane's first voxel, then stepped through the 8
+        int x = kv % g.nx, y = (kv / g.nx) % g.ny, z = kv / (g.nx * g.ny);
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const int64_t v = kv + e;
             float val = 0.0f;
-            if (nvalid && v < g.nv) {
-                const int x = (int)(v % g.nx), y = (int)((v / g.nx) % g.ny), z = (int)(v / ((int64_t)g.nx * g.ny));
-                val = src_at(s, g, x + ox, y + oy, z + oz, ci);
-            }
+            if (nvalid && kv + e < nv) val = src_at(s, g, x + ox, y + oy, z + oz, ci);
             b[e] = (bf16)val;
+            if (++x == g.nx) {
+                x = 0;
+                if (++y == g.ny) {
+                    y = 0;
+                    ++z;
+                }
+            }
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
     }
@@ -393,19 +401,27 @@ __global__ void k_sum_chunks(int64_t count, int nchunk, const float* __restrict_
     out[i] = s;
 }
 
-__global__ void k_bias_grad(Geo g, int cout, const float* __restrict__ gz, float* __restrict__ db)
+// Bias gradient, first half: block b sums gz over its chunk of voxels for
+// every channel into part[b][co].  Thread t reads channel t % cout of voxel
+// v0 + t / cout + k R (R = 256 / cout rows), so a wavefront's loads are
+// contiguous; the rows are added in a fixed order, and k_sum_chunks then
+// adds the blocks in a fixed order.  cout <= 256.
+__global__ void __launch_bounds__(256) k_bias_part(Geo g, int cout, int64_t chunk, const float* __restrict__ gz,
+                                                   float* __restrict__ part)
 {
-    const int co = blockIdx.x;
+    __shared__ float sh[256];
+    const int R = 256 / cout;
+    const int co = (int)threadIdx.x % cout, row = (int)threadIdx.x / cout;
+    const int64_t v0 = (int64_t)blockIdx.x * chunk, v1 = min(v0 + chunk, g.nv);
     float s = 0.0f;
-    for (int64_t v = threadIdx.x; v < g.nv; v += blockDim.x) s += gz[v * cout + co];
-    s = wave_sum(s);
-    __shared__ float sh[16];
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    if (row < R)
+        for (int64_t v = v0 + row; v < v1; v += R) s += gz[v * cout + co];
+    sh[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (row == 0) {
         float t = 0.0f;
-        for (int w = 0; w < (int)blockDim.x / 64; w++) t += sh[w];
-        db[co] = t;
+        for (int r = 0; r < R; r++) t += sh[r * cout + co];
+        part[(int64_t)blockIdx.x * cout + co] = t;
     }
 }
 
@@ -787,7 +803,8 @@ int ensure_train(wpf_net* n)
         (rc = alloc(&n->g_s3, v3 * 48)) || (rc = alloc(&n->g_p3, v4 * 48)) ||
         (rc = alloc(&n->g_p2, v3 * 32)) || (rc = alloc(&n->g_p1, v2 * 16)))
         return rc;
-    int64_t m16 = 0, mT = 0, mp = 0;
+    // mp also holds the bias gradient's partials (kBiasBlocks x cout <= 256)
+    int64_t m16 = 0, mT = 0, mp = (int64_t)kBiasBlocks * 256;
     for (int i = 0; i < kNL; i++) {
         const NetLayer& L = n->L[i];
         const BwdShape b = bwd_shape(L, n->lv[L.level]);
@@ -885,8 +902,17 @@ int layer_bwd(wpf_net* n, int li, const Geo& g, Src s, const float* gz, float* g
     hipLaunchKernelGGL(k_sum_chunks, dim3(nblk((int64_t)L.cout * ncol)), dim3(256), 0, n->stream,
                        (int64_t)L.cout * ncol, b.nchunk, (const float*)n->wpart, dw);
     WPF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_bias_grad, dim3(L.cout), dim3(256), 0, n->stream, g, L.cout, gz, db);
-    WPF_HIP(hipGetLastError());
+    {
+        // the bias partials reuse the weight partials' buffer after their sum
+        const int64_t bchunk = (g.nv + kBiasBlocks - 1) / kBiasBlocks;
+        const int nb = (int)((g.nv + bchunk - 1) / bchunk);
+        hipLaunchKernelGGL(k_bias_part, dim3((unsigned)nb), dim3(256), 0, n->stream, g, L.cout, bchunk, gz,
+                           n->wpart);
+        WPF_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_chunks, dim3(nblk(L.cout)), dim3(256), 0, n->stream, (int64_t)L.cout, nb,
+                           (const float*)n->wpart, db);
+        WPF_HIP(hipGetLastError());
+    }
     if (gin) {
         if (!n->packed) {
             int rc = pack_weights(n);

@@ -437,7 +437,7 @@ def bench_ensemble(a):
         "config": {"workload": "ensemble: 16x64^3 + 16x96x64x64 + 8x128x96x96 + 8x160x128x128"
                                + (f" (each dimension / {a.ens_shrink})" if a.ens_shrink > 1 else ""),
                    "cells": cells,
-                   "solver": ("" if a.no_project else "projected initial guess (last 4 solution changes) + ")
+                   "solver": ("" if a.no_project else "projected initial guess (last 5 solution changes) + ")
                    + "FP32 MG-PCG (Chebyshev) in FP64 refinement, rtol 1e-10",
                    **info},
     }
@@ -604,7 +604,7 @@ def main():
         "data": "synthetic: seeded Gaussian random field rock (corr 4 cells), Kozeny-Carman k",
         "config": {"workload": f"wormholing {n}^3, steps {a.warmup + 1}..{a.warmup + a.steps} "
                                f"of a {run_len}-step run", "grid": [n, n, n],
-                   "solver": ("" if a.no_project else "projected initial guess (last 4 solution changes) + ") +
+                   "solver": ("" if a.no_project else "projected initial guess (last 5 solution changes) + ") +
                              ("learned first stage (gated) + " if net else "") +
                              (f"FP32 MG-PCG ({a.fast_levels} levels, 2 Jacobi sweeps)" if a.jacobi else
                               f"FP32 MG-PCG ({a.fast_levels} levels, degree-2 Chebyshev, FP16-stored fine faces)") +

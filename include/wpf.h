@@ -103,7 +103,7 @@ typedef struct wpf_well {
 #define WPF_FLAG_F32_STENCIL  64  /* wpf_step: phase/concentration stencils
                                      in FP32, FP64 fields, update and taumin */
 #define WPF_FLAG_PROJECT     128  /* mixed: initial guess projected onto the
-                                     last 4 solution changes (A-norm optimal) */
+                                     last 5 solution changes (A-norm optimal) */
 
 typedef struct wpf_solver_opts {
     int32_t solver;       /* WPF_SOLVER_*                                    */

@@ -24,7 +24,7 @@
 namespace wpf {
 
 constexpr int kEnsMaxLevels = 8;
-constexpr int kEnsProjDepth = 4;  // the single-grid ring depth
+constexpr int kEnsProjDepth = 5;  // the single-grid ring depth
 constexpr int kEnsMaxStages = 8;  // refinement stages of a schedule
 
 // FP32 vectors of a member's inner solve

@@ -726,7 +726,9 @@ int sched_vcycle(const Sched& S, int dot_slot, int* zres, int rb, bool pre_done)
     const bool coarse = e->nlev > 1;
     if (coarse) {
         ELAUNCH((ke_fresid_restrict<C>), e->grid, dim3(kFX, kFY), 0, e->d_mem, e->tz, e->nz, rb, cur);
-        ECHK(sched_coarse(S, 1));
+        Sched Sc = S;
+        Sc.mo = S.mo.coarse();  // the levels below the fine one
+        ECHK(sched_coarse(Sc, 1));
     }
     for (int s = 0; s < o.sweeps; s++) {
         step_coef(o, s, &a, &w);

@@ -572,34 +572,59 @@ static __global__ void k_set_inj(int n, const int32_t* __restrict__ idx, const d
 
 
 // Smoother of every level: damped Jacobi (w1 = w2 = omega, a2 = 0) or the
-// degree-2 Chebyshev polynomial in D^-1 A on [hi/8, hi], hi = 2 (the
-// spectrum of D^-1 A of these M-matrices lies in [0, 2], and the residual
-// polynomial stays within [-1, 1] on all of it): step 1 x1 = x0 + w1 D^-1
-// r0, step 2 x2 = x1 + a2 (x1 - x0) + w2 D^-1 r1.  The same polynomial
-// before and after the coarse correction keeps the V-cycle symmetric.  The
-// lower end was measured: [0.5, 2], [0.3, 2], [0.25, 2], [0.2, 2],
-// [0.1, 2] took 9.45, 8.15, 8.05, 8.1, 9.15 FP32 iterations per step over
-// steps 6-25 of the 256^3 workload (the 2x2x2 aggregation's coarse
+// degree-2 Chebyshev polynomial in D^-1 A on [lo, 2] (the spectrum of D^-1
+// A of these M-matrices lies in [0, 2], and the residual polynomial stays
+// within [-1, 1] on all of it): step 1 x1 = x0 + w1 D^-1 r0, step 2 x2 = x1
+// + a2 (x1 - x0) + w2 D^-1 r1.  The same polynomial before and after the
+// coarse correction keeps the V-cycle symmetric.  The lower ends were
+// measured on the 256^3 workload (FP32 iterations per step over steps
+// 6-25): one interval for every level, [0.5, 2], [0.3, 2], [0.25, 2],
+// [0.2, 2], [0.1, 2] took 9.45, 8.15, 8.05, 8.1, 9.15; with the fine level
+// at [0.25, 2], the coarse levels at [0.5, 2] took 9.45 and at [0.15, 2]
+// 7.6; with the 5-deep projection ring, (fine, coarse) lower ends (0.25,
+// 0.15), (0.25, 0.1), (0.3, 0.15), (0.3, 0.1), (0.3, 0.07), (0.3, 0.05),
+// (0.35, 0.1) took 7.1, 6.85, 6.8, 6.7, 6.55, 6.7, 6.6, a plateau whose
+// middle is kept.  The 2x2x2 aggregation's piecewise-constant coarse
 // correction leaves more of the middle of the spectrum to the smoother than
-// a geometric interpolation would).
+// a geometric interpolation would.
 struct MGOpts {
     int sweeps;
     double cscale;
     int coarse_sweeps;
     double omega;      // coarsest-level Jacobi weight
-    double w1, a2, w2;
+    double w1, a2, w2;  // this level's smoothing steps
+    double cw1, ca2, cw2;  // the levels below
+    // the options of the levels below the fine one
+    MGOpts coarse() const
+    {
+        MGOpts m = *this;
+        m.w1 = cw1;
+        m.a2 = ca2;
+        m.w2 = cw2;
+        return m;
+    }
 };
+
+// (w1, a2, w2) of the degree-2 Chebyshev smoother on [lo, hi]
+static inline void cheb2(double lo, double hi, double* w1, double* a2, double* w2)
+{
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    const double rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+    *w1 = 1.0 / theta;
+    *a2 = rho1 * rho0;
+    *w2 = 2.0 * rho1 / delta;
+}
+
+// Chebyshev intervals [lo, 2] of the fine level and of the levels below
+static constexpr double kChebLoFine = 0.3, kChebLoCoarse = 0.1;
 
 static MGOpts mg_opts(const wpf_solver_opts& o)
 {
-    MGOpts m{o.mg_sweeps, o.mg_coarse_scale, 48, o.mg_omega, o.mg_omega, 0.0, o.mg_omega};
+    MGOpts m{o.mg_sweeps, o.mg_coarse_scale, 48, o.mg_omega, o.mg_omega, 0.0, o.mg_omega,
+             o.mg_omega, 0.0, o.mg_omega};
     if (o.flags & WPF_FLAG_CHEBYSHEV) {
-        const double hi = 2.0, lo = hi / 8.0;
-        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-        const double rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
-        m.w1 = 1.0 / theta;
-        m.a2 = rho1 * rho0;
-        m.w2 = 2.0 * rho1 / delta;
+        cheb2(kChebLoFine, 2.0, &m.w1, &m.a2, &m.w2);
+        cheb2(kChebLoCoarse, 2.0, &m.cw1, &m.ca2, &m.cw2);
     }
     return m;
 }

@@ -456,7 +456,7 @@ static int vcycle_fine(wpf_ctx* c, std::vector<MGLevel<R>>& lv, const Fine<R, C>
                         (const R*)F.r, (const R*)cur, C1.g.nx, C1.g.ny, C1.b.p);
             }
         }
-        if ((rc = vcycle_coarse(c, lv, 1, o))) return rc;
+        if ((rc = vcycle_coarse(c, lv, 1, o.coarse()))) return rc;
         pro = Pro<R>{C1.x.p, C1.g.nx, C1.g.plane, (R)o.cscale};
     }
     // post-smoothing; the first step adds the coarse correction as it reads

@@ -95,8 +95,9 @@ struct Reducer {
 };
 inline bool reduction_fits(const Reducer& r, int64_t nb, int nv) { return nb * (int64_t)nv <= r.cap; }
 // The most values one block of any kernel stores (the projection's Gram pass:
-// 10 packed Gram entries + 4 right-hand sides).
-static constexpr int kMaxValuesPerBlock = 16;
+// 15 packed Gram entries + 5 right-hand sides with the 5-deep ring), rounded
+// up.
+static constexpr int kMaxValuesPerBlock = 32;
 
 }  // namespace wpf
 
@@ -149,9 +150,10 @@ struct wpf_ctx {
     wpf::Dev<float> x32b;                     // second FP32 correction (the inner
                                               // solve's exact lookahead)
     // the last solution changes p_n - p_(n-1) (FP32 ring) for the projected
-    // initial guess (WPF_FLAG_PROJECT); four measured best at 256^3 (three,
-    // five and six cost more steps or more per step)
-    static constexpr int kProjDepth = 4;
+    // initial guess (WPF_FLAG_PROJECT); at 256^3 with the current smoother
+    // three, four, five and six took 8.7, 8.05, 7.6, 7.6 FP32 iterations
+    // per step over steps 6-25, and five was the fastest
+    static constexpr int kProjDepth = 5;
     wpf::Dev<float> proj[kProjDepth];
     int proj_count = 0, proj_next = 0;
     bool proj_used = false;

@@ -1092,18 +1092,24 @@ int wpf_ensemble_tune(wpf_ensemble* e, int32_t* grid_blocks, int32_t* tile_depth
         if ((rc = build_tables(e))) break;
         for (int g : grids) {
             e->grid = g;
-            auto sweep = [&]() -> int {
+            // one FP32 iteration's launches on scratch vectors: a V-cycle
+            // (the fine sweeps and the many small launches of the levels
+            // below, where the grid size matters most) and the direction
+            // update with its matvec
+            auto iteration = [&]() -> int {
+                Sched S{e, mg_opts(e->o), (int)e->mem.size()};
+                int zres = EV_Z;
                 if (e->op16)
-                    ELAUNCH((ke_vsweep<_Float16, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv,
-                            (int)EV_R, (int)EV_Z, -1, (int)EV_T, 0.0f, 0.5f, 0.0f);
+                    ECHK(sched_vcycle<_Float16>(S, S_RZ0, &zres, EV_R, false));
                 else
-                    ELAUNCH((ke_vsweep<float, false, false, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv,
-                            (int)EV_R, (int)EV_Z, -1, (int)EV_T, 0.0f, 0.5f, 0.0f);
+                    ECHK(sched_vcycle<float>(S, S_RZ0, &zres, EV_R, false));
+                ELAUNCH((ke_dir_mv<float>), e->grid, 256, 0, e->d_mem, e->tv, e->nv, zres, (int)EV_PA, (int)EV_PB,
+                        (int)S_RZ0, (int)S_RZ1, 1);
                 return WPF_OK;
             };
-            if ((rc = sweep())) break;
+            if ((rc = iteration())) break;
             (void)hipEventRecord(a, e->stream);
-            for (int r = 0; r < 3 && !rc; r++) rc = sweep();
+            for (int r = 0; r < 2 && !rc; r++) rc = iteration();
             (void)hipEventRecord(b, e->stream);
             if (rc || hipEventSynchronize(b) != hipSuccess) {
                 if (!rc) rc = WPF_ERR_HIP;

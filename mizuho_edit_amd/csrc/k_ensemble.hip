@@ -435,14 +435,16 @@ __global__ void __launch_bounds__(256) ke_vgram(ENS_ARGS)
 }
 
 // x += V y, r -= A V y, ||invD r||^2 (k_vproj_update) over vector tiles
-template <typename C>
+template <typename C, bool LDSY>
 __global__ void __launch_bounds__(256) ke_vproj_update(ENS_ARGS)
 {
+    extern __shared__ char smem_raw[];
     ENS_TILES(G_PROJ)
     const C *tx, *ty, *tz;
     ens_faces<C>(M, tx, ty, tz);
-    double acc[1] = {t_vproj_update<float, C, 4, kEnsProjDepth>(M.Z, T.b, ens_proj_vecs(M), M.result + S_PROJY, M.p,
-                                                               M.v[EV_R], tx, ty, tz, M.invD32, M.mask)};
+    double acc[1] = {t_vproj_update<float, C, 4, kEnsProjDepth, LDSY>(
+        M.Z, T.b, ens_proj_vecs(M), M.result + S_PROJY, M.p, M.v[EV_R], tx, ty, tz, M.invD32, M.mask,
+        reinterpret_cast<float*>(smem_raw))};
     block_reduce_store<1>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);
     ENS_END
 }
@@ -833,7 +835,10 @@ int sched_solve(wpf_ensemble* e)
             ELAUNCH((ke_vgram<float>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
             ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, NV, (int)S_GRAM, (int)G_PROJ);
             ELAUNCH(ke_proj_solve, S.mblocks(), S.mthreads(), 0, e->d_mem, nm);
-            ELAUNCH((ke_vproj_update<float>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
+            if (e->up_lds)
+                ELAUNCH((ke_vproj_update<float, true>), e->grid, 256, e->up_lds, e->d_mem, e->tv, e->nv);
+            else
+                ELAUNCH((ke_vproj_update<float, false>), e->grid, 256, 0, e->d_mem, e->tv, e->nv);
             ELAUNCH(ke_finalize, nm, 1024, 0, e->d_mem, (int)F_VEC, 1, (int)S_PN, (int)G_PROJ);
             ELAUNCH(ke_proj_tol, S.mblocks(), S.mthreads(), 0, e->d_mem, nm, o.inner_rtol);
         }

@@ -1084,7 +1084,12 @@ static int project_m(wpf_ctx* c, const Fine<float, C>& F, double* x, double* rn)
     hipLaunchKernelGGL((k_proj_solve<M>), dim3(1), dim3(64), 0, c->stream, (const double*)c->red.result + S_GRAM,
                        c->red.result + S_PROJY);
     WPF_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_vproj_update<float, C, 4, M>), dim3(nb), dim3(256), 0, c->stream, F.V, V,
+    // the y-neighbour rows of V y from LDS where the rows fit (as the fused
+    // CG update stages z0)
+    const bool lds = vupsweep_lds_fits<float, 4>(F.g.nx);
+    const size_t lds_bytes = lds ? vupsweep_lds<float, 4>(F.g.nx) : 0;
+    auto* const upd = lds ? k_vproj_update<float, C, 4, M, true> : k_vproj_update<float, C, 4, M, false>;
+    hipLaunchKernelGGL(upd, dim3(nb), dim3(256), lds_bytes, c->stream, F.V, V,
                        (const double*)c->red.result + S_PROJY, x, F.r, F.tx, F.ty, F.tz, F.invD,
                        (const uint8_t*)c->mask, c->red.partials);
     WPF_HIP(hipGetLastError());

@@ -434,12 +434,19 @@ __device__ __forceinline__ void stencil_rows(const R (&txv)[V], R txl, const R (
 
 // Gram pass: G = V^T A V (packed upper triangle) and c = V^T r, one pass
 // over the M vectors, r and the faces.  acc[tri(i, j)] and acc[NG + i].
+// G is summed in energy form over the faces, G_ij = sum_f t_f (dv_i)(dv_j)
+// with dv the difference across face f: the ring vectors are 0 at fixed
+// cells and the faces on the global edge are 0, so this is v_i^T A v_j of
+// the operator with the Dirichlet rows taken out.  Each cell takes its +x,
+// +y and +z faces, so only two planes of each vector stay in registers and
+// neither the -faces nor the mask are read.
 template <typename R, typename C, int V, int M>
 __device__ __forceinline__ void t_vgram(const VZ& Z, int bi, const ProjVecs<R>& vs,
                                         const R* __restrict__ r, const C* __restrict__ tx,
                                         const C* __restrict__ ty, const C* __restrict__ tz,
                                         const uint8_t* __restrict__ mask, double (&acc)[M * (M + 1) / 2 + M])
 {
+    (void)mask;
     constexpr int NG = M * (M + 1) / 2;
     int64_t q;
     int k0, k1;
@@ -448,62 +455,61 @@ __device__ __forceinline__ void t_vgram(const VZ& Z, int bi, const ProjVecs<R>& 
     const int64_t nx = Z.nx, pl = Z.plane;
 #pragma unroll
     for (int t = 0; t < NG + M; t++) acc[t] = 0.0;
-    R vm[M][V], v0[M][V], vp[M][V], tzm[V];
+    R v0[M][V], vp[M][V];
     int64_t c = q + pl * k0;
 #pragma unroll
-    for (int j = 0; j < M; j++) {
-        vld<V>(vs.v[j] + c - pl, vm[j]);
-        vld<V>(vs.v[j] + c, v0[j]);
-    }
-    vldc<V>(tz + c - pl, tzm);
+    for (int j = 0; j < M; j++) vld<V>(vs.v[j] + c, v0[j]);
+    // acc_ij += t di dj for the differences d of one face
+    auto face = [&](R t, const R (&d)[M]) {
+#pragma unroll
+        for (int i = 0; i < M; i++) {
+            const double ti = (double)(t * d[i]);
+#pragma unroll
+            for (int j = i; j < M; j++) acc[tri(i, j, M)] += ti * (double)d[j];
+        }
+    };
     for (int k = k0; k < k1; k++, c += pl) {
-        R txv[V], tyv[V], tym[V], tzv[V], rv[V];
-        uint8_t mk[V];
+        R txv[V], tyv[V], tzv[V], rv[V];
         vldc<V>(tx + c, txv);
         vldc<V>(ty + c, tyv);
-        vldc<V>(ty + c - nx, tym);
         vldc<V>(tz + c, tzv);
         vld<V>(r + c, rv);
-        vld<V>(mask + c, mk);
-        R txl = from_left(txv[V - 1]);
-        if (lane == 0) txl = (R)tx[c - 1];
-        R av[M][V];
+        R vyp[M][V], vr[M];
 #pragma unroll
         for (int j = 0; j < M; j++) {
-            R vyp[V], vym[V];
             vld<V>(vs.v[j] + c + pl, vp[j]);
-            vld<V>(vs.v[j] + c + nx, vyp);
-            vld<V>(vs.v[j] + c - nx, vym);
-            R vl = from_left(v0[j][V - 1]), vr = from_right(v0[j][0]);
-            if (lane == 0) vl = vs.v[j][c - 1];
-            if (lane == 63) vr = vs.v[j][c + V];
-            stencil_rows<R, V>(txv, txl, tyv, tym, tzv, tzm, vm[j], v0[j], vp[j], vyp, vym, vl, vr, mk, av[j]);
+            vld<V>(vs.v[j] + c + nx, vyp[j]);
+            vr[j] = from_right(v0[j][0]);
+            if (lane == 63) vr[j] = vs.v[j][c + V];
         }
         if (act) {
 #pragma unroll
             for (int h = 0; h < V; h++) {
+                R dx[M], dy[M], dz[M];
 #pragma unroll
-                for (int i = 0; i < M; i++) {
-                    acc[NG + i] += (double)v0[i][h] * (double)rv[h];
-#pragma unroll
-                    for (int j = i; j < M; j++) acc[tri(i, j, M)] += (double)v0[i][h] * (double)av[j][h];
+                for (int j = 0; j < M; j++) {
+                    dx[j] = ((h + 1 < V) ? v0[j][h + 1] : vr[j]) - v0[j][h];
+                    dy[j] = vyp[j][h] - v0[j][h];
+                    dz[j] = vp[j][h] - v0[j][h];
+                    acc[NG + j] += (double)v0[j][h] * (double)rv[h];
                 }
+                face(txv[h], dx);
+                face(tyv[h], dy);
+                face(tzv[h], dz);
             }
         }
 #pragma unroll
         for (int j = 0; j < M; j++)
 #pragma unroll
-            for (int h = 0; h < V; h++) {
-                vm[j][h] = v0[j][h];
-                v0[j][h] = vp[j][h];
-            }
-#pragma unroll
-        for (int h = 0; h < V; h++) tzm[h] = tzv[h];
+            for (int h = 0; h < V; h++) v0[j][h] = vp[j][h];
     }
 }
 
+#ifndef WPF_GRAM_WAVES
+#define WPF_GRAM_WAVES 1
+#endif
 template <typename R, typename C, int V, int M>
-__global__ void __launch_bounds__(256) k_vgram(VZ Z, ProjVecs<R> vs, const R* __restrict__ r,
+__global__ void __launch_bounds__(256, WPF_GRAM_WAVES) k_vgram(VZ Z, ProjVecs<R> vs, const R* __restrict__ r,
                                                const C* __restrict__ tx, const C* __restrict__ ty,
                                                const C* __restrict__ tz, const uint8_t* __restrict__ mask,
                                                double* __restrict__ partials)
@@ -569,18 +575,26 @@ __global__ void k_proj_solve(const double* __restrict__ res, double* __restrict_
 }
 
 // x += V y (FP64 field), r -= A V y (FP32 residual), reduces ||invD r||^2.
-template <typename R, typename C, int V, int M>
+// LDSY: u = V y of the y-1 / y+1 rows comes from LDS (us: two buffers of
+// 256*V + 2*nx values, one barrier per plane), each row's u formed once by
+// the thread that owns it or, for the rows below and above the chunk, by
+// the first 2*nx/V threads: 5 vector loads per row instead of 5 per row and
+// neighbour.  The values are those of the global-load form.
+template <typename R, typename C, int V, int M, bool LDSY = false>
 __device__ __forceinline__ double t_vproj_update(const VZ& Z, int bi, const ProjVecs<R>& vs,
                                                  const double* __restrict__ y, double* __restrict__ x,
                                                  R* __restrict__ r, const C* __restrict__ tx,
                                                  const C* __restrict__ ty, const C* __restrict__ tz,
-                                                 const R* __restrict__ invD, const uint8_t* __restrict__ mask)
+                                                 const R* __restrict__ invD, const uint8_t* __restrict__ mask,
+                                                 R* us = nullptr)
 {
     int64_t q;
     int k0, k1;
     const bool act = vz_at<V>(Z, bi, q, k0, k1);
     const int lane = threadIdx.x & 63;
     const int64_t nx = Z.nx, pl = Z.plane;
+    const int64_t q0 = vz_chunk0<V>(Z, bi);             // first plane cell of the block's chunk
+    const int zw = 256 * V + 2 * Z.nx, nxv = Z.nx / V;  // LDS values per plane, vectors per row
     R yc[M];
 #pragma unroll
     for (int j = 0; j < M; j++) yc[j] = (R)y[j];
@@ -612,8 +626,36 @@ __device__ __forceinline__ double t_vproj_update(const VZ& Z, int bi, const Proj
         urow(c + pl, up);
         R uyp[V], uym[V], txv[V], tyv[V], tym[V], tzv[V], rv[V], di[V], au[V];
         uint8_t mk[V];
-        urow(c + nx, uyp);
-        urow(c - nx, uym);
+        R ul = from_left(u0[V - 1]), ur = from_right(u0[0]);
+        if (LDSY) {
+            R* buf = us + (k & 1) * zw;
+            const int o = (int)nx + (int)threadIdx.x * V;
+#pragma unroll
+            for (int h = 0; h < V; h++) buf[o + h] = u0[h];
+            // the rows below and above the chunk (clamped into the grid's
+            // planes and pads; clamped values only meet zero coefficients)
+            for (int hv = threadIdx.x; hv < 2 * nxv; hv += 256) {
+                const int lo = hv < nxv ? hv * V : (int)nx + 256 * V + (hv - nxv) * V;
+                const int64_t m = min(q0 - nx + lo, pl - V);
+                R uu[V];
+                urow(pl * k + m, uu);
+#pragma unroll
+                for (int h = 0; h < V; h++) buf[lo + h] = uu[h];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < V; h++) {
+                uyp[h] = buf[o + (int)nx + h];
+                uym[h] = buf[o - (int)nx + h];
+            }
+            if (lane == 0) ul = buf[o - 1];
+            if (lane == 63) ur = buf[o + V];
+        } else {
+            urow(c + nx, uyp);
+            urow(c - nx, uym);
+            if (lane == 0) ul = uone(c - 1);
+            if (lane == 63) ur = uone(c + V);
+        }
         vldc<V>(tx + c, txv);
         vldc<V>(ty + c, tyv);
         vldc<V>(ty + c - nx, tym);
@@ -621,12 +663,8 @@ __device__ __forceinline__ double t_vproj_update(const VZ& Z, int bi, const Proj
         vld<V>(r + c, rv);
         vld<V>(invD + c, di);
         vld<V>(mask + c, mk);
-        R txl = from_left(txv[V - 1]), ul = from_left(u0[V - 1]), ur = from_right(u0[0]);
-        if (lane == 0) {
-            txl = (R)tx[c - 1];
-            ul = uone(c - 1);
-        }
-        if (lane == 63) ur = uone(c + V);
+        R txl = from_left(txv[V - 1]);
+        if (lane == 0) txl = (R)tx[c - 1];
         stencil_rows<R, V>(txv, txl, tyv, tym, tzv, tzm, um, u0, up, uyp, uym, ul, ur, mk, au);
         double xv[V];
         vld<V>(x + c, xv);
@@ -648,10 +686,12 @@ __device__ __forceinline__ double t_vproj_update(const VZ& Z, int bi, const Proj
             tzm[h] = tzv[h];
         }
     }
+    // the buffers are reused by the next tile of a persistent launch
+    if (LDSY) __syncthreads();
     return act ? acc : 0.0;
 }
 
-template <typename R, typename C, int V, int M>
+template <typename R, typename C, int V, int M, bool LDSY>
 __global__ void __launch_bounds__(256) k_vproj_update(VZ Z, ProjVecs<R> vs,
                                                       const double* __restrict__ y, double* __restrict__ x,
                                                       R* __restrict__ r, const C* __restrict__ tx,
@@ -659,7 +699,9 @@ __global__ void __launch_bounds__(256) k_vproj_update(VZ Z, ProjVecs<R> vs,
                                                       const R* __restrict__ invD, const uint8_t* __restrict__ mask,
                                                       double* __restrict__ partials)
 {
-    double acc[1] = {t_vproj_update<R, C, V, M>(Z, blockIdx.x, vs, y, x, r, tx, ty, tz, invD, mask)};
+    extern __shared__ char smem_raw[];
+    double acc[1] = {t_vproj_update<R, C, V, M, LDSY>(Z, blockIdx.x, vs, y, x, r, tx, ty, tz, invD, mask,
+                                                      reinterpret_cast<R*>(smem_raw))};
     block_reduce_store<1>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
 }
 

@@ -373,8 +373,9 @@ struct GidOut {
 };
 
 // Fortran's ES12.5 edit of the time: d.ddddd, then E+xx, or +xxx (no E) for
-// a three-digit exponent, right-justified in 12 characters
-std::string es12_5(double v)
+// a three-digit exponent, right-justified in 12 characters (C++ linkage:
+// it returns a std::string)
+extern "C++" std::string es12_5(double v)
 {
     char b[64];
     snprintf(b, sizeof b, "%.5E", v);

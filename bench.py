@@ -47,6 +47,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-steps/s at 256³: learned+FP32-refined vs plain FP64 GPU, ≤1e-6 p err"
 HBM_PEAK_GBS = 8000.0
+# what a kernel that streams three or more 256 MiB arrays reaches on this
+# part (the streaming probe, 5.39-5.62 TB/s: profiles/streaming_bandwidth.md)
+STREAM_GBS = 5500.0
 
 # algorithmic bytes per cell (each array element the math must read or
 # write once per launch; mask = 1 byte); r = vector precision, h = stored
@@ -634,7 +637,8 @@ def main():
                      "traffic": measured_traffic(a.kernel, n) if a.kernel == "cg_update" and not a.jacobi
                      else None,
                      "kernel": a.kernel, "launches": count, "avg_us": avg_s * 1e6,
-                     "bytes_per_cell": bpc},
+                     "bytes_per_cell": bpc,
+                     "frac_of_streaming_probe": achieved / STREAM_GBS},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -387,8 +387,9 @@ def bench_learned(a):
     from mizuho_edit_amd.wpfaw3d import WPFAW3D, solver_opts
     n, nsteps = 192, a.learned
     case = io.workload("learned-192", seed=1001, grid=(n, n, n))
+    weights = a.weights or learned.WEIGHTS_FCG
     net = learned.PCNet()
-    net.load(learned.WEIGHTS_FCG)
+    net.load(weights)
     runs = {}
     for name, use in (("cg_jacobi", False), ("flexible_cg_learned", True)):
         log(f"{name}: {nsteps} steps at {n}^3")
@@ -424,7 +425,7 @@ def bench_learned(a):
         del r["fields"]
     out = {"metric": "learned configuration: iterations and solve time, flexible CG with the learned "
                      "stage against CG + Jacobi", "grid": [n, n, n], "steps": nsteps,
-           "weights": os.path.relpath(learned.WEIGHTS_FCG, ROOT),
+           "weights": os.path.relpath(weights, ROOT),
            "data": "synthetic: seeded GRF rock (seed 1001, not in the training set)",
            "runs": runs,
            "iteration_ratio": a1["iterations"] / a0["iterations"],

@@ -139,13 +139,14 @@ def test_energy_training_gains_inside_flexible_cg(gpu):
 
 
 def test_learned_stage_inside_flexible_cg_gain(gpu):
-    """The shipped stage weights (energy-norm training on 64 realisations
-    at 192^3, snapshots every 10 steps) inside flexible CG with the Jacobi
-    preconditioner, against CG + Jacobi (the reference's KSP settings) at
-    192^3 on a realisation they were not trained on, 12 steps from the
-    previous pressure: the answers agree, and over steps 4..12 (the first
-    three include 1000-iteration capped solves on both sides) the learned
-    path needs at most 95% of the iterations and less solve time."""
+    """The shipped stage weights (energy-norm training at 192^3 on stage
+    points captured every 50 steps over 1000-step runs of 16 realisations)
+    inside flexible CG with the Jacobi preconditioner, against CG + Jacobi
+    (the reference's KSP settings) at 192^3 on a realisation they were not
+    trained on, 12 steps from the previous pressure: the answers agree, and
+    over steps 4..12 (the first three include 1000-iteration capped solves
+    of CG + Jacobi) the learned path needs at most 70% of the iterations
+    (62% measured) and at most 90% of the solve time (75% measured)."""
     net = learned.PCNet()
     net.load(learned.WEIGHTS_FCG)
     case = io.workload("plain-small", seed=1001, grid=(192, 192, 192), corr_len=4.0)
@@ -157,8 +158,8 @@ def test_learned_stage_inside_flexible_cg_gain(gpu):
           f"{r['net']['learned']} stage applications)", flush=True)
     p0, p1 = r["plain"]["pressure"], r["net"]["pressure"]
     assert np.abs(p1 - p0).max() <= 1e-7 * np.abs(p0).max()
-    assert sum(its1) <= 0.95 * sum(its0)
-    assert t1 < t0
+    assert sum(its1) <= 0.70 * sum(its0)
+    assert t1 <= 0.90 * t0
     net.close()
 
 

@@ -104,6 +104,9 @@ typedef struct wpf_well {
                                      in FP32, FP64 fields, update and taumin */
 #define WPF_FLAG_PROJECT     128  /* mixed: initial guess projected onto the
                                      last 5 solution changes (A-norm optimal) */
+#define WPF_FLAG_SPLIT_TAIL  256  /* multigrid: the last two levels as five
+                                     separate launches instead of one (the
+                                     same values; kept for comparison)      */
 
 typedef struct wpf_solver_opts {
     int32_t solver;       /* WPF_SOLVER_*                                    */

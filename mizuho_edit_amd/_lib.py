@@ -37,6 +37,7 @@ FLAG_CHEBYSHEV = 16
 FLAG_OP16 = 32
 FLAG_F32_STENCIL = 64
 FLAG_PROJECT = 128
+FLAG_SPLIT_TAIL = 256
 
 
 class WpfSolverOpts(ctypes.Structure):

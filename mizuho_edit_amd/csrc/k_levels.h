@@ -467,6 +467,163 @@ __global__ void __launch_bounds__(1024) k_coarsest(LV<R> L, R w, int nsweep, con
 }
 
 // ---------------------------------------------------------------------------
+// The last two levels of a V-cycle (two smoothing steps before and after) in
+// one workgroup of 1024 threads: level L of at most 4096 cells, whose
+// coefficients and right-hand side stay in registers (4 cells per thread),
+// and the coarsest level C below it (t_coarsest_reg).  Every intermediate
+// vector lives in LDS; only L's result x goes back to memory.
+//
+// Per cell the arithmetic is that of the separate launches, in the same
+// order: k_smooth01 (steps 1 and 2 from x = 0), k_resid_restrict, the
+// coarsest level's damped-Jacobi sweeps, k_smooth_pro<PROX> (step 1 with the
+// prolonged correction) and k_smooth_pro<PROO> or k_smooth (step 2), so x is
+// the same bit for bit; five launches become one.  The LDS vectors are laid
+// out as the padded global arrays (one zero plane before and after), so the
+// neighbours of edge cells, which meet zero coefficients, read zeros or
+// in-grid values as there.
+//
+// LDS: 2 (n + 2 plane) + (cn + 2 cplane) + cn + 2 (cn + 2 cplane) values.
+// ---------------------------------------------------------------------------
+template <typename R>
+inline size_t tail2_lds(int64_t n, int64_t plane, int64_t cn, int64_t cplane)
+{
+    return sizeof(R) * (size_t)(2 * n + 4 * plane + 4 * cn + 6 * cplane);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(1024) k_tail2(LV<R> L, LV<R> C, R w1, R a2, R w2, R cs, R omega, int nsweep,
+                                                const R* __restrict__ b, R* __restrict__ xout)
+{
+    constexpr int S = 4;
+    extern __shared__ char smem_raw[];
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int pl = (int)L.plane, n = (int)L.n;
+    const int cnx = C.nx, cny = C.ny, cpl = (int)C.plane, cn = (int)C.n;
+    R* const base = reinterpret_cast<R*>(smem_raw);
+    const int nzero = 2 * (n + 2 * pl) + cn + 2 * cpl;  // xa, xb, xc with their pads
+    R* const xa = base + pl;
+    R* const xb = base + (n + 2 * pl) + pl;
+    R* const xc = base + 2 * (n + 2 * pl) + cpl;
+    R* const bc = base + nzero;
+    char* const csm = reinterpret_cast<char*>(bc + cn);
+    for (int c = threadIdx.x; c < nzero; c += 1024) base[c] = (R)0;
+
+    R txp[S], txm[S], typ[S], tym[S], tzp[S], tzm[S], dd[S], di[S], bb[S], y1[S];
+    int ci[S], cj[S], ck[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int c = threadIdx.x + 1024 * s;
+        const bool in = c < n;
+        txp[s] = in ? L.tx[c] : (R)0;
+        txm[s] = in ? L.tx[c - 1] : (R)0;
+        typ[s] = in ? L.ty[c] : (R)0;
+        tym[s] = in ? L.ty[c - nx] : (R)0;
+        tzp[s] = in ? L.tz[c] : (R)0;
+        tzm[s] = in ? L.tz[c - pl] : (R)0;
+        dd[s] = in ? L.D[c] : (R)0;
+        di[s] = in ? L.invD[c] : (R)0;
+        bb[s] = in ? b[c] : (R)0;
+        ci[s] = c % nx;
+        cj[s] = (c / nx) % ny;
+        ck[s] = c / pl;
+        y1[s] = (R)0;
+    }
+    __syncthreads();
+    // k_smooth01: x0 = w1 invD b, then step 2 from it
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int c = threadIdx.x + 1024 * s;
+        if (c < n) xa[c] = w1 * di[s] * bb[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int c = threadIdx.x + 1024 * s;
+        if (c < n) {
+            const R sm = txp[s] * xa[c + 1] + txm[s] * xa[c - 1] + typ[s] * xa[c + nx] + tym[s] * xa[c - nx] +
+                         tzp[s] * xa[c + pl] + tzm[s] * xa[c - pl];
+            const R xv = xa[c];
+            R yv = xv + w2 * di[s] * (bb[s] - (dd[s] * xv - sm));
+            if (a2 != (R)0) yv += a2 * (xv - (R)0);
+            y1[s] = yv;
+            xb[c] = yv;
+        }
+    }
+    __syncthreads();
+    // k_resid_restrict: each cell's residual, then the children's sums
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int c = threadIdx.x + 1024 * s;
+        if (c < n) {
+            const R sm = txp[s] * xb[c + 1] + txm[s] * xb[c - 1] + typ[s] * xb[c + nx] + tym[s] * xb[c - nx] +
+                         tzp[s] * xb[c + pl] + tzm[s] * xb[c - pl];
+            xa[c] = bb[s] - (dd[s] * y1[s] - sm);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int C0 = threadIdx.x + 1024 * s;
+        if (C0 < cn) {
+            const int I = C0 % cnx, J = (C0 / cnx) % cny, K = C0 / cpl;
+            R acc = (R)0;
+            for (int dk = 0; dk < 2; dk++) {
+                const int k = 2 * K + dk;
+                if (k >= nz) break;
+                for (int dj = 0; dj < 2; dj++) {
+                    const int j = 2 * J + dj;
+                    if (j >= ny) break;
+                    for (int dx = 0; dx < 2; dx++) {
+                        const int i = 2 * I + dx;
+                        if (i >= nx) break;
+                        acc += xa[i + nx * (j + ny * k)];
+                    }
+                }
+            }
+            bc[C0] = acc;
+        }
+    }
+    __syncthreads();
+    // the coarsest level: its sweeps from 0 into xc
+    t_coarsest_reg<R>(cnx, cny, C.nz, C.tx, C.ty, C.tz, C.invD, omega, nsweep, bc, xc, csm);
+    __syncthreads();
+    auto par = [&](int i, int j, int k) { return (i >> 1) + cnx * (j >> 1) + cpl * (k >> 1); };
+    // k_smooth_pro<PROX>: step 1 from x + cs P xc (a = 0), into xa
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int c = threadIdx.x + 1024 * s;
+        if (c < n) {
+            const int i = ci[s], j = cj[s], k = ck[s];
+            auto X = [&](int m, int ii, int jj, int kk) { return xb[m] + cs * xc[par(ii, jj, kk)]; };
+            const R sm = txp[s] * X(c + 1, i + 1, j, k) + txm[s] * X(c - 1, i - 1, j, k) +
+                         typ[s] * X(c + nx, i, j + 1, k) + tym[s] * X(c - nx, i, j - 1, k) +
+                         tzp[s] * X(c + pl, i, j, k + 1) + tzm[s] * X(c - pl, i, j, k - 1);
+            const R xv = X(c, i, j, k);
+            const R yv = xv + w1 * di[s] * (bb[s] - (dd[s] * xv - sm));
+            xa[c] = yv;
+        }
+    }
+    __syncthreads();
+    // step 2 (k_smooth_pro<PROO>, or k_smooth when a2 = 0), into xout
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int c = threadIdx.x + 1024 * s;
+        if (c < n) {
+            const R sm = txp[s] * xa[c + 1] + txm[s] * xa[c - 1] + typ[s] * xa[c + nx] + tym[s] * xa[c - nx] +
+                         tzp[s] * xa[c + pl] + tzm[s] * xa[c - pl];
+            const R xv = xa[c];
+            R yv = xv + w2 * di[s] * (bb[s] - (dd[s] * xv - sm));
+            if (a2 != (R)0) {
+                R ov = y1[s];
+                ov = ov + cs * xc[par(ci[s], cj[s], ck[s])];
+                yv += a2 * (xv - ov);
+            }
+            xout[c] = yv;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // precision conversion of the fine operator: t32 = t64 * s, invD32 = 1/D32
 // ---------------------------------------------------------------------------
 static __device__ __forceinline__ void t_to_f32(LV<double> L, double s, float* __restrict__ tx,
@@ -594,6 +751,7 @@ struct MGOpts {
     double omega;      // coarsest-level Jacobi weight
     double w1, a2, w2;  // this level's smoothing steps
     double cw1, ca2, cw2;  // the levels below
+    bool tail = true;      // the last two levels in one launch where they fit
     // the options of the levels below the fine one
     MGOpts coarse() const
     {
@@ -626,6 +784,7 @@ static MGOpts mg_opts(const wpf_solver_opts& o)
         cheb2(kChebLoFine, 2.0, &m.w1, &m.a2, &m.w2);
         cheb2(kChebLoCoarse, 2.0, &m.cw1, &m.ca2, &m.cw2);
     }
+    m.tail = !(o.flags & WPF_FLAG_SPLIT_TAIL);
     return m;
 }
 

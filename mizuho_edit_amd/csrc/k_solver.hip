@@ -198,6 +198,38 @@ static int coarsest(wpf_ctx* c, MGLevel<R>& L, const R* b, R* x, R* tmp, const M
     return WPF_OK;
 }
 
+// Whether level l and the coarsest level below it run as one k_tail2
+// launch: FP32, two smoothing steps, both levels of at most 4096 cells (the
+// coarsest one solved in registers, as k_coarsest_reg does) and the LDS
+// within the 150 KiB the launch asks for.
+template <typename R>
+static bool tail2_fits(const std::vector<MGLevel<R>>& lv, size_t l, const MGOpts& o)
+{
+    if (!o.tail || sizeof(R) != 4 || o.sweeps != 2 || l + 2 != lv.size()) return false;
+    const Grid& L = lv[l].g;
+    const Grid& C = lv[l + 1].g;
+    return L.n <= 4 * 1024 && C.n <= 4 * 1024 &&
+           tail2_lds<R>(L.n, L.plane, C.n, C.plane) <= 150 * 1024;
+}
+
+// level L and the coarsest level C in one workgroup (k_tail2): L.x = M_L L.b
+template <typename R>
+static int tail2(wpf_ctx* c, MGLevel<R>& L, MGLevel<R>& C, const MGOpts& o)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        WPF_HIP(hipFuncSetAttribute((const void*)k_tail2<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    150 * 1024));
+        attr_set = true;
+    }
+    const size_t lds = tail2_lds<R>(L.g.n, L.g.plane, C.g.n, C.g.plane);
+    hipLaunchKernelGGL(k_tail2<R>, dim3(1), dim3(1024), lds, c->stream, view(L, nullptr, false),
+                       view(C, nullptr, false), (R)o.w1, (R)o.a2, (R)o.w2, (R)o.cscale, (R)o.omega,
+                       o.coarse_sweeps, (const R*)L.b.p, L.x.p);
+    WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
 // V-cycle on a coarse level l >= 1: x = M_l b, using the level's own arrays.
 // The first two pre-smoothing steps run as one pass (k_smooth01) and the
 // prolongation of the next level's correction is folded into the first
@@ -207,6 +239,8 @@ static int vcycle_coarse(wpf_ctx* c, std::vector<MGLevel<R>>& lv, size_t l, cons
 {
     MGLevel<R>& L = lv[l];
     if (l + 1 == lv.size()) return coarsest(c, L, (const R*)L.b.p, L.x.p, L.r.p, o);
+    if constexpr (sizeof(R) == 4)
+        if (tail2_fits(lv, l, o)) return tail2(c, L, lv[l + 1], o);
     LV<R> V = view(L, nullptr, false);
     const R* b = L.b.p;
     // ping-pong r <-> x; the result lands in x

@@ -337,7 +337,7 @@ static int dir_mv(wpf_ctx* c, const Fine<R, C>& F, const R* z, const R* pold, R*
 // are staged in LDS where the rows fit.
 template <typename R, typename X, typename C>
 static int upsweep(wpf_ctx* c, const Fine<R, C>& F, int num, int den, R w1, double a1, double w1s, const X* x,
-                   X* x_out, const R* p)
+                   X* x_out, const R* p, double* hpart = nullptr)
 {
     constexpr int V = vwidth<R>();
     const bool lds = vupsweep_lds_fits<R, V>(F.g.nx);
@@ -346,7 +346,7 @@ static int upsweep(wpf_ctx* c, const Fine<R, C>& F, int num, int den, R w1, doub
     prof_mark(c, WPF_KERNEL_CG_UPDATE, true);
     hipLaunchKernelGGL(kern, dim3(vz_blocks(F.V)), dim3(256), lds_bytes, c->stream, F.V,
                        (const double*)c->red.result, num, den, w1, (R)a1, (R)w1s, x, x_out, p, (const R*)F.r,
-                       (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, F.T, c->red.partials);
+                       (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, F.T, c->red.partials, hpart);
     prof_mark(c, WPF_KERNEL_CG_UPDATE, false);
     WPF_HIP(hipGetLastError());
     return WPF_OK;
@@ -577,9 +577,16 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
         WPF_HIP(hipEventCreateWithFlags(&c->ev_ring[0], hipEventDisableTiming));
         WPF_HIP(hipEventCreateWithFlags(&c->ev_ring[1], hipEventDisableTiming));
     }
+    // under lookahead the fused update's blocks store their ||invD r'||^2
+    // partials in the pinned ring themselves; the host adds them in
+    // k_finalize's order (the same value), which spares a finalize launch
+    // and a copy per iteration (nothing on the device reads that norm)
+    const int64_t ring_nb = fuse ? vz_blocks(F.V) : 0;
+    const bool host_norm = lookahead && fuse && ring_nb <= c->red.hring_n;
     // one CG iteration: direction, matvec, update; the residual norm of
-    // iteration its lands in result[S_RN] (and, under lookahead, in the
-    // pinned ring slot its & 1 behind event ev_ring[its & 1])
+    // iteration its lands in result[S_RN] (under lookahead: in the pinned
+    // slot its & 1, or as partials in the pinned ring row its & 1 with
+    // host_norm, behind event ev_ring[its & 1])
     auto iterate = [&]() -> int {
         if (int rc = FA ? dir_mv(c, *FA, (const R*)zres, (const R*)pc, pn, rz_cur, rz_prev, first)
                         : dir_mv(c, F, (const R*)zres, (const R*)pc, pn, rz_cur, rz_prev, first))
@@ -589,9 +596,10 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
             // update + pre-smoothing step 1 (into T) in one pass
             double a1, w1s;
             step_coef(o, 1, &a1, &w1s);
-            if (int rc = upsweep<R, X, C>(c, F, rz_cur, S_PQ, w, a1, w1s, xs[xi], xs[xi ^ 1], pc)) return rc;
+            double* const hp = host_norm ? c->red.hring + ((its + 1) & 1) * c->red.hring_n : nullptr;
+            if (int rc = upsweep<R, X, C>(c, F, rz_cur, S_PQ, w, a1, w1s, xs[xi], xs[xi ^ 1], pc, hp)) return rc;
             if (exact) xi ^= 1;
-            FINALIZE(vz_blocks(F.V), 2, S_RN);
+            if (!host_norm) FINALIZE(vz_blocks(F.V), 2, S_RN);
             std::swap(F.r, F.r2);
         } else {
             const int kid_ = WPF_KERNEL_CG_UPDATE;
@@ -603,8 +611,9 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
         first = 0;
         if (lookahead) {
             const int s = its & 1;
-            WPF_HIP(hipMemcpyAsync(c->red.host + kRingSlot + s, c->red.result + S_RN, sizeof(double),
-                                   hipMemcpyDeviceToHost, c->stream));
+            if (!host_norm)
+                WPF_HIP(hipMemcpyAsync(c->red.host + kRingSlot + s, c->red.result + S_RN, sizeof(double),
+                                       hipMemcpyDeviceToHost, c->stream));
             WPF_HIP(hipEventRecord(c->ev_ring[s], c->stream));
         }
         return WPF_OK;
@@ -620,7 +629,10 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
     };
     // the residual norm of iteration k (already enqueued)
     auto norm_of = [&](int k, double* out) -> int {
-        if (lookahead) {
+        if (host_norm) {
+            WPF_HIP(hipEventSynchronize(c->ev_ring[k & 1]));
+            *out = std::sqrt(finalize_on_host(c->red.hring + (k & 1) * c->red.hring_n, ring_nb));
+        } else if (lookahead) {
             WPF_HIP(hipEventSynchronize(c->ev_ring[k & 1]));
             *out = std::sqrt(c->red.host[kRingSlot + (k & 1)]);
         } else {

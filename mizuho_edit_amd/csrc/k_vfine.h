@@ -384,13 +384,13 @@ __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict
                                                   R* __restrict__ r2, const C* __restrict__ tx,
                                                   const C* __restrict__ ty, const C* __restrict__ tz,
                                                   const R* __restrict__ invD, R* __restrict__ y,
-                                                  double* __restrict__ partials)
+                                                  double* __restrict__ partials, double* __restrict__ hpart)
 {
     double acc[2];
     extern __shared__ char smem_raw[];
     t_vupsweep<R, X, C, V, LDSY>(Z, blockIdx.x, res, num, den, w1, a, w, e, e_out, p, r, q, r2, tx, ty, tz, invD, y,
                                  acc, reinterpret_cast<R*>(smem_raw));
-    block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x);
+    block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x, hpart);
 }
 
 // ---------------------------------------------------------------------------

@@ -23,10 +23,12 @@ void set_error(const char* fmt, ...)
     va_end(ap);
 }
 
-int reducer_init(Reducer& r, int slots, int64_t cap)
+int reducer_init(Reducer& r, int slots, int64_t cap, int64_t ring_n)
 {
     r.slots = slots;
     r.cap = cap;
+    r.hring_n = ring_n;
+    WPF_HIP(hipHostMalloc((void**)&r.hring, sizeof(double) * 2 * (size_t)ring_n, hipHostMallocCoherent));
     WPF_HIP(hipMalloc((void**)&r.partials, sizeof(double) * (size_t)cap));
     WPF_HIP(hipMalloc((void**)&r.counter, sizeof(unsigned) * 4));
     WPF_HIP(hipMemset(r.counter, 0, sizeof(unsigned) * 4));
@@ -36,6 +38,28 @@ int reducer_init(Reducer& r, int slots, int64_t cap)
     return WPF_OK;
 }
 
+int64_t vector_block_capacity(const Grid& g) { return ((g.plane + 511) / 512) * g.nz; }
+
+double finalize_on_host(const double* partials, int64_t nb)
+{
+    double total = 0.0;
+    for (int w = 0; w < 16; w++) {
+        double v[64];
+        for (int l = 0; l < 64; l++) {
+            double s = 0.0;
+            for (int64_t b = 64 * w + l; b < nb; b += 1024) s += partials[b];
+            v[l] = s;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            double u[64];
+            for (int l = 0; l < 64; l++) u[l] = v[l] + v[l ^ o];
+            for (int l = 0; l < 64; l++) v[l] = u[l];
+        }
+        total += v[0];
+    }
+    return total;
+}
+
 int64_t reduction_capacity(const Grid& g)
 {
     // block counts: 64 x 4 x 1 blocks of the per-procedure stencils (the
@@ -43,7 +67,7 @@ int64_t reduction_capacity(const Grid& g)
     // z-march kernels (16 bytes per thread, so at least 2 cells, and at least
     // one plane per tile), and the capped 1-D grids
     const int64_t b64x4 = (int64_t)((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz;
-    const int64_t bvec = ((g.plane + 511) / 512) * g.nz;
+    const int64_t bvec = vector_block_capacity(g);
     const int64_t nb = std::max<int64_t>({(int64_t)2048, b64x4, bvec});
     return nb * kMaxValuesPerBlock;
 }
@@ -54,6 +78,7 @@ void reducer_free(Reducer& r)
     if (r.counter) (void)hipFree(r.counter);
     if (r.result) (void)hipFree(r.result);
     if (r.host) (void)hipHostFree(r.host);
+    if (r.hring) (void)hipHostFree(r.hring);
     r = Reducer{};
 }
 
@@ -314,7 +339,7 @@ int wpf_create(const wpf_params* p, const int32_t* lw_haloed, const wpf_well* we
         wpf_destroy(c);
         return rc;
     }
-    if ((rc = reducer_init(c->red, 64, reduction_capacity(g)))) {
+    if ((rc = reducer_init(c->red, 64, reduction_capacity(g), vector_block_capacity(g)))) {
         wpf_destroy(c);
         return rc;
     }

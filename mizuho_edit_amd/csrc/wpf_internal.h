@@ -92,6 +92,11 @@ struct Reducer {
     double* host = nullptr;      // pinned mirror of result
     int64_t cap = 0;             // doubles in partials
     int slots = 0;               // result slots
+    // pinned (host-coherent) ring of two rows of per-block partials that a
+    // kernel writes itself (the fused CG update's ||invD r||^2 under
+    // lookahead), so the host sums them without a finalize launch or a copy
+    double* hring = nullptr;
+    int64_t hring_n = 0;         // doubles per row
 };
 inline bool reduction_fits(const Reducer& r, int64_t nb, int nv) { return nb * (int64_t)nv <= r.cap; }
 // The most values one block of any kernel stores (the projection's Gram pass:
@@ -303,7 +308,14 @@ void dev_free(Dev<T>& a)
     a.raw = a.p = nullptr;
 }
 
-int reducer_init(Reducer& r, int slots, int64_t cap);
+int reducer_init(Reducer& r, int slots, int64_t cap, int64_t ring_n);
+// Blocks of the largest vector z-march launch of a grid (16 bytes per
+// thread, so at least 2 cells; at least one plane per tile).
+int64_t vector_block_capacity(const Grid& g);
+// t_finalize's sum of nb partials, on the host in the same order (1024
+// strided lanes, each wave's butterfly, then the 16 wave sums in turn), so
+// the value is bitwise the device's.
+double finalize_on_host(const double* partials, int64_t nb);
 // Device address of the no-allocation entry's scratch array on device dev
 // (that device current); once per device, kept for the process.
 int noalloc_scratch_init(int dev);
@@ -358,9 +370,12 @@ __device__ __forceinline__ double dkey_inv(unsigned long long k)
 // result[v].  No device-scope fence is issued here: on a multi-XCD part such
 // a fence writes back the whole L2 of the issuing XCD, once per block.
 // blockDim.x*y*z must be a multiple of 64 and at most 1024.
+//
+// host0 (optional): the block's first value is also stored at host0[bid]
+// (pinned host memory that the host sums with finalize_on_host).
 template <int NV>
 __device__ void block_reduce_store(double (&v)[NV], double* partials, unsigned* counter,
-                                   double* result, int nblocks_total, int bid)
+                                   double* result, int nblocks_total, int bid, double* host0 = nullptr)
 {
     (void)counter;
     (void)result;
@@ -378,6 +393,7 @@ __device__ void block_reduce_store(double (&v)[NV], double* partials, unsigned* 
         double s = 0.0;
         for (int w = 0; w < nw; w++) s += sh[tid][w];
         partials[(size_t)tid * nblocks_total + bid] = s;
+        if (tid == 0 && host0) host0[bid] = s;
     }
     // a block that goes on to another tile must not overwrite sh before
     // these reads are done

@@ -1003,7 +1003,9 @@ int wpf_ensemble_create(wpf_ctx* const* members, int32_t n, const wpf_solver_opt
     size_t lds = 0;
     for (int m = 0; m < n; m++) {
         wpf_ctx* c = members[m];
-        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(WPF_ERR_HIP);
+        // the member's own last solve may have left its solution change for
+        // its next solve to store: the ensemble takes over the ring now
+        if (flush_push(c) != WPF_OK || hipStreamSynchronize(c->stream) != hipSuccess) return fail(WPF_ERR_HIP);
         int rc = solver_workspace(c, &o);
         if (rc) return fail(rc);
         if (e->op16 && !c->h_tx.raw) {

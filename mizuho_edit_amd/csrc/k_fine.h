@@ -404,17 +404,22 @@ __global__ void __launch_bounds__(256) k_update(int64_t n, const double* __restr
 
 // Initial guess from the two previous solutions: save the current pressure,
 // then p = p + ratio (p - pprev) on free cells when a previous one exists.
+// vpush (optional): the last solve's change of the solution, (float)(p -
+// pprev) on free cells and 0 on fixed ones, for the projection ring.
 static __global__ void __launch_bounds__(256) k_extrapolate(int64_t n, int valid, double ratio,
                                                      const uint8_t* __restrict__ mask,
                                                      double* __restrict__ p,
                                                      const double* __restrict__ pprev,
-                                                     double* __restrict__ psave)
+                                                     double* __restrict__ psave,
+                                                     float* __restrict__ vpush)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const double v = p[c];
     psave[c] = v;
-    if (valid && !(mask[c] & WPF_MASK_FIXED)) p[c] = v + ratio * (v - pprev[c]);
+    const bool fixed = (mask[c] & WPF_MASK_FIXED) != 0;
+    if (vpush) vpush[c] = fixed ? 0.0f : (float)(v - pprev[c]);
+    if (valid && !fixed) p[c] = v + ratio * (v - pprev[c]);
 }
 
 }  // namespace wpf

@@ -904,6 +904,13 @@ static int ensure_ws(wpf_ctx* c, bool f32)
 // the caller takes it from its first true residual (bnorm == nullptr).
 static int prepare(wpf_ctx* c, const wpf_solver_opts& o, double* bnorm)
 {
+    // a pending solution change rides on the extrapolation pass below; the
+    // fixed cells that k_set_dirichlet writes store 0 either way
+    const bool push = c->push_pending && (o.flags & (WPF_FLAG_EXTRAPOLATE | WPF_FLAG_PROJECT));
+    if (!push) {
+        int rc = flush_push(c);
+        if (rc) return rc;
+    }
     if (c->ndir > 0) {
         hipLaunchKernelGGL(k_set_dirichlet, dim3((c->ndir + 255) / 256), dim3(256), 0, c->stream,
                            c->ndir, c->dir_idx, c->dir_val, c->p.p);
@@ -919,8 +926,13 @@ static int prepare(wpf_ctx* c, const wpf_solver_opts& o, double* bnorm)
         }
         hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)((c->g.n + 255) / 256)), dim3(256), 0,
                            c->stream, c->g.n, valid, ratio, (const uint8_t*)c->mask, c->p.p,
-                           (const double*)c->xprev.p, c->psave.p);
+                           (const double*)c->xprev.p, c->psave.p, push ? c->proj[c->proj_next].p : nullptr);
         WPF_HIP(hipGetLastError());
+        if (push) {
+            c->push_pending = false;
+            c->proj_next = (c->proj_next + 1) % wpf_ctx::kProjDepth;
+            c->proj_count = std::min(c->proj_count + 1, (int)wpf_ctx::kProjDepth);
+        }
         c->extrapolated = valid != 0;
     } else {
         c->extrapolated = false;
@@ -1163,6 +1175,21 @@ static int project(wpf_ctx* c, const Fine<float, C>& F, double* x, double* rn)
     }
 }
 
+int flush_push(wpf_ctx* c)
+{
+    if (!c->push_pending) return WPF_OK;
+    c->push_pending = false;
+    if (!c->proj[c->proj_next].raw || !c->xprev.raw) return WPF_OK;
+    const int64_t n = c->g.n;
+    hipLaunchKernelGGL(k_proj_push, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       (const double*)c->p.p, (const double*)c->xprev.p, (const uint8_t*)c->mask,
+                       c->proj[c->proj_next].p);
+    WPF_HIP(hipGetLastError());
+    c->proj_next = (c->proj_next + 1) % wpf_ctx::kProjDepth;
+    c->proj_count = std::min(c->proj_count + 1, (int)wpf_ctx::kProjDepth);
+    return WPF_OK;
+}
+
 static void finish(wpf_ctx* c, const wpf_solver_opts& o)
 {
     if ((o.flags & WPF_FLAG_CAPTURE) && c->cap_target) {
@@ -1170,15 +1197,10 @@ static void finish(wpf_ctx* c, const wpf_solver_opts& o)
                            c->g.n, (const double*)c->p.p, (const double*)c->cap_guess, c->cap_target);
         c->have_capture = true;
     }
-    if ((o.flags & WPF_FLAG_PROJECT) && c->proj[c->proj_next].raw) {
-        // this solve's change of the solution joins the projection space
-        const int64_t n = c->g.n;
-        hipLaunchKernelGGL(k_proj_push, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
-                           (const double*)c->p.p, (const double*)c->psave.p, (const uint8_t*)c->mask,
-                           c->proj[c->proj_next].p);
-        c->proj_next = (c->proj_next + 1) % wpf_ctx::kProjDepth;
-        c->proj_count = std::min(c->proj_count + 1, (int)wpf_ctx::kProjDepth);
-    }
+    // this solve's change of the solution, p - psave (psave becomes xprev
+    // below), joins the projection space: the next solve's extrapolation
+    // pass stores it, or flush_push if something else comes first
+    if ((o.flags & WPF_FLAG_PROJECT) && c->proj[c->proj_next].raw) c->push_pending = true;
     if (o.flags & (WPF_FLAG_EXTRAPOLATE | WPF_FLAG_PROJECT)) {
         std::swap(c->xprev, c->psave);  // xprev now holds the previous solution
         c->have_prev = true;
@@ -1284,6 +1306,7 @@ static int mixed(wpf_ctx* c, const wpf_solver_opts& o, wpf_solve_stats* st, bool
         for (int j = 0; j < wpf_ctx::kProjDepth; j++)
             if ((rc = dev_alloc(c->proj[j], c->g))) return rc;
         c->proj_count = c->proj_next = 0;
+        c->push_pending = false;
     }
     c->proj_used = false;
     const bool project_now = (o.flags & WPF_FLAG_PROJECT) && c->proj_count > 0 && c->g.nx % 4 == 0;
@@ -1541,6 +1564,7 @@ void free_solver(wpf_ctx* c)
     dev_free(c->h_tx); dev_free(c->h_ty); dev_free(c->h_tz);
     for (int j = 0; j < wpf_ctx::kProjDepth; j++) dev_free(c->proj[j]);
     c->proj_count = c->proj_next = 0;
+    c->push_pending = false;
     dev_free(c->z2); dev_free(c->pp2); dev_free(c->psave); dev_free(c->fsrc);
 }
 

@@ -411,7 +411,10 @@ int wpf_upload(wpf_ctx* c, int32_t field, const double* h)
         }
     WPF_HIP(hipMemcpyAsync(d, tmp.data(), sizeof(double) * g.n, hipMemcpyHostToDevice, c->stream));
     WPF_HIP(hipStreamSynchronize(c->stream));
-    if (field == WPF_FIELD_PRESSURE) c->proj_count = 0;  // earlier solution changes no longer apply
+    if (field == WPF_FIELD_PRESSURE) {
+        c->proj_count = 0;  // earlier solution changes no longer apply
+        c->push_pending = false;
+    }
     if (field == WPF_FIELD_PERMEAB_ROCK) {
         double m = 0.0;
         for (double v : tmp) m = std::max(m, v);

@@ -162,6 +162,10 @@ struct wpf_ctx {
     wpf::Dev<float> proj[kProjDepth];
     int proj_count = 0, proj_next = 0;
     bool proj_used = false;
+    // the last solve's change of the solution, p - xprev, is still to be
+    // stored in slot proj_next: the next solve's extrapolation pass stores it
+    // (it reads both anyway), and flush_push before anything else changes p
+    bool push_pending = false;
     wpf::Dev<double> rb;                      // second FP64 residual (fused update)
     wpf::Dev<_Float16> h_tx, h_ty, h_tz;      // FP16 fine faces of the inner solve
     double krock_max = 0.0;                   // largest uploaded rock permeability
@@ -259,6 +263,8 @@ int launch_phase_deriv(wpf_ctx* c, double* taumin);
 int launch_concent_deriv(wpf_ctx* c);
 int launch_evolve(wpf_ctx* c, int which, double dtime);
 int launch_step_fields(wpf_ctx* c, int itime, double dtime_host, bool device_dt, bool f32);
+// Store a pending solution change into the projection ring now.
+int flush_push(wpf_ctx* c);
 int launch_export(wpf_ctx* c, int field, double* host);
 int launch_import(wpf_ctx* c, int field, const double* host);
 int launch_diag(wpf_ctx* c, wpf_diag* d);

@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(256) ke_vsweep(ENS_ARGS, int rb, int rx, int r
     ens_faces<C>(M, tx, ty, tz);
     const Pro<float> P{M.nlev > 1 ? M.lv[1].a[EL_X] : nullptr, M.nlev > 1 ? M.lv[1].nx : 0,
                        M.nlev > 1 ? M.lv[1].plane : 0, cs};
-    double acc[1] = {t_vsweep<float, C, 4, DOT, PROX, PROO>(M.Z, T.b, tx, ty, tz, M.invD32, M.mask, a, w,
+    double acc[1] = {t_vsweep<float, C, 4, DOT, PROX, PROO>(M.Z, T.b, tx, ty, tz, M.mask, a, w,
                                                            M.v[rb], M.v[rx], rxo >= 0 ? M.v[rxo] : nullptr,
                                                            P, M.v[ry])};
     if (DOT) block_reduce_store<1>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);

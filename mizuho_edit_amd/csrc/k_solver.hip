@@ -422,7 +422,7 @@ static int sweep(wpf_ctx* c, const Fine<R, C>& F, double a, double w, const R* x
     prof_mark(c, WPF_KERNEL_SMOOTH, true);
     if (F.vec)
         hipLaunchKernelGGL((k_vsweep<R, C, V, DOT, PROX, PROO>), dim3(vz_blocks(F.V)), dim3(256), 0,
-                           c->stream, F.V, F.tx, F.ty, F.tz, F.invD, (const uint8_t*)c->mask, (R)a, (R)w,
+                           c->stream, F.V, F.tx, F.ty, F.tz, (const uint8_t*)c->mask, (R)a, (R)w,
                            (const R*)F.r, x, xo, P, y, RED(slot));
     else
         hipLaunchKernelGGL((k_fsweep<R, C, DOT, PROX, PROO>), dim3(zg_blocks(F.G)), dim3(kFX, kFY), 0,

@@ -159,10 +159,16 @@ __device__ __forceinline__ R xone(const R* __restrict__ x, const uint8_t* __rest
 // a, xo, PROX, PROO and DOT.
 // Block bi of the grid's vz_blocks(Z) tiles; returns this thread's (b, y)
 // when DOT.
+//
+// invD is not read: the sweep forms D from the faces it loads for the
+// stencil, in the order the permeability pass and the operator conversions
+// sum them (+x, -x, +y, -y, +z, -z, from the same stored faces), and takes
+// 1/D with 0 on fixed cells and where D is 0, which is the stored invD bit
+// for bit; the 1-byte mask replaces a 4- or 8-byte read.
 template <typename R, typename C, int V, bool DOT, bool PROX, bool PROO>
 __device__ __forceinline__ double t_vsweep(const VZ& Z, int bi, const C* __restrict__ tx,
                                            const C* __restrict__ ty, const C* __restrict__ tz,
-                                           const R* __restrict__ invD, const uint8_t* __restrict__ mask,
+                                           const uint8_t* __restrict__ mask,
                                            R a, R w, const R* __restrict__ b, const R* __restrict__ x,
                                            const R* xo, const Pro<R>& P, R* y)
 {
@@ -180,14 +186,15 @@ __device__ __forceinline__ double t_vsweep(const VZ& Z, int bi, const C* __restr
     vldc<V>(tz + c - pl, tzm);
     for (int k = k0; k < k1; k++, c += pl) {
         xrow<V, PROX>(x, mask, P, c + pl, i, j, k + 1, xp);
-        R xyp[V], xym[V], txv[V], tyv[V], tym[V], tzv[V], di[V], bv[V];
+        R xyp[V], xym[V], txv[V], tyv[V], tym[V], tzv[V], bv[V];
+        uint8_t mk[V];
         xrow<V, PROX>(x, mask, P, c + nx, i, j + 1, k, xyp);
         xrow<V, PROX>(x, mask, P, c - nx, i, j - 1, k, xym);
         vldc<V>(tx + c, txv);
         vldc<V>(ty + c, tyv);
         vldc<V>(ty + c - nx, tym);
         vldc<V>(tz + c, tzv);
-        vld<V>(invD + c, di);
+        vld<V>(mask + c, mk);
         vld<V>(b + c, bv);
         R xl = from_left(x0[V - 1]), xr = from_right(x0[0]), txl = from_left(txv[V - 1]);
         if (lane == 0) {
@@ -214,9 +221,10 @@ __device__ __forceinline__ double t_vsweep(const VZ& Z, int bi, const C* __restr
             const R xle = e ? x0[e - 1] : xl;
             const R xre = (e + 1 < V) ? x0[e + 1] : xr;
             const R d = txv[e] + txm + tyv[e] + tym[e] + tzv[e] + tzm[e];
+            const R di = ((mk[e] & WPF_MASK_FIXED) || d == (R)0) ? (R)0 : (R)1 / d;
             const R s = txv[e] * xre + txm * xle + tyv[e] * xyp[e] + tym[e] * xym[e] + tzv[e] * xp[e] +
                         tzm[e] * xm[e];
-            R v = x0[e] + w * di[e] * (bv[e] - (d * x0[e] - s));
+            R v = x0[e] + w * di * (bv[e] - (d * x0[e] - s));
             if (a != (R)0) v += a * (x0[e] - ov[e]);
             yv[e] = v;
             if (DOT) acc[0] += (double)bv[e] * (double)v;
@@ -234,7 +242,7 @@ __device__ __forceinline__ double t_vsweep(const VZ& Z, int bi, const C* __restr
 
 template <typename R, typename C, int V, bool DOT, bool PROX, bool PROO>
 __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, const C* __restrict__ ty,
-                                                const C* __restrict__ tz, const R* __restrict__ invD,
+                                                const C* __restrict__ tz,
                                                 const uint8_t* __restrict__ mask, R a, R w,
                                                 const R* __restrict__ b, const R* __restrict__ x,
                                                 const R* xo, Pro<R> P, R* y,
@@ -242,8 +250,8 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
                                                 unsigned* __restrict__ counter,
                                                 double* __restrict__ result)
 {
-    double acc[1] = {t_vsweep<R, C, V, DOT, PROX, PROO>(Z, blockIdx.x, tx, ty, tz, invD, mask, a, w, b, x,
-                                                        xo, P, y)};
+    double acc[1] = {t_vsweep<R, C, V, DOT, PROX, PROO>(Z, blockIdx.x, tx, ty, tz, mask, a, w, b, x, xo, P,
+                                                        y)};
     if (DOT) block_reduce_store<1>(acc, partials, counter, result, gridDim.x, blockIdx.x);
 }
 

@@ -245,7 +245,9 @@ __device__ __forceinline__ void t_smooth(LV<R> L, R a, R w, const R* __restrict_
         R d, s;
         row_parts<R, FINE>(L, x, c, d, s);
         const R xv = x[c];
-        R yv = xv + w * L.invD[c] * (b[c] - (d * xv - s));
+        // coarse levels: 1/D as the Galerkin pass stores it, not read
+        const R di = FINE ? L.invD[c] : (d > (R)0 ? (R)1 / d : (R)0);
+        R yv = xv + w * di * (b[c] - (d * xv - s));
         if (a != (R)0) yv += a * (xv - (xo ? xo[c] : (R)0));
         y[c] = yv;
     }
@@ -354,7 +356,10 @@ __device__ __forceinline__ void t_smooth_pro(LV<R> L, int cnx, int cny, R cs, co
                     tym * X(c - L.nx, i, j - 1, k) + tzp * X(c + L.plane, i, j, k + 1) +
                     tzm * X(c - L.plane, i, j, k - 1);
         const R xv = X(c, i, j, k);
-        R yv = xv + w * L.invD[c] * (b[c] - (L.D[c] * xv - s));
+        // 1/D as the Galerkin pass stores it, not read
+        const R dc = L.D[c];
+        const R di = dc > (R)0 ? (R)1 / dc : (R)0;
+        R yv = xv + w * di * (b[c] - (dc * xv - s));
         if (a != (R)0) {
             R ov = xo ? xo[c] : (R)0;
             if (PROO) ov = ov + cs * xc[par(i, j, k)];

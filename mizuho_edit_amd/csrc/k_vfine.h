@@ -840,7 +840,7 @@ __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<doub
     if (bn) vld<V>(L.mask + c - pl, mzm);
     for (int k = k0; k < k1; k++, c += pl) {
         vld<V>(x + c + pl, xp);
-        double xyp[V], xym[V], txv[V], tyv[V], tym[V], tzv[V], di[V], fv[V];
+        double xyp[V], xym[V], txv[V], tyv[V], tym[V], tzv[V], fv[V];
         uint8_t m[V];
         vld<V>(x + c + nx, xyp);
         vld<V>(x + c - nx, xym);
@@ -848,7 +848,6 @@ __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<doub
         vld<V>(L.ty + c, tyv);
         vld<V>(L.ty + c - nx, tym);
         vld<V>(L.tz + c, tzv);
-        vld<V>(L.invD + c, di);
         vld<V>(L.mask + c, m);
         if (f) {
             vld<V>(f + c, fv);
@@ -881,11 +880,14 @@ __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<doub
             const double xle = h ? x0[h - 1] : xl;
             const double xre = (h + 1 < V) ? x0[h + 1] : xr;
             const double d = txv[h] + txm + tyv[h] + tym[h] + tzv[h] + tzm[h];
+            // invD as the permeability pass stores it (the same sum of the
+            // same faces), not read
+            const double di = ((m[h] & WPF_MASK_FIXED) || d == 0.0) ? 0.0 : 1.0 / d;
             const double s = txv[h] * xre + txm * xle + tyv[h] * xyp[h] + tym[h] * xym[h] + tzv[h] * xp[h] +
                              tzm[h] * xm[h];
             double rv = fv[h] - (d * x0[h] - s);
             if (m[h] & WPF_MASK_FIXED) rv = 0.0;
-            const double zv = rv * di[h];
+            const double zv = rv * di;
             if (act) acc[0] += zv * zv;
             rv32[h] = (float)(rv * s32);
             if (bn && act) {
@@ -906,7 +908,7 @@ __device__ __forceinline__ void t_vtrue_resid(const VZ& Z, int bi, const LV<doub
                     add(tym[h], mym[h], xym[h]);
                     add(tzv[h], mzp[h], xp[h]);
                     add(tzm[h], mzm[h], xm[h]);
-                    v = (fv[h] + sb) * di[h];
+                    v = (fv[h] + sb) * di;
                 }
                 acc[1] += v * v;
             }

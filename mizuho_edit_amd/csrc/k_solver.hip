@@ -352,6 +352,26 @@ static int upsweep(wpf_ctx* c, const Fine<R, C>& F, int num, int den, R w1, doub
     return WPF_OK;
 }
 
+// The first V-cycle's first two pre-smoothing steps in one pass (k_vupsweep
+// without the update): y (into F.T) from z0 = w1 invD r, the values of
+// k_smooth0 followed by the step-2 k_vsweep.
+template <typename R, typename C>
+static int presweep(wpf_ctx* c, const Fine<R, C>& F, R w1, double a1, double w1s)
+{
+    constexpr int V = vwidth<R>();
+    const bool lds = vupsweep_lds_fits<R, V>(F.g.nx);
+    const size_t lds_bytes = lds ? vupsweep_lds<R, V>(F.g.nx) : 0;
+    auto* const kern = lds ? k_vupsweep<R, R, C, V, true, false> : k_vupsweep<R, R, C, V, false, false>;
+    prof_mark(c, WPF_KERNEL_SMOOTH, true);
+    hipLaunchKernelGGL(kern, dim3(vz_blocks(F.V)), dim3(256), lds_bytes, c->stream, F.V,
+                       (const double*)c->red.result, 0, 0, w1, (R)a1, (R)w1s, (const R*)nullptr, (R*)nullptr,
+                       (const R*)nullptr, (const R*)F.r, (const R*)nullptr, (R*)nullptr, F.tx, F.ty, F.tz, F.invD,
+                       F.T, c->red.partials, (double*)nullptr);
+    prof_mark(c, WPF_KERNEL_SMOOTH, false);
+    WPF_HIP(hipGetLastError());
+    return WPF_OK;
+}
+
 static int default_zc(const Grid& g) { return g.nz >= 64 ? 8 : 4; }
 
 template <typename R, typename C>
@@ -560,11 +580,18 @@ static int pcg_core(wpf_ctx* c, std::vector<MGLevel<R>>* mg, const Fine<R, C>& F
     const R w = mg ? (R)o.w1 : (R)1;
     int rz_cur = S_RZ0, rz_prev = S_RZ1;
     R* zres = F.Z;
-    LAUNCH1D(k_smooth0<R>, n, n, w, (const R*)F.r, F.invD, F.Z);
-    if (mg) {
+    if (fuse) {
+        // the first two pre-smoothing steps in one pass
+        double a1, w1s;
+        step_coef(o, 1, &a1, &w1s);
+        if (int rc = presweep<R, C>(c, F, w, a1, w1s)) return rc;
+        if (int rc = vcycle_fine(c, *mg, F, o, rz_cur, &zres, true)) return rc;
+    } else if (mg) {
+        LAUNCH1D(k_smooth0<R>, n, n, w, (const R*)F.r, F.invD, F.Z);
         int rc = vcycle_fine(c, *mg, F, o, rz_cur, &zres);
         if (rc) return rc;
     } else {
+        LAUNCH1D(k_smooth0<R>, n, n, w, (const R*)F.r, F.invD, F.Z);
         LAUNCH1D(k_dot<R>, n, n, (const R*)F.r, (const R*)F.Z, RED(rz_cur));
         FINALIZE(blocks_for(n), 1, rz_cur);
     }

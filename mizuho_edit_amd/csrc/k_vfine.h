@@ -270,7 +270,12 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
 // staged once in zs (two buffers of 256*V + 2*nx values, one barrier per
 // plane); the neighbours' z0 are the values the owning threads computed, so
 // the result is the same.
-template <typename R, typename X, typename C, int V, bool LDSY = false>
+//
+// UPD = false: no CG update (e, p, q and r2 are not touched, nothing is
+// reduced): z0 = w1 invD r as k_smooth0 forms it, and y, the second
+// pre-smoothing step, as k_vsweep takes it from that z0, so y is the value
+// of k_smooth0 followed by k_vsweep (the first V-cycle of a solve).
+template <typename R, typename X, typename C, int V, bool LDSY = false, bool UPD = true>
 __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __restrict__ res, int num,
                                            int den, R w1, R a, R w, const X* e, X* e_out,
                                            const R* __restrict__ p, const R* __restrict__ r,
@@ -279,7 +284,7 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
                                            const C* __restrict__ tz, const R* __restrict__ invD,
                                            R* __restrict__ y, double (&acc)[2], R* zs = nullptr)
 {
-    const R al = (R)(res[num] / res[den]);
+    const R al = UPD ? (R)(res[num] / res[den]) : (R)0;
     int64_t qi;
     int k0, k1;
     const bool act = vz_at<V>(Z, bi, qi, k0, k1);
@@ -292,15 +297,22 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
     auto rz = [&](int64_t m, R(&rn)[V], R(&z0)[V], R(&dv)[V]) {
         R rv[V], qv[V];
         vld<V>(r + m, rv);
-        vld<V>(q + m, qv);
+        if (UPD) vld<V>(q + m, qv);
         vld<V>(invD + m, dv);
 #pragma unroll
         for (int h = 0; h < V; h++) {
-            rn[h] = rv[h] - al * qv[h];
-            z0[h] = w1 * (rn[h] * dv[h]);
+            if (UPD) {
+                rn[h] = rv[h] - al * qv[h];
+                z0[h] = w1 * (rn[h] * dv[h]);
+            } else {
+                rn[h] = rv[h];
+                z0[h] = w1 * dv[h] * rv[h];  // k_smooth0's product order
+            }
         }
     };
-    auto z1 = [&](int64_t m) { return w1 * ((r[m] - al * q[m]) * invD[m]); };
+    auto z1 = [&](int64_t m) {
+        return UPD ? w1 * ((r[m] - al * q[m]) * invD[m]) : w1 * invD[m] * r[m];
+    };
     R rm[V], r0[V], rp[V], zm[V], z0[V], zp[V], tzm[V], dm[V], di[V], dp[V];
     int64_t c = qi + pl * k0;
     rz(c - pl, rm, zm, dm);
@@ -313,7 +325,7 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
         vldc<V>(ty + c, tyv);
         vldc<V>(ty + c - nx, tym);
         vldc<V>(tz + c, tzv);
-        vld<V>(p + c, pv);
+        if (UPD) vld<V>(p + c, pv);
         R zl = from_left(z0[V - 1]), zr = from_right(z0[0]), txl = from_left(txv[V - 1]);
         if (lane == 0) txl = (R)tx[c - 1];
         if (LDSY) {
@@ -347,14 +359,16 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
             if (lane == 63) zr = z1(c + V);
         }
         X ev[V];
-        vld<V>(e + c, ev);
+        if (UPD) vld<V>(e + c, ev);
         R yv[V];
 #pragma unroll
         for (int h = 0; h < V; h++) {
-            ev[h] = ev[h] + (X)(al * pv[h]);
-            const R zrh = r0[h] * di[h];
-            acc[0] += (double)zrh * (double)zrh;
-            acc[1] += (double)r0[h] * (double)(w1 * zrh);
+            if (UPD) {
+                ev[h] = ev[h] + (X)(al * pv[h]);
+                const R zrh = r0[h] * di[h];
+                acc[0] += (double)zrh * (double)zrh;
+                acc[1] += (double)r0[h] * (double)(w1 * zrh);
+            }
             const R txm = h ? txv[h - 1] : txl;
             const R zle = h ? z0[h - 1] : zl;
             const R zre = (h + 1 < V) ? z0[h + 1] : zr;
@@ -366,8 +380,10 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
             yv[h] = v;
         }
         if (act) {
-            vst<V>(e_out + c, ev);
-            vst<V>(r2 + c, r0);
+            if (UPD) {
+                vst<V>(e_out + c, ev);
+                vst<V>(r2 + c, r0);
+            }
             vst<V>(y + c, yv);
         }
 #pragma unroll
@@ -385,7 +401,7 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
     if (!act) acc[0] = acc[1] = 0.0;
 }
 
-template <typename R, typename X, typename C, int V, bool LDSY>
+template <typename R, typename X, typename C, int V, bool LDSY, bool UPD = true>
 __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict__ res, int num, int den, R w1,
                                                   R a, R w, const X* e, X* e_out, const R* __restrict__ p,
                                                   const R* __restrict__ r, const R* __restrict__ q,
@@ -396,9 +412,9 @@ __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict
 {
     double acc[2];
     extern __shared__ char smem_raw[];
-    t_vupsweep<R, X, C, V, LDSY>(Z, blockIdx.x, res, num, den, w1, a, w, e, e_out, p, r, q, r2, tx, ty, tz, invD, y,
-                                 acc, reinterpret_cast<R*>(smem_raw));
-    block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x, hpart);
+    t_vupsweep<R, X, C, V, LDSY, UPD>(Z, blockIdx.x, res, num, den, w1, a, w, e, e_out, p, r, q, r2, tx, ty, tz,
+                                      invD, y, acc, reinterpret_cast<R*>(smem_raw));
+    if (UPD) block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x, hpart);
 }
 
 // ---------------------------------------------------------------------------

@@ -79,7 +79,7 @@ def bytes_per_cell(kernel: str, r: int, h: int, cheb: bool = True, launches: int
         # e, p, r, q, invD, 3 faces -> e, r', y
         "cg_update": 8 * r + 3 * h,
         "restrict": 2 * r + 3 * h + 1 + r / 8,
-        "true_resid": 6 * 8 + 1 + r,  # x, f, t x3, invD (FP64), mask -> r (precision r)
+        "true_resid": 5 * 8 + 1 + r,  # x, f, t x3 (FP64), mask -> r (precision r); 1/D from the faces
         "step_fields": 6 * 8 + 1 + 2 * 8,  # phi, c, p, t x3, mask -> phi', c'
         "perm": 2 * 8 + 1 + 4 * 8,  # phi, krock, mask -> tx, ty, tz, invD
     }[kernel]
@@ -99,9 +99,12 @@ WRITE_COUNTERS = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
 # changes, r and the FP32 faces of their stencil products; projection update
 # = x (FP64), r, 4 changes, FP32 faces, invD, mask -> x, r)
 TRAFFIC_KERNELS = {
-    "k_vupsweepIffDF16": ("cg_update", bytes_per_cell("cg_update", 4, 2)),
-    "k_vsweepIfDF16_Li4ELb1ELb0ELb1E": ("smooth_post2", 5 * 4 + 3 * 2 + 1 + 4 / 8),
-    "k_vsweepIfDF16_Li4ELb0ELb1ELb0E": ("smooth_post1", 4 * 4 + 3 * 2 + 1 + 4 / 8),
+    # the update form only (LDS-staged, UPD); the first V-cycle's form
+    # without the update is another instance
+    "k_vupsweepIffDF16_Li4ELb1ELb1E": ("cg_update", bytes_per_cell("cg_update", 4, 2)),
+    # the sweeps form 1/D from the faces: b, x, xo, 3 faces, mask, xc -> y
+    "k_vsweepIfDF16_Li4ELb1ELb0ELb1E": ("smooth_post2", 4 * 4 + 3 * 2 + 1 + 4 / 8),
+    "k_vsweepIfDF16_Li4ELb0ELb1ELb0E": ("smooth_post1", 3 * 4 + 3 * 2 + 1 + 4 / 8),
     "k_vdir_mv<float, float, 4>": ("matvec", bytes_per_cell("matvec", 4, 2)),
     "k_vfresid_restrictIfDF16": ("restrict", bytes_per_cell("restrict", 4, 2)),
     "k_step_fields<double>": ("step_fields", bytes_per_cell("step_fields", 4, 2)),
@@ -109,8 +112,10 @@ TRAFFIC_KERNELS = {
     # and an FP32 invD
     "k_perm<2>": ("perm", bytes_per_cell("perm", 4, 2) + 3 * 4 + 3 * 2 + 4),
     "k_vtrue_resid<2>": ("true_resid", bytes_per_cell("true_resid", 4, 2)),
-    "k_vgram<float, float, 4, 4>": ("gram", 33.0),
-    "k_vproj_update<float, float, 4, 4>": ("proj_update", 57.0),
+    # the 5-deep projection ring: 5 vectors, r, 3 FP32 faces; x (FP64) in
+    # and out, 5 vectors, 3 faces, r in and out, the updated r's invD, mask
+    "k_vgram<float, float, 4, 5>": ("gram", 4 * 5 + 4 + 3 * 4),
+    "k_vproj_update<float, float, 4, 5": ("proj_update", 8 * 2 + 4 * 5 + 3 * 4 + 4 * 2 + 4 + 1),
     "k_coarsen_pairsIfDF16": ("coarsen_pairs", 9.5),
 }
 

@@ -76,8 +76,8 @@ def bytes_per_cell(kernel: str, r: int, h: int, cheb: bool = True, launches: int
         # preconditioner faces, h = 2, the matvec still reads FP32 faces)
         "matvec": 4 * r + 3 * max(h, r) + 1,
         # CG update fused with the first pre-smoothing step:
-        # e, p, r, q, invD, 3 faces -> e, r', y
-        "cg_update": 8 * r + 3 * h,
+        # e, p, r, q, 3 faces, mask -> e, r', y (1/D formed from the faces)
+        "cg_update": 7 * r + 3 * h + 1,
         "restrict": 2 * r + 3 * h + 1 + r / 8,
         "true_resid": 5 * 8 + 1 + r,  # x, f, t x3 (FP64), mask -> r (precision r); 1/D from the faces
         "step_fields": 6 * 8 + 1 + 2 * 8,  # phi, c, p, t x3, mask -> phi', c'

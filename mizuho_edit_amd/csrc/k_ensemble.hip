@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) ke_upsweep(ENS_ARGS, int rp, int rr, int 
     ens_faces<C>(M, tx, ty, tz);
     double acc[2];
     t_vupsweep<float, float, C, 4, LDSY>(M.Z, T.b, M.result, num, S_PQ, w1, a, w, M.v[EV_E], M.v[EV_E], M.v[rp],
-                                         M.v[rr], M.v[EV_Q], M.v[rr2], tx, ty, tz, M.invD32, M.v[EV_T], acc,
+                                         M.v[rr], M.v[EV_Q], M.v[rr2], tx, ty, tz, M.invD32, M.mask, M.v[EV_T], acc,
                                          reinterpret_cast<float*>(smem_raw));
     block_reduce_store<2>(acc, M.partials, nullptr, nullptr, M.nbv, T.b);
     ENS_END

@@ -346,7 +346,8 @@ static int upsweep(wpf_ctx* c, const Fine<R, C>& F, int num, int den, R w1, doub
     prof_mark(c, WPF_KERNEL_CG_UPDATE, true);
     hipLaunchKernelGGL(kern, dim3(vz_blocks(F.V)), dim3(256), lds_bytes, c->stream, F.V,
                        (const double*)c->red.result, num, den, w1, (R)a1, (R)w1s, x, x_out, p, (const R*)F.r,
-                       (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, F.T, c->red.partials, hpart);
+                       (const R*)F.q, F.r2, F.tx, F.ty, F.tz, F.invD, (const uint8_t*)c->mask, F.T,
+                       c->red.partials, hpart);
     prof_mark(c, WPF_KERNEL_CG_UPDATE, false);
     WPF_HIP(hipGetLastError());
     return WPF_OK;
@@ -366,7 +367,7 @@ static int presweep(wpf_ctx* c, const Fine<R, C>& F, R w1, double a1, double w1s
     hipLaunchKernelGGL(kern, dim3(vz_blocks(F.V)), dim3(256), lds_bytes, c->stream, F.V,
                        (const double*)c->red.result, 0, 0, w1, (R)a1, (R)w1s, (const R*)nullptr, (R*)nullptr,
                        (const R*)nullptr, (const R*)F.r, (const R*)nullptr, (R*)nullptr, F.tx, F.ty, F.tz, F.invD,
-                       F.T, c->red.partials, (double*)nullptr);
+                       (const uint8_t*)c->mask, F.T, c->red.partials, (double*)nullptr);
     prof_mark(c, WPF_KERNEL_SMOOTH, false);
     WPF_HIP(hipGetLastError());
     return WPF_OK;

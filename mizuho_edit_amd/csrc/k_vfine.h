@@ -275,6 +275,12 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
 // reduced): z0 = w1 invD r as k_smooth0 forms it, and y, the second
 // pre-smoothing step, as k_vsweep takes it from that z0, so y is the value
 // of k_smooth0 followed by k_vsweep (the first V-cycle of a solve).
+//
+// The thread's own cells do not read invD: the faces of the next plane are
+// loaded one plane ahead, and 1/D (0 on fixed cells and where D is 0) is
+// formed from them in the order the permeability pass sums them, the stored
+// invD bit for bit.  The chunk's border rows and the cells beyond a
+// wavefront's ends still read invD.
 template <typename R, typename X, typename C, int V, bool LDSY = false, bool UPD = true>
 __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __restrict__ res, int num,
                                            int den, R w1, R a, R w, const X* e, X* e_out,
@@ -282,6 +288,7 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
                                            const R* __restrict__ q, R* __restrict__ r2,
                                            const C* __restrict__ tx, const C* __restrict__ ty,
                                            const C* __restrict__ tz, const R* __restrict__ invD,
+                                           const uint8_t* __restrict__ mask,
                                            R* __restrict__ y, double (&acc)[2], R* zs = nullptr)
 {
     const R al = UPD ? (R)(res[num] / res[den]) : (R)0;
@@ -294,11 +301,12 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
     const int zw = 256 * V + 2 * Z.nx, nxv = Z.nx / V;  // LDS values per plane, vectors per row
     acc[0] = acc[1] = 0.0;
     // r' and z0 of the V cells at m (and their invD)
-    auto rz = [&](int64_t m, R(&rn)[V], R(&z0)[V], R(&dv)[V]) {
+    // r' and z0 of the V cells at m, with their invD in dv (read when LD)
+    auto rzd = [&](int64_t m, R(&rn)[V], R(&z0)[V], R(&dv)[V], bool LD) {
         R rv[V], qv[V];
         vld<V>(r + m, rv);
         if (UPD) vld<V>(q + m, qv);
-        vld<V>(invD + m, dv);
+        if (LD) vld<V>(invD + m, dv);
 #pragma unroll
         for (int h = 0; h < V; h++) {
             if (UPD) {
@@ -310,6 +318,7 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
             }
         }
     };
+    auto rz = [&](int64_t m, R(&rn)[V], R(&z0)[V], R(&dv)[V]) { rzd(m, rn, z0, dv, true); };
     auto z1 = [&](int64_t m) {
         return UPD ? w1 * ((r[m] - al * q[m]) * invD[m]) : w1 * invD[m] * r[m];
     };
@@ -318,16 +327,37 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
     rz(c - pl, rm, zm, dm);
     rz(c, r0, z0, di);
     vldc<V>(tz + c - pl, tzm);
+    // this plane's faces; each plane loads the next one's
+    R txv[V], tyv[V], tym[V], tzv[V], txl;
+    vldc<V>(tx + c, txv);
+    vldc<V>(ty + c, tyv);
+    vldc<V>(ty + c - nx, tym);
+    vldc<V>(tz + c, tzv);
+    txl = from_left(txv[V - 1]);
+    if (lane == 0) txl = (R)tx[c - 1];
     for (int k = k0; k < k1; k++, c += pl) {
-        rz(c + pl, rp, zp, dp);
-        R zyp[V], zym[V], txv[V], tyv[V], tym[V], tzv[V], pv[V];
-        vldc<V>(tx + c, txv);
-        vldc<V>(ty + c, tyv);
-        vldc<V>(ty + c - nx, tym);
-        vldc<V>(tz + c, tzv);
+        // the next plane's faces (the back pad past the grid: zeros), and
+        // its 1/D for its z0
+        R txn[V], tyn[V], tymn[V], tzn[V];
+        uint8_t mn[V];
+        const int64_t cn = c + pl;
+        vldc<V>(tx + cn, txn);
+        vldc<V>(ty + cn, tyn);
+        vldc<V>(ty + cn - nx, tymn);
+        vldc<V>(tz + cn, tzn);
+        vld<V>(mask + cn, mn);
+        R txln = from_left(txn[V - 1]);
+        if (lane == 0) txln = (R)tx[cn - 1];
+#pragma unroll
+        for (int h = 0; h < V; h++) {
+            const R txmn = h ? txn[h - 1] : txln;
+            const R dn = txn[h] + txmn + tyn[h] + tymn[h] + tzn[h] + tzv[h];
+            dp[h] = ((mn[h] & WPF_MASK_FIXED) || dn == (R)0) ? (R)0 : (R)1 / dn;
+        }
+        rzd(cn, rp, zp, dp, false);
+        R zyp[V], zym[V], pv[V];
         if (UPD) vld<V>(p + c, pv);
-        R zl = from_left(z0[V - 1]), zr = from_right(z0[0]), txl = from_left(txv[V - 1]);
-        if (lane == 0) txl = (R)tx[c - 1];
+        R zl = from_left(z0[V - 1]), zr = from_right(z0[0]);
         if (LDSY) {
             R* buf = zs + (k & 1) * zw;
             const int o = (int)nx + (int)threadIdx.x * V;
@@ -394,7 +424,12 @@ __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __
             z0[h] = zp[h];
             di[h] = dp[h];
             tzm[h] = tzv[h];
+            txv[h] = txn[h];
+            tyv[h] = tyn[h];
+            tym[h] = tymn[h];
+            tzv[h] = tzn[h];
         }
+        txl = txln;
     }
     // the buffers are reused by the next tile of a persistent launch
     if (LDSY) __syncthreads();
@@ -407,13 +442,14 @@ __global__ void __launch_bounds__(256) k_vupsweep(VZ Z, const double* __restrict
                                                   const R* __restrict__ r, const R* __restrict__ q,
                                                   R* __restrict__ r2, const C* __restrict__ tx,
                                                   const C* __restrict__ ty, const C* __restrict__ tz,
-                                                  const R* __restrict__ invD, R* __restrict__ y,
-                                                  double* __restrict__ partials, double* __restrict__ hpart)
+                                                  const R* __restrict__ invD, const uint8_t* __restrict__ mask,
+                                                  R* __restrict__ y, double* __restrict__ partials,
+                                                  double* __restrict__ hpart)
 {
     double acc[2];
     extern __shared__ char smem_raw[];
     t_vupsweep<R, X, C, V, LDSY, UPD>(Z, blockIdx.x, res, num, den, w1, a, w, e, e_out, p, r, q, r2, tx, ty, tz,
-                                      invD, y, acc, reinterpret_cast<R*>(smem_raw));
+                                      invD, mask, y, acc, reinterpret_cast<R*>(smem_raw));
     if (UPD) block_reduce_store<2>(acc, partials, nullptr, nullptr, gridDim.x, blockIdx.x, hpart);
 }
 

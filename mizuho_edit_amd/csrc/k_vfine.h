@@ -279,8 +279,8 @@ __global__ void __launch_bounds__(256) k_vsweep(VZ Z, const C* __restrict__ tx, 
 // The thread's own cells do not read invD: the faces of the next plane are
 // loaded one plane ahead, and 1/D (0 on fixed cells and where D is 0) is
 // formed from them in the order the permeability pass sums them, the stored
-// invD bit for bit.  The chunk's border rows and the cells beyond a
-// wavefront's ends still read invD.
+// invD bit for bit.  The chunk's border rows (LDSY), or the y-neighbour rows
+// and the cells beyond a wavefront's ends (without LDSY), still read invD.
 template <typename R, typename X, typename C, int V, bool LDSY = false, bool UPD = true>
 __device__ __forceinline__ void t_vupsweep(const VZ& Z, int bi, const double* __restrict__ res, int num,
                                            int den, R w1, R a, R w, const X* e, X* e_out,
